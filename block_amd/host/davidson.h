@@ -1,0 +1,42 @@
+// Linear::block_davidson restated (linear.C:179-385) behind the same functor
+// boundary as the reference: Davidson_functor / multiply_h (davidson.h:16-29).
+#pragma once
+#include <vector>
+
+#include "superblock.h"
+
+namespace blk {
+
+// v = H c for one vector (v is overwritten; the reference clears sigma and
+// multiply_h accumulates into it, linear.C:239-253).
+struct DavidsonFunctor {
+  virtual ~DavidsonFunctor() = default;
+  virtual void operator()(const double* c, double* v) = 0;
+};
+
+struct DavidsonStats {
+  int iterations = 0;
+  int sigma_calls = 0;
+  double sigma_seconds = 0.0;  // the reference's hmultiply timer (linear.C:220,259)
+};
+
+// Restatement of block_davidson with its control flow: Gram-Schmidt of the
+// guesses, sigma for new vectors only, subspace diagonalisation, Ritz
+// rotation, residual of the first unconverged root, Olsen preconditioner,
+// convergence on the squared residual norm, deflation and re-orthogonalised
+// expansion.  On return b holds the Ritz vectors and h_diag[0..nroots) the
+// eigenvalues.
+void block_davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_diag, double normtol,
+                    DavidsonFunctor& h, bool useprecond, int deflation_min, int deflation_max, DavidsonStats* st,
+                    int max_iter = 1000);
+
+// vector kernels of BaseOperator.C:240-288 / linear.C:27-60 on host vectors
+double ddot(const std::vector<double>& x, const std::vector<double>& y);
+void daxpy(double a, const std::vector<double>& x, std::vector<double>& y);
+void dscal(double a, std::vector<double>& x);
+void normalise(std::vector<double>& x);
+void precondition(std::vector<double>& x, double e, const std::vector<double>& diag, double shift);
+void olsen_precondition(std::vector<double>& r, const std::vector<double>& c0, double e,
+                        const std::vector<double>& diag, double shift);
+
+}  // namespace blk

@@ -1,0 +1,73 @@
+// Work-list compiler and device residency for one superblock: the
+// MI355X-side multiply_h.  Constructing a GpuSigma corresponds to constructing
+// multiply_h(big, onedot) (davidson.h:22-29) for one Davidson solve: every
+// operator block the term list touches is packed into one HBM arena and the
+// term list is flattened into the two-phase GEMM-pair plan of
+// include/block_sigma.h.  apply() is multiply_h::operator() (v += H c).
+#pragma once
+#include <unordered_map>
+
+#include "../../include/block_sigma.h"
+#include "superblock.h"
+
+namespace blk {
+
+// fail loudly: every C-ABI status is checked
+void bsg_check(int rc, const char* what);
+
+struct GpuContext {
+  bsg_context* ctx = nullptr;
+  explicit GpuContext(int device = 0);
+  ~GpuContext();
+  GpuContext(const GpuContext&) = delete;
+  GpuContext& operator=(const GpuContext&) = delete;
+};
+
+struct WorkList {
+  std::vector<std::vector<bsg_gemm1_t>> g1;
+  std::vector<std::vector<bsg_vblock_t>> vb;
+  std::vector<std::vector<bsg_contrib_t>> ct;
+  std::vector<int64_t> wsize;
+  std::vector<bsg_diag_t> diag;
+  std::unordered_map<const SparseOp*, int64_t> arena_off;
+  int64_t arena_size = 0;
+  size_t n_pairs = 0;  // number of (GEMM1, GEMM2) sector pairs
+};
+// Flatten the term list; W scratch per chunk is kept under w_budget doubles.
+WorkList compile_worklist(const SuperBlock& sb, int64_t w_budget);
+
+class GpuSigma {
+ public:
+  GpuSigma(GpuContext& g, const SuperBlock& sb, int64_t w_budget = (int64_t)1 << 27, int max_vec = 1);
+  ~GpuSigma();
+  GpuSigma(const GpuSigma&) = delete;
+  GpuSigma& operator=(const GpuSigma&) = delete;
+
+  long wave_size() const { return wave_size_; }
+  double flops() const { return flops_; }
+  size_t n_pairs() const { return n_pairs_; }
+  size_t arena_bytes() const { return (size_t)arena_size_ * 8; }
+  // host vectors: v = H c (overwrites v)
+  void apply(const double* c_host, double* v_host);
+  // device vectors: v += H c
+  void apply_device(const double* dc, double* dv, int nvec = 1);
+  std::vector<double> diagonal();
+  bsg_context* ctx() const { return g_.ctx; }
+  double* dev_c() const { return d_c_; }
+  double* dev_v() const { return d_v_; }
+  const double* dev_arena() const { return d_arena_; }
+
+ private:
+  GpuContext& g_;
+  long wave_size_ = 0;
+  double flops_ = 0.0;
+  size_t n_pairs_ = 0;
+  int64_t arena_size_ = 0;
+  double* d_arena_ = nullptr;
+  double* d_c_ = nullptr;
+  double* d_v_ = nullptr;
+  bsg_plan* plan_ = nullptr;
+  std::vector<bsg_diag_t> diag_terms_;
+};
+
+}  // namespace blk

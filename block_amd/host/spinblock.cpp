@@ -546,6 +546,56 @@ struct Builder {
 };
 }  // namespace
 
+namespace {
+struct Task {
+  int kind, i, j, K;  // kind 0 cd, 1 cc, 2 cdc, 3 ddc, 4 ccd
+};
+void run_tasks(const Builder& bd, SpinBlock& b, const std::vector<Task>& tasks,
+               const std::function<OpPtr(OpPtr)>& post) {
+  std::string err;
+#pragma omp parallel for schedule(dynamic)
+  for (long t = 0; t < (long)tasks.size(); ++t) {
+    try {
+    const Task& k = tasks[t];
+    OpPtr o;
+    switch (k.kind) {
+      case 0: o = bd.pair_normal(false, k.i, k.j, k.K); break;
+      case 1: o = bd.pair_normal(true, k.i, k.j, k.K); break;
+      case 2: o = bd.cdcomp(k.i, k.j, k.K); break;
+      case 3: o = bd.ddcomp(k.i, k.j, k.K); break;
+      default: o = bd.ccdcomp(k.i); break;
+    }
+    if (o && post) o = post(o);
+    switch (k.kind) {
+      case 0: b.cd[b.pidx(k.i, k.j, k.K)] = o; break;
+      case 1: b.cc[b.pidx(k.i, k.j, k.K)] = o; break;
+      case 2: b.cdc[b.pidx(k.i, k.j, k.K)] = o; break;
+      case 3: b.ddc[b.pidx(k.i, k.j, k.K)] = o; break;
+      default: b.ccd[k.i] = o; break;
+    }
+    } catch (const std::exception& e) {
+#pragma omp critical(blk_task_err)
+      if (err.empty()) err = e.what();
+    }
+  }
+  if (!err.empty()) throw std::runtime_error(err);
+}
+std::vector<Task> comp_tasks(const Integrals& I, const SpinBlock& b, const std::vector<int>& outside,
+                             const Screen& scr) {
+  std::vector<Task> tasks;
+  for (int i : outside)
+    for (int j : outside) {
+      if (j > i) continue;
+      bool cdk = scr.cd_interacts(I, i, j, b.sites), ddk = scr.dd_interacts(I, i, j, b.sites);
+      for (int K : {0, 2}) {
+        if (cdk) tasks.push_back({2, i, j, K});
+        if (ddk && !(i == j && K == 2)) tasks.push_back({3, i, j, K});
+      }
+    }
+  return tasks;
+}
+}  // namespace
+
 BlockPtr build_sum_block(const Integrals& I, const SpinBlock& B1, const SpinBlock& dot, const Quantum& target,
                          int nrem, const BuildSpec& spec, const Screen& scr) {
   if (dot.sites.size() != 1) throw std::runtime_error("build_sum_block: second block must be a single site");
@@ -568,86 +618,113 @@ BlockPtr build_sum_block(const Integrals& I, const SpinBlock& B1, const SpinBloc
   b->ident = bd.make(0, 0, 0, false);
   tensor_product(V(*B1.ident), V(*dot.ident), *b->ident, 1.0);
   for (int i : b->sites) b->cre[i] = bd.cre(i);
+  std::vector<Task> tasks;
   if (spec.normal) {
     for (int i : b->sites)
       for (int j : b->sites) {
         if (j > i) continue;
         bool cdk = scr.cd_interacts(I, i, j, outside), ddk = scr.dd_interacts(I, i, j, outside);
         for (int K : {0, 2}) {
-          if (cdk) b->cd[b->pidx(i, j, K)] = bd.pair_normal(false, i, j, K);
-          if (ddk && !(i == j && K == 2)) b->cc[b->pidx(i, j, K)] = bd.pair_normal(true, i, j, K);
+          if (cdk) tasks.push_back({0, i, j, K});
+          if (ddk && !(i == j && K == 2)) tasks.push_back({1, i, j, K});
         }
       }
   }
   if (spec.comp) {
-    for (int i : outside)
-      for (int j : outside) {
-        if (j > i) continue;
-        bool cdk = scr.cd_interacts(I, i, j, b->sites), ddk = scr.dd_interacts(I, i, j, b->sites);
-        for (int K : {0, 2}) {
-          if (cdk) b->cdc[b->pidx(i, j, K)] = bd.cdcomp(i, j, K);
-          if (ddk && !(i == j && K == 2)) b->ddc[b->pidx(i, j, K)] = bd.ddcomp(i, j, K);
-        }
-      }
+    auto ct = comp_tasks(I, *b, outside, scr);
+    tasks.insert(tasks.end(), ct.begin(), ct.end());
   }
   if (spec.ccd)
-    for (int q : outside) b->ccd[q] = bd.ccdcomp(q);
+    for (int q : outside) tasks.push_back({4, q, 0, 0});
+  run_tasks(bd, *b, tasks, nullptr);
   b->ham = bd.ham();
   return b;
 }
 
 // ============================================================== renormalisation
-BlockPtr renormalise(const SpinBlock& B, const std::vector<std::vector<double>>& rot, const std::vector<int>& mkeep) {
-  const StateInfo& os = *B.si;
+Rotation make_rotation_info(const StateInfo& os, const std::vector<std::vector<double>>& rot,
+                            const std::vector<int>& mkeep) {
+  Rotation R;
   auto ns = std::make_shared<StateInfo>();
-  std::vector<int> newidx(os.nsec(), -1), oldidx;
   for (int s = 0; s < os.nsec(); ++s)
     if (mkeep[s] > 0) {
-      newidx[s] = ns->nsec();
-      oldidx.push_back(s);
+      R.oldidx.push_back(s);
       ns->q.push_back(os.q[s]);
       ns->dim.push_back(mkeep[s]);
     }
   ns->finalize();
+  R.ns = ns;
+  R.old = &os;
+  R.rot = &rot;
+  return R;
+}
+
+OpPtr rotate_op(const SparseOp& o, const Rotation& R) {
+  const StateInfo& os = *R.old;
+  auto r = std::make_shared<SparseOp>(get_pattern(R.ns, o.dn(), o.k2(), o.dirrep()), o.fermion);
+  const auto& P = *r->pat;
+  std::vector<double> tmp;
+  for (size_t ib = 0; ib < P.blocks.size(); ++ib) {
+    const auto& bk = P.blocks[ib];
+    int orow = R.oldidx[bk.r], ocol = R.oldidx[bk.c];
+    int obi = o.pat->find(orow, ocol);
+    if (obi < 0) continue;
+    int dr = os.dim[orow], dc = os.dim[ocol], mr = bk.rows, mc = bk.cols;
+    // tmp = O(orow,ocol) R_c  (dr x mc);  new = R_r^T tmp
+    tmp.assign((size_t)dr * mc, 0.0);
+    dgemm(false, false, dr, mc, dc, 1.0, o.blk(obi), dc, (*R.rot)[ocol].data(), mc, 0.0, tmp.data(), mc);
+    dgemm(true, false, mr, mc, dr, 1.0, (*R.rot)[orow].data(), mr, tmp.data(), mc, 0.0, r->blk((int)ib), mc);
+  }
+  return r;
+}
+
+BlockPtr renormalise(const SpinBlock& B, const std::vector<std::vector<double>>& rot, const std::vector<int>& mkeep) {
+  return renormalise(B, make_rotation_info(*B.si, rot, mkeep));
+}
+
+BlockPtr renormalise(const SpinBlock& B, const Rotation& R) {
+  if (R.old != B.si.get()) throw std::runtime_error("renormalise: rotation built for another block");
   auto nb = std::make_shared<SpinBlock>();
   nb->norb = B.norb;
   nb->sites = B.sites;
   nb->inblock = B.inblock;
-  nb->si = ns;
+  nb->si = R.ns;
   nb->has_normal = B.has_normal;
   nb->has_comp = B.has_comp;
-
-  auto rotop = [&](const OpPtr& o) -> OpPtr {
-    if (!o) return nullptr;
-    auto r = std::make_shared<SparseOp>(get_pattern(ns, o->dn(), o->k2(), o->dirrep()), o->fermion);
-    const auto& P = *r->pat;
-    std::vector<double> tmp;
-    for (size_t ib = 0; ib < P.blocks.size(); ++ib) {
-      const auto& bk = P.blocks[ib];
-      int orow = oldidx[bk.r], ocol = oldidx[bk.c];
-      int obi = o->pat->find(orow, ocol);
-      if (obi < 0) continue;
-      int dr = os.dim[orow], dc = os.dim[ocol], mr = bk.rows, mc = bk.cols;
-      // tmp = O(orow,ocol) * R_c  (dr x mc);  new = R_r^T tmp
-      tmp.assign((size_t)dr * mc, 0.0);
-      dgemm(false, false, dr, mc, dc, 1.0, o->blk(obi), dc, rot[ocol].data(), mc, 0.0, tmp.data(), mc);
-      dgemm(true, false, mr, mc, dr, 1.0, rot[orow].data(), mr, tmp.data(), mc, 0.0, r->blk((int)ib), mc);
+  nb->ident = rotate_op(*B.ident, R);
+  nb->ham = rotate_op(*B.ham, R);
+  std::vector<std::pair<const std::vector<OpPtr>*, std::vector<OpPtr>*>> lists = {
+      {&B.cre, &nb->cre}, {&B.ccd, &nb->ccd}, {&B.cd, &nb->cd}, {&B.cc, &nb->cc}, {&B.cdc, &nb->cdc}, {&B.ddc, &nb->ddc}};
+  std::vector<std::pair<const OpPtr*, OpPtr*>> work;
+  for (auto& l : lists) {
+    l.second->assign(l.first->size(), nullptr);
+    for (size_t i = 0; i < l.first->size(); ++i)
+      if ((*l.first)[i]) work.push_back({&(*l.first)[i], &(*l.second)[i]});
+  }
+  std::string err;
+#pragma omp parallel for schedule(dynamic)
+  for (long w = 0; w < (long)work.size(); ++w) {
+    try {
+      *work[w].second = rotate_op(**work[w].first, R);
+    } catch (const std::exception& e) {
+#pragma omp critical(blk_task_err)
+      if (err.empty()) err = e.what();
     }
-    return r;
-  };
-  nb->ident = rotop(B.ident);
-  nb->ham = rotop(B.ham);
-  auto rotv = [&](const std::vector<OpPtr>& in, std::vector<OpPtr>& out) {
-    out.assign(in.size(), nullptr);
-    for (size_t i = 0; i < in.size(); ++i) out[i] = rotop(in[i]);
-  };
-  rotv(B.cre, nb->cre);
-  rotv(B.ccd, nb->ccd);
-  rotv(B.cd, nb->cd);
-  rotv(B.cc, nb->cc);
-  rotv(B.cdc, nb->cdc);
-  rotv(B.ddc, nb->ddc);
+  }
+  if (!err.empty()) throw std::runtime_error(err);
   return nb;
+}
+
+void add_rotated_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
+                          const Screen& scr, const Rotation& R, SpinBlock& out) {
+  const int d = dot.sites[0];
+  Builder bd{I, B1, dot, d, B.si};
+  std::vector<int> outside;
+  for (int i = 0; i < I.norb; ++i)
+    if (!B.in(i)) outside.push_back(i);
+  auto tasks = comp_tasks(I, B, outside, scr);
+  run_tasks(bd, out, tasks, [&](OpPtr o) { return rotate_op(*o, R); });
+  out.has_comp = true;
 }
 
 }  // namespace blk

@@ -108,6 +108,21 @@ BlockPtr build_sum_block(const Integrals& I, const SpinBlock& B1, const SpinBloc
 // Renormalisation: new block with rotated operators O' = R^T O R per sector
 // (SparseMatrix::renormalise_transform, BaseOperator.C:341-459).  rot[Q] is
 // d_Q x m_Q row-major (m_Q may be 0: sector dropped).
+struct Rotation {
+  SIPtr ns;                 // kept sectors with their new dimensions
+  const StateInfo* old = nullptr;
+  std::vector<int> oldidx;  // new sector -> old sector
+  const std::vector<std::vector<double>>* rot = nullptr;
+};
+Rotation make_rotation_info(const StateInfo& os, const std::vector<std::vector<double>>& rot,
+                            const std::vector<int>& mkeep);
+OpPtr rotate_op(const SparseOp& o, const Rotation& R);
 BlockPtr renormalise(const SpinBlock& B, const std::vector<std::vector<double>>& rot, const std::vector<int>& mkeep);
+BlockPtr renormalise(const SpinBlock& B, const Rotation& R);
+// Build the complementary operators of the composite block B = B1 (x) dot one
+// at a time and store them rotated into out, the streaming form of
+// build_and_renormalise_transform (BaseOperator.C:341-459).
+void add_rotated_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
+                          const Screen& scr, const Rotation& R, SpinBlock& out);
 
 }  // namespace blk

@@ -3,7 +3,6 @@
 // Momentum", ch. 8-10).  Plays the role of new_anglib.C / couplingCoeffs.C in
 // the reference: clebsch (new_anglib.C:134), six_j (:73), nine_j (:19) and the
 // normalised Ninej (couplingCoeffs.h:88-95).  All arguments are doubled spins.
-#include <mutex>
 #include <unordered_map>
 
 #include "blk.h"
@@ -94,25 +93,18 @@ double wninej(int jlk, int jrk, int jk, int ka, int kb, int K, int jlb, int jrb,
   // all arguments are small non-negative ints (< 128)
   uint64_t key = 0;
   for (int v : {jlk, jrk, jk, ka, kb, K, jlb, jrb, jb}) key = (key << 7) | (uint64_t)(v & 127);
-  static std::unordered_map<Key9, double, Hash9> cache;
-  static std::mutex mu;
-  {
-    std::lock_guard<std::mutex> g(mu);
-    auto it = cache.find(Key9{key});
-    if (it != cache.end()) return it->second;
-  }
+  thread_local std::unordered_map<Key9, double, Hash9> cache;
+  auto it = cache.find(Key9{key});
+  if (it != cache.end()) return it->second;
   double w = std::sqrt((double)(jlb + 1) * (jrb + 1) * (jk + 1) * (K + 1)) * ninej(jlk, jrk, jk, ka, kb, K, jlb, jrb, jb);
-  std::lock_guard<std::mutex> g(mu);
   cache[Key9{key}] = w;
   return w;
 }
 
 double dagger_phase(int J2, int Jp2, int k2) {
   uint64_t key = ((uint64_t)J2 << 16) | ((uint64_t)Jp2 << 8) | (uint64_t)k2;
-  static std::unordered_map<uint64_t, double> cache;
-  static std::mutex mu;
+  thread_local std::unordered_map<uint64_t, double> cache;
   {
-    std::lock_guard<std::mutex> g(mu);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
   }
@@ -130,7 +122,6 @@ double dagger_phase(int J2, int Jp2, int k2) {
       found = true;
     }
   if (!found) throw std::runtime_error("dagger_phase: spins do not couple");
-  std::lock_guard<std::mutex> g(mu);
   cache[key] = ph;
   return ph;
 }

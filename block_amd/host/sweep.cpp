@@ -1,0 +1,313 @@
+#include "sweep.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <map>
+#include <numeric>
+#include <random>
+
+namespace blk {
+
+namespace {
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Hartree-Fock-like occupation from the orbital diagonal (hf_occ integral):
+// fill the lowest h_ii first, sharing electrons equally among degenerate levels.
+std::vector<double> hf_occupation(const Integrals& I, int nelec) {
+  const int n = I.norb;
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return I.t(a, a) < I.t(b, b); });
+  std::vector<double> occ(n, 0.0);
+  double left = nelec;
+  for (size_t g0 = 0; g0 < ord.size() && left > 0;) {
+    size_t g1 = g0 + 1;
+    while (g1 < ord.size() && std::fabs(I.t(ord[g1], ord[g1]) - I.t(ord[g0], ord[g0])) < 1e-8) ++g1;
+    double cap = 2.0 * (g1 - g0);
+    double put = std::min(cap, left);
+    for (size_t k = g0; k < g1; ++k) occ[ord[k]] = put / (g1 - g0);
+    left -= put;
+    g0 = g1;
+  }
+  return occ;
+}
+
+// Wilson-type truncation of a composite block: lowest eigenstates of its
+// Hamiltonian with the particle number held near its Hartree-Fock value.
+void wilson_rotation(const SpinBlock& B, double nbar, double lambda, int M, std::vector<std::vector<double>>& rot,
+                     std::vector<int>& mkeep) {
+  const StateInfo& si = *B.si;
+  const int ns = si.nsec();
+  std::vector<std::vector<double>> evec(ns), eval(ns);
+  struct Cand {
+    double score;
+    int q, k;
+  };
+  std::vector<Cand> cands;
+  for (int q = 0; q < ns; ++q) {
+    int d = si.dim[q];
+    int ib = B.ham->pat->find(q, q);
+    evec[q].assign((size_t)d * d, 0.0);
+    eval[q].assign(d, 0.0);
+    if (ib >= 0) std::copy(B.ham->blk(ib), B.ham->blk(ib) + (size_t)d * d, evec[q].begin());
+    syev(d, evec[q].data(), eval[q].data());
+    double pen = lambda * (si.q[q].n - nbar) * (si.q[q].n - nbar);
+    for (int k = 0; k < d; ++k) cands.push_back({eval[q][k] + pen, q, k});
+  }
+  std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.score < b.score; });
+  mkeep.assign(ns, 0);
+  std::vector<std::vector<int>> picked(ns);
+  for (int i = 0; i < (int)cands.size() && i < M; ++i) picked[cands[i].q].push_back(cands[i].k);
+  rot.assign(ns, {});
+  for (int q = 0; q < ns; ++q) {
+    int d = si.dim[q], m = (int)picked[q].size();
+    mkeep[q] = m;
+    rot[q].assign((size_t)d * m, 0.0);
+    for (int c = 0; c < m; ++c)
+      for (int r = 0; r < d; ++r) rot[q][(size_t)r * m + c] = evec[q][(size_t)r * d + picked[q][c]];
+  }
+}
+}  // namespace
+
+double make_rotation(const StateInfo& si, const std::vector<std::vector<double>>& rho, int M,
+                     std::vector<std::vector<double>>& rot, std::vector<int>& mkeep) {
+  const int ns = si.nsec();
+  std::vector<std::vector<double>> evec(ns), eval(ns);
+  // weights sorted descending; ties in (sector, index) order (sort_weights)
+  std::multimap<double, std::pair<int, int>> wmap;
+  double total = 0.0;
+  for (int q = 0; q < ns; ++q) {
+    int d = si.dim[q];
+    evec[q] = rho[q];
+    eval[q].assign(d, 0.0);
+    syev(d, evec[q].data(), eval[q].data());
+    for (int k = 0; k < d; ++k) {
+      if (eval[q][k] < 1e-14) eval[q][k] = 0.0;
+      wmap.insert({eval[q][k], {q, k}});
+      total += eval[q][k];
+    }
+  }
+  std::vector<std::vector<int>> picked(ns);
+  double kept = 0.0;
+  int taken = 0;
+  for (auto it = wmap.rbegin(); it != wmap.rend() && taken < M; ++it, ++taken) {
+    if (it->first <= 1e-13) continue;  // assign_matrix_by_dm keeps weights > 1e-13 only
+    picked[it->second.first].push_back(it->second.second);
+    kept += it->first;
+  }
+  rot.assign(ns, {});
+  mkeep.assign(ns, 0);
+  for (int q = 0; q < ns; ++q) {
+    int d = si.dim[q], m = (int)picked[q].size();
+    mkeep[q] = m;
+    rot[q].assign((size_t)d * m, 0.0);
+    for (int c = 0; c < m; ++c)
+      for (int r = 0; r < d; ++r) rot[q][(size_t)r * m + c] = evec[q][(size_t)r * d + picked[q][c]];
+  }
+  return total - kept;
+}
+
+BlockPtr warmup_block(const Integrals& I, const std::vector<int>& order, int nsites, const Quantum& target, int M,
+                      bool normal_ops, const Screen& scr) {
+  const int k = I.norb;
+  auto occ = hf_occupation(I, target.n);
+  double lambda = 1.0;
+  for (int i = 0; i < k; ++i) lambda = std::max(lambda, std::fabs(I.t(i, i)) + std::fabs(I.v(i, i, i, i)));
+  BlockPtr B = make_dot(I, order[0], scr);
+  double nbar = occ[order[0]];
+  BuildSpec sp;
+  sp.normal = normal_ops;
+  for (int s = 1; s < nsites; ++s) {
+    const int site = order[s];
+    nbar += occ[site];
+    auto dot = make_dot(I, site, scr);
+    sp.comp = false;
+    BlockPtr C = build_sum_block(I, *B, *dot, target, k - (s + 1), sp, scr);
+    std::vector<std::vector<double>> rot;
+    std::vector<int> mkeep;
+    wilson_rotation(*C, nbar, lambda, M, rot, mkeep);
+    Rotation R = make_rotation_info(*C->si, rot, mkeep);
+    BlockPtr nb = renormalise(*C, R);
+    add_rotated_comp_ops(I, *C, *B, *dot, scr, R, *nb);
+    B = nb;
+  }
+  return B;
+}
+
+SuperBlock snapshot_superblock(const Integrals& I, const Quantum& target, int nsys, int M, const Screen& scr) {
+  const int k = I.norb;
+  std::vector<int> fwd(k), bwd(k);
+  std::iota(fwd.begin(), fwd.end(), 0);
+  for (int i = 0; i < k; ++i) bwd[i] = k - 1 - i;
+  // sys: sites 0..nsys-1, dots nsys and nsys+1, env: nsys+2..k-1
+  BlockPtr sys = warmup_block(I, fwd, nsys, target, M, true, scr);
+  BlockPtr env = warmup_block(I, bwd, k - nsys - 2, target, M, false, scr);
+  BuildSpec sl;
+  sl.normal = true;
+  sl.comp = false;
+  BuildSpec sr;
+  sr.normal = false;
+  sr.comp = true;
+  BlockPtr L = build_sum_block(I, *sys, *make_dot(I, nsys, scr), target, k - nsys - 1, sl, scr);
+  BlockPtr R = build_sum_block(I, *env, *make_dot(I, nsys + 1, scr), target, nsys + 1, sr, scr);
+  return make_superblock(I, L, R, target);
+}
+
+DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory& make_sigma) {
+  const int k = I.norb;
+  if (k < 4) throw std::runtime_error("run_dmrg: need at least 4 orbitals for two-site sweeps");
+  if (P.schedule.empty()) throw std::runtime_error("run_dmrg: empty schedule");
+  const Quantum target{P.nelec, P.twoS, P.irrep};
+  Screen scr;
+  std::vector<double> weights = P.weights;
+  if ((int)weights.size() != P.nroots) weights.assign(P.nroots, 1.0 / P.nroots);
+  std::mt19937_64 rng(P.seed);
+  std::normal_distribution<double> gauss(0.0, 1.0);
+  DmrgResult res;
+
+  std::vector<BlockPtr> fwd(k), bwd(k);  // fwd[l]: sites 0..l ; bwd[l]: sites l..k-1
+  std::vector<int> rorder(k);
+  for (int i = 0; i < k; ++i) rorder[i] = k - 1 - i;
+  // environment warm-up (right to left)
+  {
+    int Mw = P.warmup_M > 0 ? P.warmup_M : P.schedule[0].M;
+    auto occ = hf_occupation(I, P.nelec);
+    double lambda = 1.0;
+    for (int i = 0; i < k; ++i) lambda = std::max(lambda, std::fabs(I.t(i, i)) + std::fabs(I.v(i, i, i, i)));
+    bwd[k - 1] = make_dot(I, k - 1, scr);
+    double nbar = occ[k - 1];
+    for (int l = k - 2; l >= 3; --l) {
+      auto dot = make_dot(I, l, scr);
+      nbar += occ[l];
+      BuildSpec sp;
+      sp.normal = false;
+      sp.comp = false;
+      BlockPtr C = build_sum_block(I, *bwd[l + 1], *dot, target, l, sp, scr);
+      std::vector<std::vector<double>> rot;
+      std::vector<int> mkeep;
+      wilson_rotation(*C, nbar, lambda, Mw, rot, mkeep);
+      Rotation R = make_rotation_info(*C->si, rot, mkeep);
+      BlockPtr nb = renormalise(*C, R);
+      add_rotated_comp_ops(I, *C, *bwd[l + 1], *dot, scr, R, *nb);
+      bwd[l] = nb;
+    }
+  }
+  fwd[0] = make_dot(I, 0, scr);
+
+  double prev_full = 1e300;
+  std::vector<double> sweep_e;
+  for (int sweep = 0; sweep < P.max_sweeps; ++sweep) {
+    // schedule entry for this sweep
+    const ScheduleEntry* S = &P.schedule[0];
+    for (const auto& e : P.schedule)
+      if (sweep >= e.start_sweep) S = &e;
+    const bool forward = (sweep % 2 == 0);
+    SweepRecord rec;
+    rec.sweep = sweep;
+    rec.forward = forward;
+    rec.M = S->M;
+    rec.energies.assign(P.nroots, 1e300);
+    double t0 = now_s();
+    const int nsteps = k - 3;
+    for (int step = 0; step < nsteps; ++step) {
+      // forward: sys = 0..l, dots l+1, l+2, env l+3..; backward mirrored
+      int l = forward ? step : (k - 1 - step);
+      int d1 = forward ? l + 1 : l - 1;
+      int d2 = forward ? l + 2 : l - 2;
+      BlockPtr sys = forward ? fwd[l] : bwd[l];
+      BlockPtr env = forward ? bwd[l + 3] : fwd[l - 3];
+      auto dot1 = make_dot(I, d1, scr);
+      auto dot2 = make_dot(I, d2, scr);
+      int nL = (int)sys->sites.size() + 1, nR = (int)env->sites.size() + 1;
+      BuildSpec sl;
+      sl.normal = true;
+      sl.comp = false;
+      BuildSpec sr;
+      sr.normal = false;
+      sr.comp = true;
+      BlockPtr L = build_sum_block(I, *sys, *dot1, target, nR, sl, scr);
+      BlockPtr R = build_sum_block(I, *env, *dot2, target, nL, sr, scr);
+      SuperBlock sb = make_superblock(I, L, R, target);
+      const long N = sb.wave->size;
+      if (N < P.nroots) throw std::runtime_error("run_dmrg: superblock smaller than the number of roots");
+      auto hfun = make_sigma(sb);
+      std::vector<double> diag = diagonal_h(sb);
+      // guess: lowest diagonal elements with a little seeded noise
+      std::vector<long> ord(N);
+      std::iota(ord.begin(), ord.end(), 0);
+      std::partial_sort(ord.begin(), ord.begin() + P.nroots, ord.end(),
+                        [&](long a, long b) { return diag[a] < diag[b]; });
+      std::vector<std::vector<double>> b(P.nroots, std::vector<double>(N));
+      for (int r = 0; r < P.nroots; ++r) {
+        for (long x = 0; x < N; ++x) b[r][x] = 1e-3 * gauss(rng);
+        b[r][ord[r]] += 1.0;
+      }
+      DavidsonStats st;
+      std::vector<double> e = diag;
+      block_davidson(b, e, S->davidson_tol, *hfun, true, P.deflation_min, std::max(P.deflation_max, P.nroots + 2),
+                     &st, P.davidson_max_iter);
+      hfun.reset();
+      rec.sigma_seconds += st.sigma_seconds;
+      rec.sigma_calls += st.sigma_calls;
+      rec.sigma_flops += st.sigma_calls * sigma_flops(sb);
+      for (int r = 0; r < P.nroots; ++r) rec.energies[r] = std::min(rec.energies[r], e[r]);
+      // density matrix of L (makedensitymatrix, density.C:27-90) plus noise
+      const StateInfo& ls = *sb.wave->L;
+      std::vector<std::vector<double>> rho(ls.nsec());
+      for (int q = 0; q < ls.nsec(); ++q) rho[q].assign((size_t)ls.dim[q] * ls.dim[q], 0.0);
+      for (int r = 0; r < P.nroots; ++r)
+        for (const auto& bk : sb.wave->blocks)
+          dgemm(false, true, bk.rows, bk.rows, bk.cols, weights[r], b[r].data() + bk.off, bk.cols, b[r].data() + bk.off,
+                bk.cols, 1.0, rho[bk.l].data(), bk.rows);
+      if (S->noise > 0.0) {
+        for (int q = 0; q < ls.nsec(); ++q) {
+          int d = ls.dim[q];
+          std::vector<double> G((size_t)d * d);
+          for (auto& x : G) x = gauss(rng);
+          dgemm(false, true, d, d, d, S->noise / ((double)d * d), G.data(), d, G.data(), d, 1.0, rho[q].data(), d);
+        }
+      }
+      std::vector<std::vector<double>> rot;
+      std::vector<int> mkeep;
+      double disc = make_rotation(ls, rho, S->M, rot, mkeep);
+      rec.max_discarded = std::max(rec.max_discarded, disc);
+      Rotation Rot = make_rotation_info(*L->si, rot, mkeep);
+      BlockPtr nsys = renormalise(*L, Rot);
+      add_rotated_comp_ops(I, *L, *sys, *dot1, scr, Rot, *nsys);
+      if (forward)
+        fwd[l + 1] = nsys;
+      else
+        bwd[l - 1] = nsys;
+      if (P.verbose > 1)
+        std::fprintf(stderr, "  sweep %d step %d  N=%ld  E0=%.12f  iters=%d  disc=%.2e\n", sweep, step, N, e[0],
+                     st.iterations, disc);
+    }
+    // the sweep ends at the far edge: the last block is the single-site dot
+    if (forward)
+      bwd[k - 1] = make_dot(I, k - 1, scr);
+    else
+      fwd[0] = make_dot(I, 0, scr);
+    rec.seconds = now_s() - t0;
+    res.sweeps.push_back(rec);
+    if (P.verbose)
+      std::fprintf(stderr, "sweep %d %s M=%d E=%.12f disc=%.2e t=%.2fs\n", sweep, forward ? "fwd" : "bwd", S->M,
+                   rec.energies[0], rec.max_discarded, rec.seconds);
+    res.energies = rec.energies;
+    // convergence between consecutive full sweeps once the schedule is exhausted
+    if (!forward) {
+      double e = rec.energies[0];
+      bool last_entry = sweep >= P.schedule.back().start_sweep;
+      if (last_entry && std::fabs(e - prev_full) < P.sweep_tol) {
+        res.converged = true;
+        break;
+      }
+      prev_full = e;
+    }
+  }
+  return res;
+}
+
+}  // namespace blk

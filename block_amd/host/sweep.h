@@ -1,0 +1,73 @@
+// Two-site sweep driver: the caller of the accelerated path.  Restates
+// Sweep::do_one / BlockAndDecimate (sweep.C:180-651), the environment warm-up,
+// Solver::solve_wavefunction (solver.C:19-143) and the density-matrix
+// truncation of makeRotateMatrix (rotationmat.C:60-347) with its own code; the
+// Davidson sigma is supplied through a functor factory so the sweep calls the
+// GPU multiply_h as a drop-in.
+#pragma once
+#include <functional>
+#include <memory>
+
+#include "davidson.h"
+
+namespace blk {
+
+struct ScheduleEntry {
+  int start_sweep;  // first sweep (half-sweep counter) this entry applies to
+  int M;
+  double davidson_tol;  // on the squared residual norm, as linear.C:301
+  double noise;
+};
+
+struct DmrgParams {
+  int nelec = 0, twoS = 0, irrep = 0;
+  int nroots = 1;
+  std::vector<double> weights;  // state-average weights (default equal)
+  std::vector<ScheduleEntry> schedule;
+  int max_sweeps = 20;
+  double sweep_tol = 1e-8;  // on the change of the lowest sweep energy between full sweeps
+  int warmup_M = 0;         // 0: first schedule M
+  int deflation_min = 2, deflation_max = 20;
+  int davidson_max_iter = 200;
+  uint64_t seed = 12345;
+  int verbose = 0;
+};
+
+struct SweepRecord {
+  int sweep;
+  bool forward;
+  int M;
+  std::vector<double> energies;  // lowest per root over the sweep
+  double max_discarded = 0.0;
+  double seconds = 0.0;
+  double sigma_seconds = 0.0;
+  long sigma_calls = 0;
+  double sigma_flops = 0.0;
+};
+
+struct DmrgResult {
+  std::vector<SweepRecord> sweeps;
+  std::vector<double> energies;  // final per-root energies
+  bool converged = false;
+};
+
+using SigmaFactory = std::function<std::unique_ptr<DavidsonFunctor>(const SuperBlock&)>;
+
+DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory& make_sigma);
+
+// Truncation of one block's density matrix to at most M multiplets
+// (diagonalise_dm + sort_weights + assign_matrix_by_dm, rotationmat.C):
+// rot[Q] gets the kept eigenvectors as columns, mkeep[Q] their number;
+// returns the discarded weight.
+double make_rotation(const StateInfo& si, const std::vector<std::vector<double>>& rho, int M,
+                     std::vector<std::vector<double>>& rot, std::vector<int>& mkeep);
+
+// Environment warm-up block sequence and mid-chain superblock snapshots used
+// by the benchmark: blocks grown from one end with a Wilson-type truncation
+// (lowest energy states of the block Hamiltonian, particle number held near
+// its Hartree-Fock value).
+BlockPtr warmup_block(const Integrals& I, const std::vector<int>& order, int nsites, const Quantum& target, int M,
+                      bool normal_ops, const Screen& scr);
+SuperBlock snapshot_superblock(const Integrals& I, const Quantum& target, int nsys, int M, const Screen& scr);
+
+}  // namespace blk
