@@ -1,0 +1,206 @@
+"""MI355X-native Davidson sigma step (H.C) of spin-adapted two-site DMRG.
+
+The compute path is native: ``libblock_sigma.so`` (hand-written HIP kernels
+for gfx950 behind the C-ABI of ``include/block_sigma.h``) driven by
+``libblock_host.so`` (the C++ restatement of Block's superblock, operator
+algebra, Davidson solver and sweep).  This module only loads them with ctypes;
+there is no Python or CPU fallback: if the libraries are missing the import
+fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _load(name):
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run __graft_entry__.build() (make -C block_amd)")
+    return ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
+sigma_lib = _load("libblock_sigma.so")
+host_lib = _load("libblock_host.so")
+
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_h = host_lib
+_sig(_h, "blkh_last_error", ctypes.c_char_p, [])
+_sig(_h, "blkh_load_fcidump", _vp, [ctypes.c_char_p])
+_sig(_h, "blkh_free_integrals", None, [_vp])
+for _n in ("blkh_norb", "blkh_nelec", "blkh_isym", "blkh_ms2"):
+    _sig(_h, _n, ctypes.c_int, [_vp])
+_sig(_h, "blkh_ecore", ctypes.c_double, [_vp])
+_sig(_h, "blkh_snapshot", _vp, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int])
+_sig(_h, "blkh_free_superblock", None, [_vp])
+_sig(_h, "blkh_wave_size", ctypes.c_long, [_vp])
+_sig(_h, "blkh_flops", ctypes.c_double, [_vp])
+_sig(_h, "blkh_nterms", ctypes.c_int, [_vp])
+_sig(_h, "blkh_nsectors", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int)])
+_sig(_h, "blkh_operator_bytes", ctypes.c_long, [_vp])
+_sig(_h, "blkh_diag_cpu", ctypes.c_int, [_vp, _dp])
+_sig(_h, "blkh_gpu_sigma_create", _vp, [_vp, ctypes.c_int, ctypes.c_long])
+_sig(_h, "blkh_gpu_sigma_free", None, [_vp])
+_sig(_h, "blkh_gpu_sigma_apply", ctypes.c_int, [_vp, _dp, _dp])
+_sig(_h, "blkh_gpu_sigma_diag", ctypes.c_int, [_vp, _dp])
+_sig(_h, "blkh_gpu_sigma_flops", ctypes.c_double, [_vp])
+_sig(_h, "blkh_gpu_sigma_pairs", ctypes.c_long, [_vp])
+_sig(_h, "blkh_gpu_sigma_arena_bytes", ctypes.c_long, [_vp])
+_sig(_h, "blkh_gpu_sigma_timed", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _lp])
+_sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
+     [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
+      ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp])
+
+
+class BlockError(RuntimeError):
+    pass
+
+
+def _err():
+    return BlockError(host_lib.blkh_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Integrals:
+    """One- and two-electron integrals read from an FCIDUMP (input.C:1407-1560)."""
+
+    def __init__(self, path):
+        self._h = host_lib.blkh_load_fcidump(str(path).encode())
+        if not self._h:
+            raise _err()
+        self.path = str(path)
+
+    norb = property(lambda s: host_lib.blkh_norb(s._h))
+    nelec = property(lambda s: host_lib.blkh_nelec(s._h))
+    isym = property(lambda s: host_lib.blkh_isym(s._h))
+    ms2 = property(lambda s: host_lib.blkh_ms2(s._h))
+    ecore = property(lambda s: host_lib.blkh_ecore(s._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            host_lib.blkh_free_integrals(self._h)
+            self._h = None
+
+
+class SuperBlock:
+    """Mid-chain two-site superblock L (x) R (sweep.C:91-178): sys = orbitals
+    [0, nsys), dots nsys and nsys+1, environment the rest, blocks grown with a
+    Wilson-type truncation to M multiplets."""
+
+    def __init__(self, ints, nelec, twoS, irrep, nsys, M, _handle=None):
+        self.ints = ints
+        self._h = _handle if _handle is not None else host_lib.blkh_snapshot(ints._h, nelec, twoS, irrep, nsys, M)
+        if not self._h:
+            raise _err()
+
+    wave_size = property(lambda s: host_lib.blkh_wave_size(s._h))
+    flops = property(lambda s: host_lib.blkh_flops(s._h))
+    nterms = property(lambda s: host_lib.blkh_nterms(s._h))
+    operator_bytes = property(lambda s: host_lib.blkh_operator_bytes(s._h))
+
+    def sectors(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        host_lib.blkh_nsectors(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
+    def diagonal_cpu(self):
+        out = np.zeros(self.wave_size)
+        if host_lib.blkh_diag_cpu(self._h, _ptr(out)) != 0:
+            raise _err()
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            host_lib.blkh_free_superblock(self._h)
+            self._h = None
+
+
+class multiply_h:
+    """GPU sigma functor for one superblock: ``v = H c`` (davidson.h:22-29).
+
+    Construction flattens the superblock into the GEMM-pair work list and makes
+    every operator block resident in HBM; calls run the FP64 MFMA kernels."""
+
+    def __init__(self, sb, device=0, w_budget=0):
+        self.sb = sb
+        self._h = host_lib.blkh_gpu_sigma_create(sb._h, device, w_budget)
+        if not self._h:
+            raise _err()
+        self.n = sb.wave_size
+
+    flops = property(lambda s: host_lib.blkh_gpu_sigma_flops(s._h))
+    pairs = property(lambda s: host_lib.blkh_gpu_sigma_pairs(s._h))
+    arena_bytes = property(lambda s: host_lib.blkh_gpu_sigma_arena_bytes(s._h))
+
+    def __call__(self, c):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        if c.shape != (self.n,):
+            raise ValueError(f"wavefunction must have shape ({self.n},)")
+        v = np.zeros(self.n)
+        if host_lib.blkh_gpu_sigma_apply(self._h, _ptr(c), _ptr(v)) != 0:
+            raise _err()
+        return v
+
+    def diagonal(self):
+        out = np.zeros(self.n)
+        if host_lib.blkh_gpu_sigma_diag(self._h, _ptr(out)) != 0:
+            raise _err()
+        return out
+
+    def timed(self, c, warmup, reps):
+        """Device-resident repeated sigma: returns (wall_ms, [phase1_ms, phase2_ms],
+        [phase1_launches, phase2_launches]) measured with HIP events on the plan stream."""
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        wall = ctypes.c_double()
+        km = np.zeros(2)
+        la = (ctypes.c_long * 2)()
+        if host_lib.blkh_gpu_sigma_timed(self._h, _ptr(c), warmup, reps, ctypes.byref(wall), _ptr(km), la) != 0:
+            raise _err()
+        return wall.value, km.tolist(), [la[0], la[1]]
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            host_lib.blkh_gpu_sigma_free(self._h)
+            self._h = None
+
+
+def _schedule_array(schedule):
+    rows = [(float(s), float(m), float(t), float(n)) for (s, m, t, n) in schedule]
+    return np.ascontiguousarray(np.array(rows, dtype=np.float64).reshape(-1))
+
+
+def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=None, max_sweeps=20,
+         sweep_tol=1e-8, device=0, verbose=0):
+    """Two-site spin-adapted DMRG (Sweep::do_one, sweep.C:341) with the Davidson
+    sigma on the GPU.  schedule: rows (start_sweep, M, davidson_tol, noise).
+    Returns (energies, sweeps) with sweeps rows (sweep, forward, M, E0,
+    discarded, seconds, sigma_seconds, sigma_flops)."""
+    nelec = ints.nelec if nelec is None else nelec
+    irrep = ints.isym if irrep is None else irrep
+    schedule = schedule or [(0, 50, 1e-6, 1e-5), (2, 100, 1e-8, 1e-6), (6, 100, 1e-12, 0.0)]
+    w = np.ascontiguousarray(weights if weights is not None else np.full(nroots, 1.0 / nroots), dtype=np.float64)
+    sched = _schedule_array(schedule)
+    e = np.zeros(nroots)
+    sw = np.zeros(8 * max_sweeps)
+    n = host_lib.blkh_dmrg_gpu(ints._h, nelec, twoS, irrep, nroots, _ptr(w), len(schedule), _ptr(sched), max_sweeps,
+                               sweep_tol, device, verbose, _ptr(e), _ptr(sw))
+    if n < 0:
+        raise _err()
+    return e, sw[: 8 * n].reshape(n, 8)

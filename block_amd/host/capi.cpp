@@ -1,0 +1,194 @@
+// extern "C" surface of the host library (libblock_host.so) used by the
+// Python layer: integrals, superblock snapshots, the GPU multiply_h and the
+// sweep driver running its Davidson sigma on the GPU.  Errors are reported as
+// negative return codes / null handles with blkh_last_error() holding the text.
+#include <cstring>
+#include <string>
+
+#include "gpu_sigma.h"
+#include "sweep.h"
+
+using namespace blk;
+
+namespace {
+thread_local std::string g_err;
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+template <class F>
+void* guarded_ptr(F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+struct GpuSigmaHandle {
+  std::unique_ptr<GpuContext> ctx;
+  std::unique_ptr<GpuSigma> sig;
+};
+
+struct GpuFunctor : DavidsonFunctor {
+  GpuSigma sig;
+  GpuFunctor(GpuContext& g, const SuperBlock& sb) : sig(g, sb) {}
+  void operator()(const double* c, double* v) override { sig.apply(c, v); }
+};
+}  // namespace
+
+extern "C" {
+
+const char* blkh_last_error(void) { return g_err.c_str(); }
+
+void* blkh_load_fcidump(const char* path) {
+  return guarded_ptr([&]() -> void* { return new Integrals(read_fcidump(path)); });
+}
+void blkh_free_integrals(void* h) { delete static_cast<Integrals*>(h); }
+int blkh_norb(void* h) { return static_cast<Integrals*>(h)->norb; }
+int blkh_nelec(void* h) { return static_cast<Integrals*>(h)->nelec; }
+int blkh_isym(void* h) { return static_cast<Integrals*>(h)->isym; }
+int blkh_ms2(void* h) { return static_cast<Integrals*>(h)->ms2; }
+double blkh_ecore(void* h) { return static_cast<Integrals*>(h)->ecore; }
+
+// mid-chain superblock: sys = sites [0, nsys), dots nsys and nsys+1, env the rest,
+// blocks grown with a Wilson truncation to M
+void* blkh_snapshot(void* ints, int nelec, int twoS, int irrep, int nsys, int M) {
+  return guarded_ptr([&]() -> void* {
+    const Integrals& I = *static_cast<Integrals*>(ints);
+    if (nsys < 1 || nsys + 2 > I.norb - 1) throw std::runtime_error("blkh_snapshot: nsys out of range");
+    Screen scr;
+    return new SuperBlock(snapshot_superblock(I, Quantum{nelec, twoS, irrep}, nsys, M, scr));
+  });
+}
+void blkh_free_superblock(void* h) { delete static_cast<SuperBlock*>(h); }
+long blkh_wave_size(void* h) { return static_cast<SuperBlock*>(h)->wave->size; }
+double blkh_flops(void* h) { return sigma_flops(*static_cast<SuperBlock*>(h)); }
+int blkh_nterms(void* h) { return (int)static_cast<SuperBlock*>(h)->terms.size(); }
+int blkh_nsectors(void* h, int* nl, int* nr, int* nblocks) {
+  auto* sb = static_cast<SuperBlock*>(h);
+  *nl = sb->L->si->nsec();
+  *nr = sb->R->si->nsec();
+  *nblocks = (int)sb->wave->blocks.size();
+  return 0;
+}
+long blkh_operator_bytes(void* h) {
+  auto* sb = static_cast<SuperBlock*>(h);
+  return (long)(sb->L->nbytes() + sb->R->nbytes());
+}
+int blkh_diag_cpu(void* h, double* out) {
+  return guarded([&]() {
+    auto d = diagonal_h(*static_cast<SuperBlock*>(h));
+    std::memcpy(out, d.data(), sizeof(double) * d.size());
+    return 0;
+  });
+}
+
+void* blkh_gpu_sigma_create(void* sb, int device, long w_budget) {
+  return guarded_ptr([&]() -> void* {
+    auto* h = new GpuSigmaHandle();
+    try {
+      h->ctx.reset(new GpuContext(device));
+      h->sig.reset(new GpuSigma(*h->ctx, *static_cast<SuperBlock*>(sb), w_budget > 0 ? w_budget : (int64_t)1 << 27));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    return h;
+  });
+}
+void blkh_gpu_sigma_free(void* h) { delete static_cast<GpuSigmaHandle*>(h); }
+int blkh_gpu_sigma_apply(void* h, const double* c, double* v) {
+  return guarded([&]() {
+    static_cast<GpuSigmaHandle*>(h)->sig->apply(c, v);
+    return 0;
+  });
+}
+int blkh_gpu_sigma_diag(void* h, double* out) {
+  return guarded([&]() {
+    auto d = static_cast<GpuSigmaHandle*>(h)->sig->diagonal();
+    std::memcpy(out, d.data(), sizeof(double) * d.size());
+    return 0;
+  });
+}
+double blkh_gpu_sigma_flops(void* h) { return static_cast<GpuSigmaHandle*>(h)->sig->flops(); }
+long blkh_gpu_sigma_pairs(void* h) { return (long)static_cast<GpuSigmaHandle*>(h)->sig->n_pairs(); }
+long blkh_gpu_sigma_arena_bytes(void* h) { return (long)static_cast<GpuSigmaHandle*>(h)->sig->arena_bytes(); }
+// device-resident loop for timing: c is uploaded once, v accumulates `reps` sigmas.
+// kernel_ms[0..1] receive the summed phase-1 / phase-2 kernel times measured
+// with HIP events on the plan's stream, wall_ms the whole loop.
+int blkh_gpu_sigma_timed(void* h, const double* c, int warmup, int reps, double* wall_ms, double* kernel_ms,
+                         long* launches) {
+  return guarded([&]() {
+    auto* g = static_cast<GpuSigmaHandle*>(h);
+    GpuSigma& s = *g->sig;
+    size_t bytes = sizeof(double) * s.wave_size();
+    bsg_check(bsg_memcpy_h2d(s.ctx(), s.dev_c(), c, bytes), "c upload");
+    bsg_check(bsg_memset_zero(s.ctx(), s.dev_v(), bytes), "v clear");
+    for (int i = 0; i < warmup; ++i) s.apply_device(s.dev_c(), s.dev_v());
+    bsg_check(bsg_synchronize(s.ctx()), "sync");
+    bsg_check(bsg_sigma_profile(s.ctx(), s.plan(), s.dev_arena(), s.dev_c(), s.dev_v(), reps, wall_ms, kernel_ms,
+                                launches),
+              "profile");
+    return 0;
+  });
+}
+
+// Two-site DMRG with the sigma on the GPU.  schedule: n_sched rows of
+// (start_sweep, M, davidson_tol, noise).  energies_out[nroots];
+// sweep_out[max_sweeps * 8]: sweep, forward, M, E0, discarded, seconds,
+// sigma_seconds, sigma_flops.  Returns the number of sweeps run.
+int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
+                  const double* sched, int max_sweeps, double sweep_tol, int device, int verbose, double* energies_out,
+                  double* sweep_out) {
+  return guarded([&]() {
+    const Integrals& I = *static_cast<Integrals*>(ints);
+    DmrgParams P;
+    P.nelec = nelec;
+    P.twoS = twoS;
+    P.irrep = irrep;
+    P.nroots = nroots;
+    if (weights) P.weights.assign(weights, weights + nroots);
+    for (int i = 0; i < n_sched; ++i)
+      P.schedule.push_back({(int)sched[4 * i], (int)sched[4 * i + 1], sched[4 * i + 2], sched[4 * i + 3]});
+    P.max_sweeps = max_sweeps;
+    P.sweep_tol = sweep_tol;
+    P.verbose = verbose;
+    GpuContext ctx(device);
+    auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {
+      return std::unique_ptr<DavidsonFunctor>(new GpuFunctor(ctx, sb));
+    });
+    for (int r = 0; r < nroots; ++r) energies_out[r] = res.energies[r];
+    for (size_t i = 0; i < res.sweeps.size(); ++i) {
+      const auto& s = res.sweeps[i];
+      double* o = sweep_out + 8 * i;
+      o[0] = s.sweep;
+      o[1] = s.forward;
+      o[2] = s.M;
+      o[3] = s.energies[0];
+      o[4] = s.max_discarded;
+      o[5] = s.seconds;
+      o[6] = s.sigma_seconds;
+      o[7] = s.sigma_flops;
+    }
+    return (int)res.sweeps.size();
+  });
+}
+
+}  // extern "C"
+
+extern "C" {
+// SU(2) coefficients (doubled spins), exposed for the closed-form checks
+double blkh_cg(int j1, int m1, int j2, int m2, int J, int M) { return cg(j1, m1, j2, m2, J, M); }
+double blkh_sixj(int a, int b, int c, int d, int e, int f) { return sixj(a, b, c, d, e, f); }
+double blkh_ninej(int a, int b, int c, int d, int e, int f, int g, int h, int i) {
+  return ninej(a, b, c, d, e, f, g, h, i);
+}
+double blkh_dagger_phase(int J2, int Jp2, int k2) { return dagger_phase(J2, Jp2, k2); }
+}

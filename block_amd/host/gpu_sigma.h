@@ -56,6 +56,7 @@ class GpuSigma {
   double* dev_c() const { return d_c_; }
   double* dev_v() const { return d_v_; }
   const double* dev_arena() const { return d_arena_; }
+  bsg_plan* plan() const { return plan_; }
 
  private:
   GpuContext& g_;

@@ -1,0 +1,143 @@
+/* C-ABI of the MI355X Davidson sigma path (libblock_sigma.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns BSG_OK (0) or a
+ * negative error code; bsg_last_error() gives the message of the last failure
+ * on the calling thread.  All work of one context runs on that context's
+ * single HIP stream, in call order (the single-stream contract).
+ *
+ * Reference interfaces these replace (sanshar/Block 1.1.1):
+ *   multiply_h::operator()(Wavefunction& c, Wavefunction& v)   davidson.h:22-29, davidson.C:17-22
+ *     -> bsg_sigma_apply  (v += H.c, accumulate semantics as in
+ *        operatorfunctions.C:530 and spinblock.C:722-789)
+ *   multiply_h(const SpinBlock& b, bool onedot) constructor      davidson.h:26
+ *     -> bsg_plan_create (the superblock flattened into GEMM pairs by the host
+ *        work-list compiler; operator blocks resident in HBM)
+ *   SpinBlock::diagonalH(DiagonalMatrix&)                         spinblock.h:240, spinblock.C:855-899
+ *     -> bsg_diag_apply
+ *   DotProduct / ScaleAdd / Scale                                 BaseOperator.h:287-291, BaseOperator.C:240-288
+ *     -> bsg_dot / bsg_axpy / bsg_scal
+ *   Linear::precondition / olsenPrecondition                      linear.C:27-60
+ *     -> bsg_precondition / bsg_olsen_precondition
+ */
+#ifndef BLOCK_SIGMA_H
+#define BLOCK_SIGMA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  BSG_OK = 0,
+  BSG_ERR_ARG = -1,
+  BSG_ERR_HIP = -2,
+  BSG_ERR_NOMEM = -3,
+  BSG_ERR_UNSUPPORTED = -4
+};
+
+enum { BSG_FP64 = 0, BSG_SPLIT_BF16 = 1 };
+
+typedef struct bsg_context bsg_context;
+typedef struct bsg_plan bsg_plan;
+
+/* Phase-1 item: W[w_off] (m x n) = alpha * op(A) (m x k) * C[c_off] (k x n).
+ * A lives in the operator arena at a_off, stored m x k row-major (a_trans=0)
+ * or k x m row-major (a_trans=1).  C is a wavefunction block, k x n row-major. */
+typedef struct {
+  int64_t a_off, c_off, w_off;
+  int32_t m, n, k, a_trans;
+  double alpha;
+} bsg_gemm1_t;
+
+/* Phase-2 contribution to one V block (m x n):
+ *   V += alpha * S (m x k) * B' (k x n)
+ * S is W scratch (src_kind 0) or a block of the input C (src_kind 1), row-major
+ * with leading dimension k.  B' = op(B)^T from the arena at b_off, stored
+ * n x k row-major (b_trans=1) or k x n row-major (b_trans=0); b_off < 0 means
+ * B' is the identity (then k == n). */
+typedef struct {
+  int64_t src_off, b_off;
+  int32_t src_kind, b_trans, k, pad;
+  double alpha;
+} bsg_contrib_t;
+
+/* V block at v_off (m x n row-major) with contributions [first, first+count),
+ * summed in list order (deterministic). */
+typedef struct {
+  int64_t v_off;
+  int32_t m, n, first, count;
+} bsg_vblock_t;
+
+/* One chunk of the work list: phase 1 then phase 2. */
+typedef struct {
+  const bsg_gemm1_t* gemm1;
+  int32_t n_gemm1;
+  const bsg_vblock_t* vblocks;
+  int32_t n_vblocks;
+  const bsg_contrib_t* contribs;
+  int32_t n_contribs;
+  int64_t w_size; /* doubles of W scratch this chunk needs */
+} bsg_chunk_t;
+
+/* Diagonal term: diag[v_off + i*n + j] += alpha * A[a_off + i*(m+1)] * B[b_off + j*(n+1)]
+ * for i < m, j < n (a_off/b_off < 0: identity). */
+typedef struct {
+  int64_t v_off, a_off, b_off;
+  int32_t m, n;
+  double alpha;
+} bsg_diag_t;
+
+const char* bsg_last_error(void);
+
+int bsg_init(bsg_context** ctx, int device);
+int bsg_free(bsg_context* ctx);
+int bsg_synchronize(bsg_context* ctx);
+
+/* device memory owned by the context's caller */
+int bsg_malloc(bsg_context* ctx, size_t bytes, void** dptr);
+int bsg_mfree(bsg_context* ctx, void* dptr);
+int bsg_memcpy_h2d(bsg_context* ctx, void* dst, const void* src, size_t bytes);
+int bsg_memcpy_d2h(bsg_context* ctx, void* dst, const void* src, size_t bytes);
+int bsg_memset_zero(bsg_context* ctx, void* dst, size_t bytes);
+
+/* Build a device plan from host work-list chunks (copied; the host arrays may
+ * be freed afterwards).  The plan allocates the W scratch of its largest chunk. */
+int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunks, int64_t wave_size,
+                    bsg_plan** plan);
+int bsg_plan_destroy(bsg_plan* plan);
+/* Flops (algorithmic, GEMM pairs) of one application, for reporting. */
+double bsg_plan_flops(const bsg_plan* plan);
+
+/* v += H c for nvec vectors stored back to back (stride wave_size doubles).
+ * arena: device operator arena; c, v: device pointers.  precision: BSG_FP64. */
+int bsg_sigma_apply(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t nvec,
+                    int32_t precision);
+
+/* Timing helper: runs `reps` sigma applications (FP64) on the context stream
+ * with HIP events around every kernel launch.  wall_ms: whole loop;
+ * kernel_ms[0], kernel_ms[1]: summed phase-1 / phase-2 kernel time;
+ * launches[0..1]: number of launches of each kernel. */
+int bsg_sigma_profile(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t reps,
+                      double* wall_ms, double* kernel_ms, long* launches);
+
+/* diag += diagonal terms (device pointers). */
+int bsg_diag_apply(bsg_context* ctx, const bsg_diag_t* terms_host, int32_t n_terms, const double* arena,
+                   double* diag);
+
+/* Davidson vector kernels on device vectors of length n. */
+int bsg_dot(bsg_context* ctx, const double* x, const double* y, int64_t n, double* result_host);
+int bsg_axpy(bsg_context* ctx, double a, const double* x, double* y, int64_t n);
+int bsg_scal(bsg_context* ctx, double a, double* x, int64_t n);
+/* x[i] /= (e - diag[i] + shift) where |e - diag[i]| > 1e-12 (linear.C:27-47) */
+int bsg_precondition(bsg_context* ctx, double* x, const double* diag, double e, double shift, int64_t n);
+/* Olsen step (linear.C:50-60): r -= (<C~,r>/<C,C~>) C with C~ = precond(C),
+ * then r = precond(r).  work: device scratch of n doubles. */
+int bsg_olsen_precondition(bsg_context* ctx, double* r, const double* c, const double* diag, double e, double shift,
+                           double* work, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

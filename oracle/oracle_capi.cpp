@@ -1,0 +1,171 @@
+// extern "C" surface of the oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY:
+// loaded by tests/, the smoke check and bench.py's cpu_baseline leg.
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "../block_amd/host/gpu_sigma.h"
+#include "../block_amd/host/sweep.h"
+#include "oracle.h"
+
+using namespace blk;
+
+namespace {
+thread_local std::string g_err;
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+struct OracleFunctor : DavidsonFunctor {
+  const SuperBlock& sb;
+  int nthreads;
+  OracleFunctor(const SuperBlock& s, int nt) : sb(s), nthreads(nt) {}
+  void operator()(const double* c, double* v) override {
+    std::fill(v, v + sb.wave->size, 0.0);
+    oracle::oracle_multiply_h(sb, c, v, nthreads);
+  }
+};
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+// v = H c by the direct restatement (v overwritten)
+int orc_sigma(void* sb, const double* c, double* v, int nthreads) {
+  return guarded([&]() {
+    auto* s = static_cast<SuperBlock*>(sb);
+    std::fill(v, v + s->wave->size, 0.0);
+    oracle::oracle_multiply_h(*s, c, v, nthreads);
+    return 0;
+  });
+}
+// same, timed (seconds returned through secs)
+int orc_sigma_timed(void* sb, const double* c, double* v, int nthreads, double* secs) {
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = orc_sigma(sb, c, v, nthreads);
+  *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+int orc_diag(void* sb, double* out) {
+  return guarded([&]() {
+    auto d = oracle::oracle_diagonal(*static_cast<SuperBlock*>(sb));
+    std::memcpy(out, d.data(), sizeof(double) * d.size());
+    return 0;
+  });
+}
+// CPU execution of the compiled GPU work list (host arena, same offsets)
+int orc_exec_worklist(void* sbp, const double* c, double* v) {
+  return guarded([&]() {
+    const SuperBlock& sb = *static_cast<SuperBlock*>(sbp);
+    WorkList wl = compile_worklist(sb, (int64_t)1 << 22);
+    std::vector<double> arena(wl.arena_size);
+    for (const auto& kv : wl.arena_off) std::memcpy(arena.data() + kv.second, kv.first->data.data(), 8 * kv.first->data.size());
+    const long N = sb.wave->size;
+    std::fill(v, v + N, 0.0);
+    for (size_t ch = 0; ch < wl.g1.size(); ++ch) {
+      std::vector<double> W(wl.wsize[ch], 0.0);
+      for (const auto& g : wl.g1[ch])
+        for (int i = 0; i < g.m; ++i)
+          for (int p = 0; p < g.k; ++p) {
+            double a = g.alpha * (g.a_trans ? arena[g.a_off + (size_t)p * g.m + i] : arena[g.a_off + (size_t)i * g.k + p]);
+            for (int j = 0; j < g.n; ++j) W[g.w_off + (size_t)i * g.n + j] += a * c[g.c_off + (size_t)p * g.n + j];
+          }
+      for (const auto& vb : wl.vb[ch])
+        for (int q = 0; q < vb.count; ++q) {
+          const auto& ct = wl.ct[ch][vb.first + q];
+          const double* S = (ct.src_kind == 0 ? W.data() : c) + ct.src_off;
+          for (int i = 0; i < vb.m; ++i)
+            for (int j = 0; j < vb.n; ++j) {
+              double s = 0.0;
+              if (ct.b_off < 0) {
+                s = S[(size_t)i * ct.k + j];
+              } else {
+                for (int p = 0; p < ct.k; ++p) {
+                  double b = ct.b_trans ? arena[ct.b_off + (size_t)j * ct.k + p] : arena[ct.b_off + (size_t)p * vb.n + j];
+                  s += S[(size_t)i * ct.k + p] * b;
+                }
+              }
+              v[vb.v_off + (size_t)i * vb.n + j] += ct.alpha * s;
+            }
+        }
+    }
+    return 0;
+  });
+}
+
+// determinant FCI: lowest nroots of the given total spin sector (penalty on S^2)
+int orc_fci(void* ints, int nelec, int nroots, double penalty, int irrep, double* energies, double* s2) {
+  return guarded([&]() {
+    auto r = oracle::fci_lowest(*static_cast<Integrals*>(ints), nelec, nroots, penalty, 1e-9, irrep);
+    for (int i = 0; i < (int)r.energies.size(); ++i) {
+      energies[i] = r.energies[i];
+      s2[i] = r.s2[i];
+    }
+    return (int)r.energies.size();
+  });
+}
+
+// untruncated superblock L = sites [0, nL), R = sites [nL, norb)
+void* orc_exact_superblock(void* ints, int nelec, int twoS, int irrep, int nL) {
+  try {
+    const Integrals& I = *static_cast<Integrals*>(ints);
+    const int n = I.norb;
+    Screen scr;
+    Quantum target{nelec, twoS, irrep};
+    BuildSpec sp;
+    BlockPtr L = make_dot(I, 0, scr);
+    for (int l = 1; l < nL; ++l) L = build_sum_block(I, *L, *make_dot(I, l, scr), target, n - (l + 1), sp, scr);
+    BlockPtr R = make_dot(I, n - 1, scr);
+    for (int r = n - 2; r >= nL; --r) R = build_sum_block(I, *R, *make_dot(I, r, scr), target, r, sp, scr);
+    return new SuperBlock(make_superblock(I, L, R, target));
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+// the sweep driver with the oracle sigma (CPU reference run); same outputs as
+// blkh_dmrg_gpu
+int orc_dmrg_cpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
+                 const double* sched, int max_sweeps, double sweep_tol, int nthreads, int verbose,
+                 double* energies_out, double* sweep_out) {
+  return guarded([&]() {
+    const Integrals& I = *static_cast<Integrals*>(ints);
+    DmrgParams P;
+    P.nelec = nelec;
+    P.twoS = twoS;
+    P.irrep = irrep;
+    P.nroots = nroots;
+    if (weights) P.weights.assign(weights, weights + nroots);
+    for (int i = 0; i < n_sched; ++i)
+      P.schedule.push_back({(int)sched[4 * i], (int)sched[4 * i + 1], sched[4 * i + 2], sched[4 * i + 3]});
+    P.max_sweeps = max_sweeps;
+    P.sweep_tol = sweep_tol;
+    P.verbose = verbose;
+    auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {
+      return std::unique_ptr<DavidsonFunctor>(new OracleFunctor(sb, nthreads));
+    });
+    for (int r = 0; r < nroots; ++r) energies_out[r] = res.energies[r];
+    for (size_t i = 0; i < res.sweeps.size(); ++i) {
+      const auto& s = res.sweeps[i];
+      double* o = sweep_out + 8 * i;
+      o[0] = s.sweep;
+      o[1] = s.forward;
+      o[2] = s.M;
+      o[3] = s.energies[0];
+      o[4] = s.max_discarded;
+      o[5] = s.seconds;
+      o[6] = s.sigma_seconds;
+      o[7] = s.sigma_flops;
+    }
+    return (int)res.sweeps.size();
+  });
+}
+
+}  // extern "C"

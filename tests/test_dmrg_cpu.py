@@ -1,0 +1,27 @@
+"""Sweep driver + Davidson restatement with the oracle sigma, pinned by the
+reference's regression energies (dmrg_tests/runtest) and exact diagonalisation."""
+import os
+
+import pytest
+
+import _oracle
+import block_amd
+from helpers import config_fcidump
+
+SCHED = [(0, 50, 1e-6, 1e-5), (2, 100, 1e-8, 1e-6), (6, 100, 1e-12, 0.0)]
+
+
+def test_hubbard_ring_two_roots(golden):
+    # dmrg_tests/hubbard: nroots 2, weights 0.5 0.5; runtest:30 asserts -6.56819216
+    ints = block_amd.Integrals(os.path.join(golden, "hubbard8_ring.FCIDUMP"))
+    e, sweeps = _oracle.dmrg_cpu(ints, nroots=2, schedule=SCHED, max_sweeps=12, sweep_tol=1e-9)
+    assert e[0] == pytest.approx(-6.56819216, abs=1e-6)
+    assert e[0] == pytest.approx(-6.568192162874748, abs=1e-9)
+
+
+def test_hubbard_chain_12_sites_M100():
+    # open chain t=1 U=4, N=12, S=0: exact ground state -6.526243384454524
+    ints = config_fcidump("hubbard12")
+    e, sweeps = _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=14, sweep_tol=1e-9)
+    assert e[0] >= -6.526243384454524 - 1e-9
+    assert e[0] == pytest.approx(-6.526243384454524, abs=1e-6)

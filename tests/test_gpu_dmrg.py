@@ -1,0 +1,31 @@
+"""Sweeps with the GPU sigma as the drop-in multiply_h reproduce the
+reference fixture energies and the CPU-oracle sweep energies."""
+import os
+
+import pytest
+
+import _oracle
+import block_amd
+from helpers import config_fcidump
+
+pytestmark = pytest.mark.gpu
+SCHED = [(0, 50, 1e-6, 1e-5), (2, 100, 1e-8, 1e-6), (6, 100, 1e-12, 0.0)]
+
+
+def test_gpu_dmrg_hubbard_ring(golden):
+    print("[gpu] dmrg hubbard ring", flush=True)
+    ints = block_amd.Integrals(os.path.join(golden, "hubbard8_ring.FCIDUMP"))
+    e, sw = block_amd.dmrg(ints, nroots=2, schedule=SCHED, max_sweeps=12, sweep_tol=1e-9)
+    assert e[0] == pytest.approx(-6.56819216, abs=1e-6)
+    e_cpu, sw_cpu = _oracle.dmrg_cpu(ints, nroots=2, schedule=SCHED, max_sweeps=12, sweep_tol=1e-9)
+    assert abs(e[0] - e_cpu[0]) <= 1e-10
+    assert abs(e[1] - e_cpu[1]) <= 1e-10
+
+
+def test_gpu_dmrg_hubbard_chain12():
+    print("[gpu] dmrg hubbard chain 12", flush=True)
+    ints = config_fcidump("hubbard12")
+    e, sw = block_amd.dmrg(ints, schedule=SCHED, max_sweeps=14, sweep_tol=1e-9)
+    assert e[0] == pytest.approx(-6.526243384454524, abs=1e-6)
+    e_cpu, _ = _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=14, sweep_tol=1e-9)
+    assert abs(e[0] - e_cpu[0]) <= 1e-10
