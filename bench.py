@@ -1,0 +1,137 @@
+"""Benchmark of the Davidson sigma step (H.C) on one MI355X.
+
+A step is one H.C application (multiply_h, davidson.C:17-22) of the FP64 GPU
+path on the mid-chain two-site superblock of the 24-orbital / 24-electron C1
+synthetic Hamiltonian at M = 500.  Operator blocks
+and the wavefunction are resident in HBM before the timed region; the timed
+region is exactly --steps sigma applications bracketed by stream
+synchronisation, timed with HIP events on the sigma stream.
+
+Prints one JSON line (see the fields below); with --gpus > 1 it exits with a
+message, the path runs on one GPU.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import block_amd  # noqa: E402
+from block_amd import integrals as gen  # noqa: E402
+
+METRIC = "H·C sigma TFLOP/s + sweep time, mixed vs FP64 GPU at |ΔE|≤1e-8 Eh"
+PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+
+
+def cpu_baseline(sb, seconds_budget):
+    """The oracle's direct multiplyH restatement (OpenMP over operator pairs,
+    per-thread sigma copies) on the host cores: one full H.C of the same
+    superblock, repeated while under the time budget."""
+    path = os.path.join(ROOT, "oracle", "liboracle.so")
+    lib = ctypes.CDLL(path)
+    f = lib.orc_sigma_timed
+    dp = ctypes.POINTER(ctypes.c_double)
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_int, dp]
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(threads, 16))
+    c = np.random.default_rng(1).standard_normal(sb.wave_size)
+    v = np.zeros(sb.wave_size)
+    total, n = 0.0, 0
+    while n == 0 or total < seconds_budget:
+        t = ctypes.c_double()
+        rc = f(sb._h, c.ctypes.data_as(dp), v.ctypes.data_as(dp), threads, ctypes.byref(t))
+        if rc != 0:
+            raise RuntimeError("oracle sigma failed")
+        total += t.value
+        n += 1
+        if total > seconds_budget / 2 and n >= 1:
+            break
+    return {"value": sb.flops * n / total / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"{n} full H.C of the same superblock ({sb.flops:.3e} flop each), {total:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="syn24")
+    ap.add_argument("--M", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this path runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        return 2
+
+    cfg = gen.CONFIGS[args.config]
+    M = args.M or cfg["M"]
+    norb = cfg["norb"]
+    nsys = norb // 2 - 1
+    tmp = tempfile.mkdtemp(prefix="blkbench_")
+    fcidump = os.path.join(tmp, f"{args.config}.FCIDUMP")
+    gen.make_config_fcidump(args.config, fcidump)
+    ints = block_amd.Integrals(fcidump)
+    t0 = time.time()
+    sb = block_amd.SuperBlock(ints, cfg["nelec"], 0, 0, nsys, M)
+    t_build = time.time() - t0
+    h = block_amd.multiply_h(sb, device=0)
+    c = np.random.default_rng(0).standard_normal(sb.wave_size)
+    c /= np.linalg.norm(c)
+    wall_ms, kern_ms, launches = h.timed(c, args.warmup, args.steps)
+    flops = h.flops
+    f1, f2 = h.phase_flops()
+    ms_per_step = wall_ms / args.steps
+    value = flops * args.steps / (wall_ms * 1e-3) / 1e12
+    # dominant MFMA kernel by time (k_reduce is the ordered partial-tile sum)
+    k = 0 if kern_ms[0] >= kern_ms[1] else 1
+    kname = ["k_phase1", "k_phase2"][k]
+    kflops = [f1, f2][k]
+    achieved = kflops * args.steps / (kern_ms[k] * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_MFMA, 5), "traffic": None, "kernel": kname,
+                "kernel_ms_per_step": round(kern_ms[k] / args.steps, 4),
+                "launches_per_step": launches[k] / args.steps,
+                "kernel_ms_per_step_all": {"k_phase1": round(kern_ms[0] / args.steps, 4),
+                                           "k_phase2": round(kern_ms[1] / args.steps, 4),
+                                           "k_reduce": round(kern_ms[2] / args.steps, 4)},
+                "tflops_by_kernel": {"k_phase1": round(f1 * args.steps / max(kern_ms[0], 1e-9) / 1e9, 3),
+                                     "k_phase2": round(f2 * args.steps / max(kern_ms[1], 1e-9) / 1e9, 3)}}
+    nl, nr, nb = sb.sectors()
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: one H.C on the mid-chain two-site superblock, {norb} orbitals / "
+                               f"{cfg['nelec']} electrons, S=0, {'D2h' if cfg.get('d2h') else 'C1'}, M={M}, "
+                               f"sys={nsys} sites (Wilson-warmed blocks), FP64 sigma",
+                   "wave_size": sb.wave_size, "flop_per_step": flops, "terms": sb.nterms,
+                   "sectors_L_R_blocks": [nl, nr, nb], "gemm_pairs": h.pairs,
+                   "operator_arena_GB": round(h.arena_bytes / 1e9, 3), "superblock_build_s": round(t_build, 2)},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sb, args.cpu_seconds)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -59,6 +59,7 @@ _sig(_h, "blkh_gpu_sigma_apply", ctypes.c_int, [_vp, _dp, _dp])
 _sig(_h, "blkh_gpu_sigma_diag", ctypes.c_int, [_vp, _dp])
 _sig(_h, "blkh_gpu_sigma_flops", ctypes.c_double, [_vp])
 _sig(_h, "blkh_gpu_sigma_pairs", ctypes.c_long, [_vp])
+_sig(_h, "blkh_gpu_sigma_phase_flops", ctypes.c_int, [_vp, _dp, _dp])
 _sig(_h, "blkh_gpu_sigma_arena_bytes", ctypes.c_long, [_vp])
 _sig(_h, "blkh_gpu_sigma_timed", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _lp])
 _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
@@ -149,6 +150,11 @@ class multiply_h:
     pairs = property(lambda s: host_lib.blkh_gpu_sigma_pairs(s._h))
     arena_bytes = property(lambda s: host_lib.blkh_gpu_sigma_arena_bytes(s._h))
 
+    def phase_flops(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        host_lib.blkh_gpu_sigma_phase_flops(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
     def __call__(self, c):
         c = np.ascontiguousarray(c, dtype=np.float64)
         if c.shape != (self.n,):
@@ -165,15 +171,15 @@ class multiply_h:
         return out
 
     def timed(self, c, warmup, reps):
-        """Device-resident repeated sigma: returns (wall_ms, [phase1_ms, phase2_ms],
-        [phase1_launches, phase2_launches]) measured with HIP events on the plan stream."""
+        """Device-resident repeated sigma: returns (wall_ms, [phase1, phase2, reduce] kernel ms,
+        [launches of each]) measured with HIP events on the plan stream."""
         c = np.ascontiguousarray(c, dtype=np.float64)
         wall = ctypes.c_double()
-        km = np.zeros(2)
-        la = (ctypes.c_long * 2)()
+        km = np.zeros(3)
+        la = (ctypes.c_long * 3)()
         if host_lib.blkh_gpu_sigma_timed(self._h, _ptr(c), warmup, reps, ctypes.byref(wall), _ptr(km), la) != 0:
             raise _err()
-        return wall.value, km.tolist(), [la[0], la[1]]
+        return wall.value, km.tolist(), [la[0], la[1], la[2]]
 
     def __del__(self):
         if getattr(self, "_h", None):

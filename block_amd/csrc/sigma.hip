@@ -34,9 +34,7 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(BSG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-constexpr int TM = 64, TN = 64, TK = 16, NT = 256;
-constexpr int LDA = TK + 1;   // LDS row pitch of the A slice (doubles)
-constexpr int LDB = TN + 2;   // LDS row pitch of the B slice
+constexpr int TK = 16;  // K depth of one LDS slice
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
@@ -44,9 +42,21 @@ struct TileRef {
   int32_t item, tm, tn, pad;
 };
 
-// Accumulator fragment of one wavefront: 2x2 blocks of 16x16.  Lane layout of
-// v_mfma_f64_16x16x4f64 (checked on gfx950): A element (lane%16, lane/16),
-// B element (lane/16, lane%16), result register i holds (lane/16 + 4*i, lane%16).
+// Tile shapes: a workgroup of WR x WC wavefronts owns a (32 WR) x (32 WC)
+// output tile and every wavefront a 32x32 quarter of it, i.e. 2x2 blocks of
+// 16x16 accumulated with v_mfma_f64_16x16x4f64.  Two shapes are built:
+// 64x64 (4 waves, for large sector blocks) and 32x32 (1 wave, for the many
+// small ones); the host picks per GEMM the shape with the least padding.
+// Lane layout of v_mfma_f64_16x16x4f64 (checked on gfx950): A element
+// (lane%16, lane/16), B element (lane/16, lane%16), result register i holds
+// (lane/16 + 4*i, lane%16).
+template <int WR, int WC>
+struct Shape {
+  static constexpr int TM = 32 * WR, TN = 32 * WC, NT = 64 * WR * WC;
+  static constexpr int LDA = TK + 1;  // LDS row pitch of the A slice (doubles)
+  static constexpr int LDB = TN + 2;  // LDS row pitch of the B slice
+};
+
 struct Acc {
   dbl4 r[2][2];
 };
@@ -58,60 +68,62 @@ __device__ inline void acc_zero(Acc& a) {
 
 // Stage op(X) rows [row0,row0+TM) x cols [k0,k0+TK) into As (zero outside m x k).
 // trans=0: X is m x k row-major (ld = k); trans=1: X is k x m row-major (ld = m).
+template <class S>
 __device__ inline void load_a(double* As, const double* X, int trans, int m, int k, int row0, int k0, double alpha) {
-  const int t = threadIdx.x;
   if (!trans) {
-    const int p = t & 15;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int i = (t >> 4) + 16 * r;
+    for (int e = threadIdx.x; e < S::TM * TK; e += S::NT) {
+      int i = e / TK, p = e % TK;
       int gi = row0 + i, gp = k0 + p;
-      As[i * LDA + p] = (gi < m && gp < k) ? alpha * X[(size_t)gi * k + gp] : 0.0;
+      As[i * S::LDA + p] = (gi < m && gp < k) ? alpha * X[(size_t)gi * k + gp] : 0.0;
     }
   } else {
-    const int i = t & 63;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int p = (t >> 6) + 4 * r;
+    for (int e = threadIdx.x; e < S::TM * TK; e += S::NT) {
+      int p = e / S::TM, i = e % S::TM;
       int gi = row0 + i, gp = k0 + p;
-      As[i * LDA + p] = (gi < m && gp < k) ? alpha * X[(size_t)gp * m + gi] : 0.0;
+      As[i * S::LDA + p] = (gi < m && gp < k) ? alpha * X[(size_t)gp * m + gi] : 0.0;
     }
   }
 }
 
 // Stage B' rows [k0,k0+TK) x cols [col0,col0+TN) into Bs (zero outside k x n).
 // trans=0: B' stored k x n row-major (ld = ldb); trans=1: stored n x k (ld = k).
+template <class S>
 __device__ inline void load_b(double* Bs, const double* Y, int trans, int k, int n, int ldb, int k0, int col0) {
-  const int t = threadIdx.x;
   if (!trans) {
-    const int j = t & 63;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int p = (t >> 6) + 4 * r;
+    for (int e = threadIdx.x; e < TK * S::TN; e += S::NT) {
+      int p = e / S::TN, j = e % S::TN;
       int gp = k0 + p, gj = col0 + j;
-      Bs[p * LDB + j] = (gp < k && gj < n) ? Y[(size_t)gp * ldb + gj] : 0.0;
+      Bs[p * S::LDB + j] = (gp < k && gj < n) ? Y[(size_t)gp * ldb + gj] : 0.0;
     }
   } else {
-    const int p = t & 15;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int j = (t >> 4) + 16 * r;
+    for (int e = threadIdx.x; e < TK * S::TN; e += S::NT) {
+      int j = e / TK, p = e % TK;
       int gp = k0 + p, gj = col0 + j;
-      Bs[p * LDB + j] = (gp < k && gj < n) ? Y[(size_t)gj * k + gp] : 0.0;
+      Bs[p * S::LDB + j] = (gp < k && gj < n) ? Y[(size_t)gj * k + gp] : 0.0;
     }
   }
 }
 
+template <int WR, int WC>
+__device__ inline void wave_origin(int& wr, int& wc) {
+  const int wave = threadIdx.x >> 6;
+  wr = (wave / WC) * 32;
+  wc = (wave % WC) * 32;
+}
+
+template <int WR, int WC>
 __device__ inline void mma_slice(Acc& acc, const double* As, const double* Bs) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  using S = Shape<WR, WC>;
+  const int lane = threadIdx.x & 63;
+  int wr, wc;
+  wave_origin<WR, WC>(wr, wc);
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int kk = 0; kk < TK; kk += 4) {
-    double a0 = As[(wr + li) * LDA + kk + lk];
-    double a1 = As[(wr + 16 + li) * LDA + kk + lk];
-    double b0 = Bs[(kk + lk) * LDB + wc + li];
-    double b1 = Bs[(kk + lk) * LDB + wc + 16 + li];
+    double a0 = As[(wr + li) * S::LDA + kk + lk];
+    double a1 = As[(wr + 16 + li) * S::LDA + kk + lk];
+    double b0 = Bs[(kk + lk) * S::LDB + wc + li];
+    double b1 = Bs[(kk + lk) * S::LDB + wc + 16 + li];
     acc.r[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc.r[0][0], 0, 0, 0);
     acc.r[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc.r[0][1], 0, 0, 0);
     acc.r[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc.r[1][0], 0, 0, 0);
@@ -119,33 +131,38 @@ __device__ inline void mma_slice(Acc& acc, const double* As, const double* Bs) {
   }
 }
 
-// (row, col) inside the 64x64 tile of accumulator element (bi, bj, i)
+// (row, col) inside the workgroup tile of accumulator element (bi, bj, i)
+template <int WR, int WC>
 __device__ inline void acc_pos(int bi, int bj, int i, int& row, int& col) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  row = (wave >> 1) * 32 + bi * 16 + (lane >> 4) + 4 * i;
-  col = (wave & 1) * 32 + bj * 16 + (lane & 15);
+  const int lane = threadIdx.x & 63;
+  int wr, wc;
+  wave_origin<WR, WC>(wr, wc);
+  row = wr + bi * 16 + (lane >> 4) + 4 * i;
+  col = wc + bj * 16 + (lane & 15);
 }
 
-__global__ __launch_bounds__(NT) void k_phase1(const bsg_gemm1_t* __restrict__ items,
-                                               const TileRef* __restrict__ tiles, int ntiles,
-                                               const double* __restrict__ arena, const double* __restrict__ c,
-                                               double* __restrict__ w) {
-  __shared__ double As[TM * LDA];
-  __shared__ double Bs[TK * LDB];
+template <int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void k_phase1(const bsg_gemm1_t* __restrict__ items,
+                                                         const TileRef* __restrict__ tiles, int ntiles,
+                                                         const double* __restrict__ arena,
+                                                         const double* __restrict__ c, double* __restrict__ w) {
+  using S = Shape<WR, WC>;
+  __shared__ double As[S::TM * S::LDA];
+  __shared__ double Bs[TK * S::LDB];
   const int tid = blockIdx.x;
   if (tid >= ntiles) return;
   const TileRef tr = tiles[tid];
   const bsg_gemm1_t it = items[tr.item];
-  const int row0 = tr.tm * TM, col0 = tr.tn * TN;
+  const int row0 = tr.tm * S::TM, col0 = tr.tn * S::TN;
   const double* A = arena + it.a_off;
   const double* C = c + it.c_off;
   Acc acc;
   acc_zero(acc);
   for (int k0 = 0; k0 < it.k; k0 += TK) {
-    load_a(As, A, it.a_trans, it.m, it.k, row0, k0, it.alpha);
-    load_b(Bs, C, 0, it.k, it.n, it.n, k0, col0);
+    load_a<S>(As, A, it.a_trans, it.m, it.k, row0, k0, it.alpha);
+    load_b<S>(Bs, C, 0, it.k, it.n, it.n, k0, col0);
     __syncthreads();
-    mma_slice(acc, As, Bs);
+    mma_slice<WR, WC>(acc, As, Bs);
     __syncthreads();
   }
   double* W = w + it.w_off;
@@ -153,62 +170,107 @@ __global__ __launch_bounds__(NT) void k_phase1(const bsg_gemm1_t* __restrict__ i
     for (int bj = 0; bj < 2; ++bj)
       for (int i = 0; i < 4; ++i) {
         int r, cc;
-        acc_pos(bi, bj, i, r, cc);
+        acc_pos<WR, WC>(bi, bj, i, r, cc);
         r += row0;
         cc += col0;
         if (r < it.m && cc < it.n) W[(size_t)r * it.n + cc] = acc.r[bi][bj][i];
       }
 }
 
-__global__ __launch_bounds__(NT) void k_phase2(const bsg_vblock_t* __restrict__ vbs,
-                                               const bsg_contrib_t* __restrict__ cts,
-                                               const TileRef* __restrict__ tiles, int ntiles,
-                                               const double* __restrict__ arena, const double* __restrict__ c,
-                                               const double* __restrict__ w, double* __restrict__ v) {
-  __shared__ double As[TM * LDA];
-  __shared__ double Bs[TK * LDB];
+// A phase-2 work unit: contributions [first, first+count) of one V block.
+// p_off < 0: the unit owns the whole contribution list and accumulates into V;
+// otherwise it writes its partial tile to P[p_off...] for k_reduce.
+struct Seg {
+  int32_t vb, first, count, pad;
+  int64_t p_off;
+};
+// Ordered reduction of the partial tiles of one V block slice:
+// V[v_off + e] += sum_s P[p_off + s*mn + e] for e in [e0, e0+len), len <= 256.
+struct RedItem {
+  int64_t v_off, p_off;
+  int32_t nseg, mn, e0, len;
+};
+
+template <int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void k_phase2(const bsg_vblock_t* __restrict__ vbs,
+                                                         const bsg_contrib_t* __restrict__ cts,
+                                                         const Seg* __restrict__ segs,
+                                                         const TileRef* __restrict__ tiles, int ntiles,
+                                                         const double* __restrict__ arena,
+                                                         const double* __restrict__ c,
+                                                         const double* __restrict__ w, double* __restrict__ v,
+                                                         double* __restrict__ part) {
+  using S = Shape<WR, WC>;
+  __shared__ double As[S::TM * S::LDA];
+  __shared__ double Bs[TK * S::LDB];
   const int tid = blockIdx.x;
   if (tid >= ntiles) return;
   const TileRef tr = tiles[tid];
-  const bsg_vblock_t vb = vbs[tr.item];
-  const int row0 = tr.tm * TM, col0 = tr.tn * TN;
+  const Seg sg = segs[tr.item];
+  const bsg_vblock_t vb = vbs[sg.vb];
+  const int row0 = tr.tm * S::TM, col0 = tr.tn * S::TN;
   Acc acc;
   acc_zero(acc);
-  for (int q = 0; q < vb.count; ++q) {
-    const bsg_contrib_t ct = cts[vb.first + q];
-    const double* S = (ct.src_kind == 0 ? w : c) + ct.src_off;
+  for (int q = 0; q < sg.count; ++q) {
+    const bsg_contrib_t ct = cts[sg.first + q];
+    const double* Sx = (ct.src_kind == 0 ? w : c) + ct.src_off;
     if (ct.b_off < 0) {  // identity partner: V += alpha * S (k == n)
       for (int bi = 0; bi < 2; ++bi)
         for (int bj = 0; bj < 2; ++bj)
           for (int i = 0; i < 4; ++i) {
             int r, cc;
-            acc_pos(bi, bj, i, r, cc);
+            acc_pos<WR, WC>(bi, bj, i, r, cc);
             r += row0;
             cc += col0;
-            if (r < vb.m && cc < vb.n) acc.r[bi][bj][i] += ct.alpha * S[(size_t)r * ct.k + cc];
+            if (r < vb.m && cc < vb.n) acc.r[bi][bj][i] += ct.alpha * Sx[(size_t)r * ct.k + cc];
           }
       continue;
     }
     const double* B = arena + ct.b_off;
     for (int k0 = 0; k0 < ct.k; k0 += TK) {
-      load_a(As, S, 0, vb.m, ct.k, row0, k0, ct.alpha);
-      load_b(Bs, B, ct.b_trans, ct.k, vb.n, vb.n, k0, col0);
+      load_a<S>(As, Sx, 0, vb.m, ct.k, row0, k0, ct.alpha);
+      load_b<S>(Bs, B, ct.b_trans, ct.k, vb.n, vb.n, k0, col0);
       __syncthreads();
-      mma_slice(acc, As, Bs);
+      mma_slice<WR, WC>(acc, As, Bs);
       __syncthreads();
     }
   }
-  double* V = v + vb.v_off;
+  const bool direct = sg.p_off < 0;
+  double* out = direct ? v + vb.v_off : part + sg.p_off;
   for (int bi = 0; bi < 2; ++bi)
     for (int bj = 0; bj < 2; ++bj)
       for (int i = 0; i < 4; ++i) {
         int r, cc;
-        acc_pos(bi, bj, i, r, cc);
+        acc_pos<WR, WC>(bi, bj, i, r, cc);
         r += row0;
         cc += col0;
-        if (r < vb.m && cc < vb.n) V[(size_t)r * vb.n + cc] += acc.r[bi][bj][i];
+        if (r < vb.m && cc < vb.n) {
+          if (direct)
+            out[(size_t)r * vb.n + cc] += acc.r[bi][bj][i];
+          else
+            out[(size_t)r * vb.n + cc] = acc.r[bi][bj][i];
+        }
       }
 }
+
+__global__ __launch_bounds__(256) void k_reduce(const RedItem* __restrict__ items, const double* __restrict__ part,
+                                                double* __restrict__ v) {
+  const RedItem it = items[blockIdx.x];
+  const int e = threadIdx.x;
+  if (e >= it.len) return;
+  const int64_t x = it.e0 + e;
+  double s = 0.0;
+  for (int q = 0; q < it.nseg; ++q) s += part[it.p_off + (int64_t)q * it.mn + x];
+  v[it.v_off + x] += s;
+}
+
+// tile-shape choice: the shape with the smaller padded area (ties: 64x64)
+inline int pick_shape(int m, int n) {
+  long a64 = (long)((m + 63) / 64) * ((n + 63) / 64) * 4096;
+  long a32 = (long)((m + 31) / 32) * ((n + 31) / 32) * 1024;
+  return a32 < a64 ? 1 : 0;
+}
+inline int shape_tm(int s) { return s ? 32 : 64; }
 
 // ---------------------------------------------------------------- vector kernels
 constexpr int RB = 256, RMAXB = 1024;
@@ -289,20 +351,24 @@ struct bsg_context {
 
 struct PlanChunk {
   bsg_gemm1_t* d_g1 = nullptr;
-  TileRef* d_t1 = nullptr;
-  int n_t1 = 0;
+  TileRef* d_t1[2] = {nullptr, nullptr};  // per tile shape (0: 64x64, 1: 32x32)
+  int n_t1[2] = {0, 0};
   bsg_vblock_t* d_vb = nullptr;
   bsg_contrib_t* d_ct = nullptr;
-  TileRef* d_t2 = nullptr;
-  int n_t2 = 0;
+  Seg* d_seg = nullptr;
+  TileRef* d_t2[2] = {nullptr, nullptr};
+  int n_t2[2] = {0, 0};
+  RedItem* d_red = nullptr;
+  int n_red = 0;
 };
 
 struct bsg_plan {
   bsg_context* ctx = nullptr;
   std::vector<PlanChunk> chunks;
   double* w = nullptr;
+  double* part = nullptr;
   int64_t wave_size = 0;
-  double flops = 0.0;
+  double flops = 0.0, flops1 = 0.0, flops2 = 0.0;
 };
 
 extern "C" {
@@ -389,7 +455,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
   auto* p = new bsg_plan();
   p->ctx = ctx;
   p->wave_size = wave_size;
-  int64_t wmax = 0;
+  int64_t wmax = 0, pmax = 0;
+  const int KSEG = 512;  // contraction depth per phase-2 work unit
   auto cleanup = [&](int code, const std::string& m) {
     bsg_plan_destroy(p);
     return fail(code, m);
@@ -398,16 +465,21 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     const bsg_chunk_t& ch = chunks[ci];
     PlanChunk pc;
     // validate and tile
-    std::vector<TileRef> t1, t2;
+    std::vector<TileRef> t1[2], t2[2];
     for (int i = 0; i < ch.n_gemm1; ++i) {
       const auto& g = ch.gemm1[i];
       if (g.m <= 0 || g.n <= 0 || g.k <= 0 || g.w_off < 0 || g.w_off + (int64_t)g.m * g.n > ch.w_size ||
           g.c_off < 0 || g.c_off + (int64_t)g.k * g.n > wave_size || g.a_off < 0)
         return cleanup(BSG_ERR_ARG, "bsg_plan_create: gemm1 item out of range");
-      for (int tm = 0; tm < (g.m + TM - 1) / TM; ++tm)
-        for (int tn = 0; tn < (g.n + TN - 1) / TN; ++tn) t1.push_back({i, tm, tn, 0});
+      const int sh = pick_shape(g.m, g.n), T = shape_tm(sh);
+      for (int tm = 0; tm < (g.m + T - 1) / T; ++tm)
+        for (int tn = 0; tn < (g.n + T - 1) / T; ++tn) t1[sh].push_back({i, tm, tn, 0});
       p->flops += 2.0 * g.m * g.n * g.k;
+      p->flops1 += 2.0 * g.m * g.n * g.k;
     }
+    std::vector<Seg> segs;
+    std::vector<RedItem> reds;
+    int64_t poff = 0;
     for (int i = 0; i < ch.n_vblocks; ++i) {
       const auto& vb = ch.vblocks[i];
       if (vb.m <= 0 || vb.n <= 0 || vb.v_off < 0 || vb.v_off + (int64_t)vb.m * vb.n > wave_size || vb.first < 0 ||
@@ -418,12 +490,55 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         int64_t lim = ct.src_kind == 0 ? ch.w_size : wave_size;
         if (ct.k <= 0 || ct.src_off < 0 || ct.src_off + (int64_t)vb.m * ct.k > lim || (ct.b_off < 0 && ct.k != vb.n))
           return cleanup(BSG_ERR_ARG, "bsg_plan_create: contribution out of range");
-        if (ct.b_off >= 0) p->flops += 2.0 * vb.m * vb.n * ct.k;
+        if (ct.b_off >= 0) {
+          p->flops += 2.0 * vb.m * vb.n * ct.k;
+          p->flops2 += 2.0 * vb.m * vb.n * ct.k;
+        }
       }
-      for (int tm = 0; tm < (vb.m + TM - 1) / TM; ++tm)
-        for (int tn = 0; tn < (vb.n + TN - 1) / TN; ++tn) t2.push_back({i, tm, tn, 0});
+      // split the contribution list into units of about KSEG contraction depth
+      std::vector<std::pair<int, int>> units;
+      int start = 0, depth = 0;
+      for (int q = 0; q < vb.count; ++q) {
+        const auto& ct = ch.contribs[vb.first + q];
+        depth += ct.b_off < 0 ? TK : ct.k;
+        if (depth >= KSEG && q + 1 < vb.count) {
+          units.push_back({start, q + 1 - start});
+          start = q + 1;
+          depth = 0;
+        }
+      }
+      units.push_back({start, vb.count - start});
+      const int nseg = (int)units.size();
+      const int64_t mn = (int64_t)vb.m * vb.n;
+      for (int u = 0; u < nseg; ++u) {
+        Seg sg;
+        sg.vb = i;
+        sg.first = vb.first + units[u].first;
+        sg.count = units[u].second;
+        sg.pad = 0;
+        sg.p_off = nseg == 1 ? -1 : poff + (int64_t)u * mn;
+        int si = (int)segs.size();
+        segs.push_back(sg);
+        const int sh = pick_shape(vb.m, vb.n), T = shape_tm(sh);
+        for (int tm = 0; tm < (vb.m + T - 1) / T; ++tm)
+          for (int tn = 0; tn < (vb.n + T - 1) / T; ++tn) t2[sh].push_back({si, tm, tn, 0});
+      }
+      if (nseg > 1) {
+        for (int64_t e0 = 0; e0 < mn; e0 += 256) {
+          RedItem r;
+          r.v_off = vb.v_off;
+          r.p_off = poff;
+          r.nseg = nseg;
+          r.mn = (int32_t)mn;
+          r.e0 = (int32_t)e0;
+          r.len = (int32_t)std::min<int64_t>(256, mn - e0);
+          reds.push_back(r);
+        }
+        poff += nseg * mn;
+      }
     }
     wmax = std::max(wmax, ch.w_size);
+    pmax = std::max(pmax, poff);
     auto up = [&](void** d, const void* h, size_t bytes) -> int {
       if (bytes == 0) return BSG_OK;
       HIPCHK(hipMalloc(d, bytes));
@@ -432,21 +547,32 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     };
     int rc = BSG_OK;
     if ((rc = up((void**)&pc.d_g1, ch.gemm1, sizeof(bsg_gemm1_t) * ch.n_gemm1)) ||
-        (rc = up((void**)&pc.d_t1, t1.data(), sizeof(TileRef) * t1.size())) ||
+        (rc = up((void**)&pc.d_t1[0], t1[0].data(), sizeof(TileRef) * t1[0].size())) ||
+        (rc = up((void**)&pc.d_t1[1], t1[1].data(), sizeof(TileRef) * t1[1].size())) ||
         (rc = up((void**)&pc.d_vb, ch.vblocks, sizeof(bsg_vblock_t) * ch.n_vblocks)) ||
         (rc = up((void**)&pc.d_ct, ch.contribs, sizeof(bsg_contrib_t) * ch.n_contribs)) ||
-        (rc = up((void**)&pc.d_t2, t2.data(), sizeof(TileRef) * t2.size()))) {
+        (rc = up((void**)&pc.d_seg, segs.data(), sizeof(Seg) * segs.size())) ||
+        (rc = up((void**)&pc.d_red, reds.data(), sizeof(RedItem) * reds.size())) ||
+        (rc = up((void**)&pc.d_t2[0], t2[0].data(), sizeof(TileRef) * t2[0].size())) ||
+        (rc = up((void**)&pc.d_t2[1], t2[1].data(), sizeof(TileRef) * t2[1].size()))) {
       p->chunks.push_back(pc);
       std::string m = g_err;
       return cleanup(rc, m);
     }
-    pc.n_t1 = (int)t1.size();
-    pc.n_t2 = (int)t2.size();
+    for (int sh = 0; sh < 2; ++sh) {
+      pc.n_t1[sh] = (int)t1[sh].size();
+      pc.n_t2[sh] = (int)t2[sh].size();
+    }
+    pc.n_red = (int)reds.size();
     p->chunks.push_back(pc);
   }
   if (wmax > 0) {
     hipError_t e = hipMalloc(&p->w, sizeof(double) * wmax);
     if (e != hipSuccess) return cleanup(BSG_ERR_NOMEM, "bsg_plan_create: W scratch");
+  }
+  if (pmax > 0) {
+    hipError_t e = hipMalloc(&p->part, sizeof(double) * pmax);
+    if (e != hipSuccess) return cleanup(BSG_ERR_NOMEM, "bsg_plan_create: partial-tile scratch");
   }
   *plan = p;
   return BSG_OK;
@@ -457,17 +583,60 @@ int bsg_plan_destroy(bsg_plan* p) {
   if (p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
   for (auto& c : p->chunks) {
     (void)hipFree(c.d_g1);
-    (void)hipFree(c.d_t1);
+    (void)hipFree(c.d_t1[0]);
+    (void)hipFree(c.d_t1[1]);
     (void)hipFree(c.d_vb);
     (void)hipFree(c.d_ct);
-    (void)hipFree(c.d_t2);
+    (void)hipFree(c.d_seg);
+    (void)hipFree(c.d_red);
+    (void)hipFree(c.d_t2[0]);
+    (void)hipFree(c.d_t2[1]);
   }
   (void)hipFree(p->w);
+  (void)hipFree(p->part);
   delete p;
   return BSG_OK;
 }
 
 double bsg_plan_flops(const bsg_plan* p) { return p ? p->flops : 0.0; }
+int bsg_plan_phase_flops(const bsg_plan* p, double* f1, double* f2) {
+  if (!p || !f1 || !f2) return fail(BSG_ERR_ARG, "bsg_plan_phase_flops: bad args");
+  *f1 = p->flops1;
+  *f2 = p->flops2;
+  return BSG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Launch one chunk: phase 1 (both shapes), phase 2 (both shapes), ordered
+// reduction.  ev (optional) receives events before phase 1, phase 2, the
+// reduction and after it.
+int enqueue_chunk(hipStream_t st, const bsg_plan* p, const PlanChunk& ch, const double* arena, const double* c,
+                  double* v, hipEvent_t* ev) {
+  if (ev) HIPCHK(hipEventRecord(ev[0], st));
+  if (ch.n_t1[0] > 0)
+    hipLaunchKernelGGL((k_phase1<2, 2>), dim3(ch.n_t1[0]), dim3(256), 0, st, ch.d_g1, ch.d_t1[0], ch.n_t1[0], arena, c,
+                       p->w);
+  if (ch.n_t1[1] > 0)
+    hipLaunchKernelGGL((k_phase1<1, 1>), dim3(ch.n_t1[1]), dim3(64), 0, st, ch.d_g1, ch.d_t1[1], ch.n_t1[1], arena, c,
+                       p->w);
+  if (ev) HIPCHK(hipEventRecord(ev[1], st));
+  if (ch.n_t2[0] > 0)
+    hipLaunchKernelGGL((k_phase2<2, 2>), dim3(ch.n_t2[0]), dim3(256), 0, st, ch.d_vb, ch.d_ct, ch.d_seg, ch.d_t2[0],
+                       ch.n_t2[0], arena, c, p->w, v, p->part);
+  if (ch.n_t2[1] > 0)
+    hipLaunchKernelGGL((k_phase2<1, 1>), dim3(ch.n_t2[1]), dim3(64), 0, st, ch.d_vb, ch.d_ct, ch.d_seg, ch.d_t2[1],
+                       ch.n_t2[1], arena, c, p->w, v, p->part);
+  if (ev) HIPCHK(hipEventRecord(ev[2], st));
+  if (ch.n_red > 0) hipLaunchKernelGGL(k_reduce, dim3(ch.n_red), dim3(256), 0, st, ch.d_red, p->part, v);
+  if (ev) HIPCHK(hipEventRecord(ev[3], st));
+  HIPCHK(hipGetLastError());
+  return BSG_OK;
+}
+}  // namespace
+
+extern "C" {
 
 int bsg_sigma_apply(bsg_context* ctx, bsg_plan* p, const double* arena, const double* c, double* v, int32_t nvec,
                     int32_t precision) {
@@ -478,14 +647,10 @@ int bsg_sigma_apply(bsg_context* ctx, bsg_plan* p, const double* arena, const do
     const double* cv = c + (size_t)iv * p->wave_size;
     double* vv = v + (size_t)iv * p->wave_size;
     for (auto& ch : p->chunks) {
-      if (ch.n_t1 > 0)
-        hipLaunchKernelGGL(k_phase1, dim3(ch.n_t1), dim3(NT), 0, ctx->stream, ch.d_g1, ch.d_t1, ch.n_t1, arena, cv, p->w);
-      if (ch.n_t2 > 0)
-        hipLaunchKernelGGL(k_phase2, dim3(ch.n_t2), dim3(NT), 0, ctx->stream, ch.d_vb, ch.d_ct, ch.d_t2, ch.n_t2, arena,
-                           cv, p->w, vv);
+      int rc = enqueue_chunk(ctx->stream, p, ch, arena, cv, vv, nullptr);
+      if (rc) return rc;
     }
   }
-  HIPCHK(hipGetLastError());
   return BSG_OK;
 }
 
@@ -494,43 +659,38 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
   if (!ctx || !p || !arena || !c || !v || reps <= 0 || !wall_ms || !kernel_ms || !launches)
     return fail(BSG_ERR_ARG, "bsg_sigma_profile: bad args");
   const size_t nch = p->chunks.size();
-  std::vector<hipEvent_t> ev((size_t)reps * nch * 3 + 2);
+  std::vector<hipEvent_t> ev((size_t)reps * nch * 4 + 2);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-  size_t q = 0;
-  HIPCHK(hipEventRecord(ev[q++], ctx->stream));
+  HIPCHK(hipEventRecord(ev[0], ctx->stream));
+  size_t q = 1;
   for (int r = 0; r < reps; ++r)
     for (auto& ch : p->chunks) {
-      HIPCHK(hipEventRecord(ev[q++], ctx->stream));
-      if (ch.n_t1 > 0)
-        hipLaunchKernelGGL(k_phase1, dim3(ch.n_t1), dim3(NT), 0, ctx->stream, ch.d_g1, ch.d_t1, ch.n_t1, arena, c, p->w);
-      HIPCHK(hipEventRecord(ev[q++], ctx->stream));
-      if (ch.n_t2 > 0)
-        hipLaunchKernelGGL(k_phase2, dim3(ch.n_t2), dim3(NT), 0, ctx->stream, ch.d_vb, ch.d_ct, ch.d_t2, ch.n_t2, arena,
-                           c, p->w, v);
-      HIPCHK(hipEventRecord(ev[q++], ctx->stream));
+      int rc = enqueue_chunk(ctx->stream, p, ch, arena, c, v, &ev[q]);
+      if (rc) return rc;
+      q += 4;
     }
-  HIPCHK(hipEventRecord(ev[q++], ctx->stream));
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventSynchronize(ev[q - 1]));
+  HIPCHK(hipEventRecord(ev[q], ctx->stream));
+  HIPCHK(hipEventSynchronize(ev[q]));
   float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[q - 1]));
+  HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[q]));
   *wall_ms = ms;
-  kernel_ms[0] = kernel_ms[1] = 0.0;
-  launches[0] = launches[1] = 0;
+  for (int i = 0; i < 3; ++i) {
+    kernel_ms[i] = 0.0;
+    launches[i] = 0;
+  }
   size_t k = 1;
   for (int r = 0; r < reps; ++r)
     for (auto& ch : p->chunks) {
-      HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
-      if (ch.n_t1 > 0) {
-        kernel_ms[0] += ms;
-        launches[0]++;
+      const int present[3] = {(ch.n_t1[0] > 0) + (ch.n_t1[1] > 0), (ch.n_t2[0] > 0) + (ch.n_t2[1] > 0),
+                              ch.n_red > 0};
+      for (int i = 0; i < 3; ++i) {
+        HIPCHK(hipEventElapsedTime(&ms, ev[k + i], ev[k + i + 1]));
+        if (present[i]) {
+          kernel_ms[i] += ms;
+          launches[i] += present[i];
+        }
       }
-      HIPCHK(hipEventElapsedTime(&ms, ev[k + 1], ev[k + 2]));
-      if (ch.n_t2 > 0) {
-        kernel_ms[1] += ms;
-        launches[1]++;
-      }
-      k += 3;
+      k += 4;
     }
   for (auto& e : ev) HIPCHK(hipEventDestroy(e));
   return BSG_OK;

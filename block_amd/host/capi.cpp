@@ -118,6 +118,9 @@ int blkh_gpu_sigma_diag(void* h, double* out) {
   });
 }
 double blkh_gpu_sigma_flops(void* h) { return static_cast<GpuSigmaHandle*>(h)->sig->flops(); }
+int blkh_gpu_sigma_phase_flops(void* h, double* f1, double* f2) {
+  return bsg_plan_phase_flops(static_cast<GpuSigmaHandle*>(h)->sig->plan(), f1, f2);
+}
 long blkh_gpu_sigma_pairs(void* h) { return (long)static_cast<GpuSigmaHandle*>(h)->sig->n_pairs(); }
 long blkh_gpu_sigma_arena_bytes(void* h) { return (long)static_cast<GpuSigmaHandle*>(h)->sig->arena_bytes(); }
 // device-resident loop for timing: c is uploaded once, v accumulates `reps` sigmas.

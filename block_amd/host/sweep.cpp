@@ -36,17 +36,19 @@ std::vector<double> hf_occupation(const Integrals& I, int nelec) {
 }
 
 // Wilson-type truncation of a composite block: lowest eigenstates of its
-// Hamiltonian with the particle number held near its Hartree-Fock value.
-void wilson_rotation(const SpinBlock& B, double nbar, double lambda, int M, std::vector<std::vector<double>>& rot,
+// Hamiltonian, with the kept multiplets spread over particle numbers by a
+// Gaussian quota centred on the block's Hartree-Fock occupation (width sig),
+// so the environment offers the superblock every nearby particle number.
+void wilson_rotation(const SpinBlock& B, double nbar, double sig, int M, std::vector<std::vector<double>>& rot,
                      std::vector<int>& mkeep) {
   const StateInfo& si = *B.si;
   const int ns = si.nsec();
   std::vector<std::vector<double>> evec(ns), eval(ns);
   struct Cand {
-    double score;
+    double e;
     int q, k;
   };
-  std::vector<Cand> cands;
+  std::map<int, std::vector<Cand>> byn;
   for (int q = 0; q < ns; ++q) {
     int d = si.dim[q];
     int ib = B.ham->pat->find(q, q);
@@ -54,15 +56,45 @@ void wilson_rotation(const SpinBlock& B, double nbar, double lambda, int M, std:
     eval[q].assign(d, 0.0);
     if (ib >= 0) std::copy(B.ham->blk(ib), B.ham->blk(ib) + (size_t)d * d, evec[q].begin());
     syev(d, evec[q].data(), eval[q].data());
-    double pen = lambda * (si.q[q].n - nbar) * (si.q[q].n - nbar);
-    for (int k = 0; k < d; ++k) cands.push_back({eval[q][k] + pen, q, k});
+    for (int k = 0; k < d; ++k) byn[si.q[q].n].push_back({eval[q][k], q, k});
   }
-  std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.score < b.score; });
+  for (auto& kv : byn)
+    std::stable_sort(kv.second.begin(), kv.second.end(), [](const Cand& a, const Cand& b) { return a.e < b.e; });
+  // quotas: fill by descending weight, one state at a time proportionally
+  std::vector<std::pair<double, int>> wn;
+  for (auto& kv : byn) wn.push_back({std::exp(-0.5 * (kv.first - nbar) * (kv.first - nbar) / (sig * sig)), kv.first});
+  std::map<int, int> taken;
+  int total = 0;
+  double wsum = 0.0;
+  for (auto& x : wn) wsum += x.first;
+  for (auto& x : wn) {
+    int q = (int)std::floor(M * x.first / wsum);
+    q = std::min<int>(q, (int)byn[x.second].size());
+    taken[x.second] = q;
+    total += q;
+  }
+  std::sort(wn.begin(), wn.end(), [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+    return a.first > b.first;
+  });
+  bool progress = true;
+  while (total < M && progress) {
+    progress = false;
+    for (auto& x : wn) {
+      if (total >= M) break;
+      if (taken[x.second] < (int)byn[x.second].size()) {
+        taken[x.second]++;
+        total++;
+        progress = true;
+      }
+    }
+  }
   mkeep.assign(ns, 0);
   std::vector<std::vector<int>> picked(ns);
-  for (int i = 0; i < (int)cands.size() && i < M; ++i) picked[cands[i].q].push_back(cands[i].k);
+  for (auto& kv : byn)
+    for (int i = 0; i < taken[kv.first]; ++i) picked[kv.second[i].q].push_back(kv.second[i].k);
   rot.assign(ns, {});
   for (int q = 0; q < ns; ++q) {
+    std::sort(picked[q].begin(), picked[q].end());
     int d = si.dim[q], m = (int)picked[q].size();
     mkeep[q] = m;
     rot[q].assign((size_t)d * m, 0.0);
@@ -114,8 +146,6 @@ BlockPtr warmup_block(const Integrals& I, const std::vector<int>& order, int nsi
                       bool normal_ops, const Screen& scr) {
   const int k = I.norb;
   auto occ = hf_occupation(I, target.n);
-  double lambda = 1.0;
-  for (int i = 0; i < k; ++i) lambda = std::max(lambda, std::fabs(I.t(i, i)) + std::fabs(I.v(i, i, i, i)));
   BlockPtr B = make_dot(I, order[0], scr);
   double nbar = occ[order[0]];
   BuildSpec sp;
@@ -128,7 +158,7 @@ BlockPtr warmup_block(const Integrals& I, const std::vector<int>& order, int nsi
     BlockPtr C = build_sum_block(I, *B, *dot, target, k - (s + 1), sp, scr);
     std::vector<std::vector<double>> rot;
     std::vector<int> mkeep;
-    wilson_rotation(*C, nbar, lambda, M, rot, mkeep);
+    wilson_rotation(*C, nbar, std::max(1.0, 0.5 * std::sqrt((double)(s + 1))), M, rot, mkeep);
     Rotation R = make_rotation_info(*C->si, rot, mkeep);
     BlockPtr nb = renormalise(*C, R);
     add_rotated_comp_ops(I, *C, *B, *dot, scr, R, *nb);
@@ -175,8 +205,6 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
   {
     int Mw = P.warmup_M > 0 ? P.warmup_M : P.schedule[0].M;
     auto occ = hf_occupation(I, P.nelec);
-    double lambda = 1.0;
-    for (int i = 0; i < k; ++i) lambda = std::max(lambda, std::fabs(I.t(i, i)) + std::fabs(I.v(i, i, i, i)));
     bwd[k - 1] = make_dot(I, k - 1, scr);
     double nbar = occ[k - 1];
     for (int l = k - 2; l >= 3; --l) {
@@ -188,7 +216,7 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       BlockPtr C = build_sum_block(I, *bwd[l + 1], *dot, target, l, sp, scr);
       std::vector<std::vector<double>> rot;
       std::vector<int> mkeep;
-      wilson_rotation(*C, nbar, lambda, Mw, rot, mkeep);
+      wilson_rotation(*C, nbar, std::max(1.0, 0.5 * std::sqrt((double)(k - l))), Mw, rot, mkeep);
       Rotation R = make_rotation_info(*C->si, rot, mkeep);
       BlockPtr nb = renormalise(*C, R);
       add_rotated_comp_ops(I, *C, *bwd[l + 1], *dot, scr, R, *nb);
@@ -282,7 +310,7 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       else
         bwd[l - 1] = nsys;
       if (P.verbose > 1)
-        std::fprintf(stderr, "  sweep %d step %d  N=%ld  E0=%.12f  iters=%d  disc=%.2e\n", sweep, step, N, e[0],
+        std::fprintf(stderr, "  sweep %d step %d  N=%ld secL=%d secR=%d E0=%.12f  iters=%d  disc=%.2e\n", sweep, step, N, L->si->nsec(), R->si->nsec(), e[0],
                      st.iterations, disc);
     }
     // the sweep ends at the far edge: the last block is the single-site dot

@@ -109,6 +109,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
 int bsg_plan_destroy(bsg_plan* plan);
 /* Flops (algorithmic, GEMM pairs) of one application, for reporting. */
 double bsg_plan_flops(const bsg_plan* plan);
+/* The same split by kernel: phase 1 (W = op(A) C) and phase 2 (V += S op(B)^T). */
+int bsg_plan_phase_flops(const bsg_plan* plan, double* phase1, double* phase2);
 
 /* v += H c for nvec vectors stored back to back (stride wave_size doubles).
  * arena: device operator arena; c, v: device pointers.  precision: BSG_FP64. */
@@ -117,8 +119,8 @@ int bsg_sigma_apply(bsg_context* ctx, bsg_plan* plan, const double* arena, const
 
 /* Timing helper: runs `reps` sigma applications (FP64) on the context stream
  * with HIP events around every kernel launch.  wall_ms: whole loop;
- * kernel_ms[0], kernel_ms[1]: summed phase-1 / phase-2 kernel time;
- * launches[0..1]: number of launches of each kernel. */
+ * kernel_ms[0..2]: summed time of the phase-1, phase-2 and partial-tile
+ * reduction kernels; launches[0..2]: number of launches of each. */
 int bsg_sigma_profile(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t reps,
                       double* wall_ms, double* kernel_ms, long* launches);
 
