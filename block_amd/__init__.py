@@ -45,6 +45,8 @@ for _n in ("blkh_norb", "blkh_nelec", "blkh_isym", "blkh_ms2"):
     _sig(_h, _n, ctypes.c_int, [_vp])
 _sig(_h, "blkh_ecore", ctypes.c_double, [_vp])
 _sig(_h, "blkh_snapshot", _vp, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int])
+_sig(_h, "blkh_snapshot_dmrg", _vp, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int])
 _sig(_h, "blkh_free_superblock", None, [_vp])
 _sig(_h, "blkh_wave_size", ctypes.c_long, [_vp])
 _sig(_h, "blkh_flops", ctypes.c_double, [_vp])
@@ -115,6 +117,16 @@ class SuperBlock:
     flops = property(lambda s: host_lib.blkh_flops(s._h))
     nterms = property(lambda s: host_lib.blkh_nterms(s._h))
     operator_bytes = property(lambda s: host_lib.blkh_operator_bytes(s._h))
+
+    @classmethod
+    def from_dmrg_warmup(cls, ints, nelec, twoS, irrep, M, nroots=1, device=0, verbose=0):
+        """The mid-chain superblock reached by the infinite-system DMRG warm-up:
+        both halves truncated to M by density matrices of the partial-chain
+        ground states (Davidson sigma on the GPU)."""
+        h = host_lib.blkh_snapshot_dmrg(ints._h, nelec, twoS, irrep, nroots, M, device, verbose)
+        if not h:
+            raise _err()
+        return cls(ints, nelec, twoS, irrep, 0, M, _handle=h)
 
     def sectors(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

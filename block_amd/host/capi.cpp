@@ -67,6 +67,25 @@ void* blkh_snapshot(void* ints, int nelec, int twoS, int irrep, int nsys, int M)
     return new SuperBlock(snapshot_superblock(I, Quantum{nelec, twoS, irrep}, nsys, M, scr));
   });
 }
+// mid-chain superblock from the infinite-system DMRG warm-up (density-matrix
+// truncation of both halves to M, Davidson sigma on the GPU)
+void* blkh_snapshot_dmrg(void* ints, int nelec, int twoS, int irrep, int nroots, int M, int device, int verbose) {
+  return guarded_ptr([&]() -> void* {
+    const Integrals& I = *static_cast<Integrals*>(ints);
+    DmrgParams P;
+    P.nelec = nelec;
+    P.twoS = twoS;
+    P.irrep = irrep;
+    P.nroots = nroots;
+    P.schedule = {{0, M, 1e-7, 1e-6}};
+    P.verbose = verbose;
+    GpuContext ctx(device);
+    SuperBlock sb = dmrg_snapshot(I, P, [&](const SuperBlock& s) {
+      return std::unique_ptr<DavidsonFunctor>(new GpuFunctor(ctx, s));
+    });
+    return new SuperBlock(std::move(sb));
+  });
+}
 void blkh_free_superblock(void* h) { delete static_cast<SuperBlock*>(h); }
 long blkh_wave_size(void* h) { return static_cast<SuperBlock*>(h)->wave->size; }
 double blkh_flops(void* h) { return sigma_flops(*static_cast<SuperBlock*>(h)); }

@@ -186,6 +186,136 @@ SuperBlock snapshot_superblock(const Integrals& I, const Quantum& target, int ns
   return make_superblock(I, L, R, target);
 }
 
+namespace {
+struct Solved {
+  std::vector<std::vector<double>> b;  // wave solutions
+  std::vector<double> e;               // energies
+  DavidsonStats st;
+};
+
+// Solver::solve_wavefunction (solver.C:19-143): diagonal, guess, block Davidson
+Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, const SigmaFactory& make_sigma,
+                        std::mt19937_64& rng) {
+  std::normal_distribution<double> gauss(0.0, 1.0);
+  const long N = sb.wave->size;
+  if (N < P.nroots) throw std::runtime_error("run_dmrg: superblock smaller than the number of roots");
+  auto hfun = make_sigma(sb);
+  std::vector<double> diag = diagonal_h(sb);
+  // guess: lowest diagonal elements with a little seeded noise
+  std::vector<long> ord(N);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::partial_sort(ord.begin(), ord.begin() + P.nroots, ord.end(), [&](long a, long b) { return diag[a] < diag[b]; });
+  Solved out;
+  out.b.assign(P.nroots, std::vector<double>(N));
+  for (int r = 0; r < P.nroots; ++r) {
+    for (long x = 0; x < N; ++x) out.b[r][x] = 1e-3 * gauss(rng);
+    out.b[r][ord[r]] += 1.0;
+  }
+  out.e = diag;
+  block_davidson(out.b, out.e, tol, *hfun, true, P.deflation_min, std::max(P.deflation_max, P.nroots + 2), &out.st,
+                 P.davidson_max_iter);
+  out.e.resize(P.nroots);
+  return out;
+}
+
+// reduced density matrix of the left (left=true) or right half, state
+// averaged (makedensitymatrix, density.C:27-90), plus seeded random noise
+std::vector<std::vector<double>> density_matrix(const SuperBlock& sb, const Solved& sol,
+                                                const std::vector<double>& weights, bool left, double noise,
+                                                std::mt19937_64& rng) {
+  std::normal_distribution<double> gauss(0.0, 1.0);
+  const StateInfo& si = left ? *sb.wave->L : *sb.wave->R;
+  std::vector<std::vector<double>> rho(si.nsec());
+  for (int q = 0; q < si.nsec(); ++q) rho[q].assign((size_t)si.dim[q] * si.dim[q], 0.0);
+  for (size_t r = 0; r < sol.b.size(); ++r)
+    for (const auto& bk : sb.wave->blocks) {
+      const double* C = sol.b[r].data() + bk.off;
+      if (left)
+        dgemm(false, true, bk.rows, bk.rows, bk.cols, weights[r], C, bk.cols, C, bk.cols, 1.0, rho[bk.l].data(), bk.rows);
+      else
+        dgemm(true, false, bk.cols, bk.cols, bk.rows, weights[r], C, bk.cols, C, bk.cols, 1.0, rho[bk.r].data(), bk.cols);
+    }
+  if (noise > 0.0) {
+    for (int q = 0; q < si.nsec(); ++q) {
+      int d = si.dim[q];
+      std::vector<double> G((size_t)d * d);
+      for (auto& x : G) x = gauss(rng);
+      dgemm(false, true, d, d, d, noise / ((double)d * d), G.data(), d, G.data(), d, 1.0, rho[q].data(), d);
+    }
+  }
+  return rho;
+}
+
+BlockPtr truncate_block(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
+                        const std::vector<std::vector<double>>& rho, int M, const Screen& scr, double* disc) {
+  std::vector<std::vector<double>> rot;
+  std::vector<int> mkeep;
+  *disc = make_rotation(*B.si, rho, M, rot, mkeep);
+  Rotation Rot = make_rotation_info(*B.si, rot, mkeep);
+  BlockPtr nb = renormalise(B, Rot);
+  if (!B.has_comp) add_rotated_comp_ops(I, B, B1, dot, scr, Rot, *nb);
+  return nb;
+}
+
+// target of a partial chain of kp of the k orbitals (infinite-system growth)
+Quantum partial_target(const Quantum& t, int kp, int k) {
+  if (kp >= k) return t;
+  int n = (int)std::lround((double)t.n * kp / k);
+  if ((n & 1) != (t.s2 & 1)) n += (n < t.n) ? 1 : -1;
+  n = std::max(n, t.s2);
+  n = std::min(n, 2 * kp - t.s2);
+  return Quantum{n, t.s2, t.irrep};
+}
+}  // namespace
+
+// Infinite-system growth from both ends to the middle: every step solves the
+// partial superblock sys (x) dot | dot (x) env and truncates both halves by
+// their density matrices.  Fills fwd[0..kh-1] and bwd[k-kh..k-1]; returns the
+// last (full-chain, mid-point) superblock.
+SuperBlock infinite_warmup(const Integrals& I, const DmrgParams& P, const SigmaFactory& make_sigma,
+                           std::vector<BlockPtr>& fwd, std::vector<BlockPtr>& bwd, std::mt19937_64& rng) {
+  const int k = I.norb;
+  const Quantum target{P.nelec, P.twoS, P.irrep};
+  Screen scr;
+  std::vector<double> weights = P.weights;
+  if ((int)weights.size() != P.nroots) weights.assign(P.nroots, 1.0 / P.nroots);
+  const int M = P.warmup_M > 0 ? P.warmup_M : P.schedule[0].M;
+  const double tol = P.schedule[0].davidson_tol, noise = P.schedule[0].noise;
+  fwd[0] = make_dot(I, 0, scr);
+  bwd[k - 1] = make_dot(I, k - 1, scr);
+  SuperBlock last;
+  for (int s = 0; 2 * (s + 2) <= k; ++s) {
+    const int kp = 2 * (s + 2);
+    const Quantum tp = partial_target(target, kp, k);
+    auto dot1 = make_dot(I, s + 1, scr);
+    auto dot2 = make_dot(I, k - 2 - s, scr);
+    BuildSpec sl;
+    sl.normal = true;
+    sl.comp = false;
+    // R carries both sets: its complementary operators serve this superblock,
+    // its normal ones the backward sweep that will use it as a system block
+    BuildSpec sr;
+    sr.normal = true;
+    sr.comp = true;
+    BlockPtr L = build_sum_block(I, *fwd[s], *dot1, target, k - (s + 2), sl, scr);
+    BlockPtr R = build_sum_block(I, *bwd[k - 1 - s], *dot2, target, k - (s + 2), sr, scr);
+    SuperBlock sb = make_superblock(I, L, R, tp);
+    if (sb.wave->size < P.nroots) throw std::runtime_error("infinite_warmup: partial superblock too small");
+    Solved sol = solve_superblock(sb, P, tol, make_sigma, rng);
+    if (P.verbose > 1)
+      std::fprintf(stderr, "  warmup step %d  sites %d  N=%d  wave=%ld  E0=%.12f iters=%d\n", s, kp, tp.n,
+                   sb.wave->size, sol.e[0], sol.st.iterations);
+    double d1 = 0, d2 = 0;
+    auto rhoL = density_matrix(sb, sol, weights, true, noise, rng);
+    auto rhoR = density_matrix(sb, sol, weights, false, noise, rng);
+    fwd[s + 1] = truncate_block(I, *L, *fwd[s], *dot1, rhoL, M, scr, &d1);
+    BlockPtr Rn = truncate_block(I, *R, *bwd[k - 1 - s], *dot2, rhoR, M, scr, &d2);
+    bwd[k - 2 - s] = Rn;
+    if (kp == k || 2 * (s + 3) > k) last = sb;
+  }
+  return last;
+}
+
 DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory& make_sigma) {
   const int k = I.norb;
   if (k < 4) throw std::runtime_error("run_dmrg: need at least 4 orbitals for two-site sweeps");
@@ -195,40 +325,19 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
   std::vector<double> weights = P.weights;
   if ((int)weights.size() != P.nroots) weights.assign(P.nroots, 1.0 / P.nroots);
   std::mt19937_64 rng(P.seed);
-  std::normal_distribution<double> gauss(0.0, 1.0);
   DmrgResult res;
 
   std::vector<BlockPtr> fwd(k), bwd(k);  // fwd[l]: sites 0..l ; bwd[l]: sites l..k-1
-  std::vector<int> rorder(k);
-  for (int i = 0; i < k; ++i) rorder[i] = k - 1 - i;
-  // environment warm-up (right to left)
-  {
-    int Mw = P.warmup_M > 0 ? P.warmup_M : P.schedule[0].M;
-    auto occ = hf_occupation(I, P.nelec);
-    bwd[k - 1] = make_dot(I, k - 1, scr);
-    double nbar = occ[k - 1];
-    for (int l = k - 2; l >= 3; --l) {
-      auto dot = make_dot(I, l, scr);
-      nbar += occ[l];
-      BuildSpec sp;
-      sp.normal = false;
-      sp.comp = false;
-      BlockPtr C = build_sum_block(I, *bwd[l + 1], *dot, target, l, sp, scr);
-      std::vector<std::vector<double>> rot;
-      std::vector<int> mkeep;
-      wilson_rotation(*C, nbar, std::max(1.0, 0.5 * std::sqrt((double)(k - l))), Mw, rot, mkeep);
-      Rotation R = make_rotation_info(*C->si, rot, mkeep);
-      BlockPtr nb = renormalise(*C, R);
-      add_rotated_comp_ops(I, *C, *bwd[l + 1], *dot, scr, R, *nb);
-      bwd[l] = nb;
-    }
-  }
-  fwd[0] = make_dot(I, 0, scr);
+  double tw = now_s();
+  infinite_warmup(I, P, make_sigma, fwd, bwd, rng);
+  int fwd_max = 0;
+  while (fwd_max + 1 < k && fwd[fwd_max + 1]) ++fwd_max;
+  const int l0 = fwd_max - 1;  // first forward step (env = bwd[l0 + 3] exists)
+  if (P.verbose) std::fprintf(stderr, "warmup: infinite-system growth to %d|%d sites, %.2fs\n", fwd_max + 1,
+                              k - fwd_max - 1, now_s() - tw);
 
   double prev_full = 1e300;
-  std::vector<double> sweep_e;
   for (int sweep = 0; sweep < P.max_sweeps; ++sweep) {
-    // schedule entry for this sweep
     const ScheduleEntry* S = &P.schedule[0];
     for (const auto& e : P.schedule)
       if (sweep >= e.start_sweep) S = &e;
@@ -239,14 +348,16 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
     rec.M = S->M;
     rec.energies.assign(P.nroots, 1e300);
     double t0 = now_s();
+    const int first = (sweep == 0) ? std::max(0, l0) : 0;
     const int nsteps = k - 3;
-    for (int step = 0; step < nsteps; ++step) {
+    for (int step = first; step < nsteps; ++step) {
       // forward: sys = 0..l, dots l+1, l+2, env l+3..; backward mirrored
       int l = forward ? step : (k - 1 - step);
       int d1 = forward ? l + 1 : l - 1;
       int d2 = forward ? l + 2 : l - 2;
       BlockPtr sys = forward ? fwd[l] : bwd[l];
       BlockPtr env = forward ? bwd[l + 3] : fwd[l - 3];
+      if (!sys || !env) throw std::runtime_error("run_dmrg: missing block");
       auto dot1 = make_dot(I, d1, scr);
       auto dot2 = make_dot(I, d2, scr);
       int nL = (int)sys->sites.size() + 1, nR = (int)env->sites.size() + 1;
@@ -260,58 +371,22 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       BlockPtr R = build_sum_block(I, *env, *dot2, target, nL, sr, scr);
       SuperBlock sb = make_superblock(I, L, R, target);
       const long N = sb.wave->size;
-      if (N < P.nroots) throw std::runtime_error("run_dmrg: superblock smaller than the number of roots");
-      auto hfun = make_sigma(sb);
-      std::vector<double> diag = diagonal_h(sb);
-      // guess: lowest diagonal elements with a little seeded noise
-      std::vector<long> ord(N);
-      std::iota(ord.begin(), ord.end(), 0);
-      std::partial_sort(ord.begin(), ord.begin() + P.nroots, ord.end(),
-                        [&](long a, long b) { return diag[a] < diag[b]; });
-      std::vector<std::vector<double>> b(P.nroots, std::vector<double>(N));
-      for (int r = 0; r < P.nroots; ++r) {
-        for (long x = 0; x < N; ++x) b[r][x] = 1e-3 * gauss(rng);
-        b[r][ord[r]] += 1.0;
-      }
-      DavidsonStats st;
-      std::vector<double> e = diag;
-      block_davidson(b, e, S->davidson_tol, *hfun, true, P.deflation_min, std::max(P.deflation_max, P.nroots + 2),
-                     &st, P.davidson_max_iter);
-      hfun.reset();
-      rec.sigma_seconds += st.sigma_seconds;
-      rec.sigma_calls += st.sigma_calls;
-      rec.sigma_flops += st.sigma_calls * sigma_flops(sb);
-      for (int r = 0; r < P.nroots; ++r) rec.energies[r] = std::min(rec.energies[r], e[r]);
-      // density matrix of L (makedensitymatrix, density.C:27-90) plus noise
-      const StateInfo& ls = *sb.wave->L;
-      std::vector<std::vector<double>> rho(ls.nsec());
-      for (int q = 0; q < ls.nsec(); ++q) rho[q].assign((size_t)ls.dim[q] * ls.dim[q], 0.0);
-      for (int r = 0; r < P.nroots; ++r)
-        for (const auto& bk : sb.wave->blocks)
-          dgemm(false, true, bk.rows, bk.rows, bk.cols, weights[r], b[r].data() + bk.off, bk.cols, b[r].data() + bk.off,
-                bk.cols, 1.0, rho[bk.l].data(), bk.rows);
-      if (S->noise > 0.0) {
-        for (int q = 0; q < ls.nsec(); ++q) {
-          int d = ls.dim[q];
-          std::vector<double> G((size_t)d * d);
-          for (auto& x : G) x = gauss(rng);
-          dgemm(false, true, d, d, d, S->noise / ((double)d * d), G.data(), d, G.data(), d, 1.0, rho[q].data(), d);
-        }
-      }
-      std::vector<std::vector<double>> rot;
-      std::vector<int> mkeep;
-      double disc = make_rotation(ls, rho, S->M, rot, mkeep);
+      Solved sol = solve_superblock(sb, P, S->davidson_tol, make_sigma, rng);
+      rec.sigma_seconds += sol.st.sigma_seconds;
+      rec.sigma_calls += sol.st.sigma_calls;
+      rec.sigma_flops += sol.st.sigma_calls * sigma_flops(sb);
+      for (int r = 0; r < P.nroots; ++r) rec.energies[r] = std::min(rec.energies[r], sol.e[r]);
+      auto rho = density_matrix(sb, sol, weights, true, S->noise, rng);
+      double disc = 0.0;
+      BlockPtr nsys = truncate_block(I, *L, *sys, *dot1, rho, S->M, scr, &disc);
       rec.max_discarded = std::max(rec.max_discarded, disc);
-      Rotation Rot = make_rotation_info(*L->si, rot, mkeep);
-      BlockPtr nsys = renormalise(*L, Rot);
-      add_rotated_comp_ops(I, *L, *sys, *dot1, scr, Rot, *nsys);
       if (forward)
         fwd[l + 1] = nsys;
       else
         bwd[l - 1] = nsys;
       if (P.verbose > 1)
-        std::fprintf(stderr, "  sweep %d step %d  N=%ld secL=%d secR=%d E0=%.12f  iters=%d  disc=%.2e\n", sweep, step, N, L->si->nsec(), R->si->nsec(), e[0],
-                     st.iterations, disc);
+        std::fprintf(stderr, "  sweep %d step %d  N=%ld secL=%d secR=%d E0=%.12f  iters=%d  disc=%.2e\n", sweep, step,
+                     N, L->si->nsec(), R->si->nsec(), sol.e[0], sol.st.iterations, disc);
     }
     // the sweep ends at the far edge: the last block is the single-site dot
     if (forward)
@@ -336,6 +411,13 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
     }
   }
   return res;
+}
+
+SuperBlock dmrg_snapshot(const Integrals& I, const DmrgParams& P, const SigmaFactory& make_sigma) {
+  const int k = I.norb;
+  std::vector<BlockPtr> fwd(k), bwd(k);
+  std::mt19937_64 rng(P.seed);
+  return infinite_warmup(I, P, make_sigma, fwd, bwd, rng);
 }
 
 }  // namespace blk

@@ -55,6 +55,11 @@ using SigmaFactory = std::function<std::unique_ptr<DavidsonFunctor>(const SuperB
 
 DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory& make_sigma);
 
+// Infinite-system growth of both halves to the middle of the chain (the
+// warm-up of run_dmrg).  Returns the final mid-chain superblock, whose blocks
+// are truncated by real density matrices: the benchmark's workload.
+SuperBlock dmrg_snapshot(const Integrals& I, const DmrgParams& P, const SigmaFactory& make_sigma);
+
 // Truncation of one block's density matrix to at most M multiplets
 // (diagonalise_dm + sort_weights + assign_matrix_by_dm, rotationmat.C):
 // rot[Q] gets the kept eigenvectors as columns, mkeep[Q] their number;
