@@ -64,9 +64,14 @@ _sig(_h, "blkh_gpu_sigma_pairs", ctypes.c_long, [_vp])
 _sig(_h, "blkh_gpu_sigma_phase_flops", ctypes.c_int, [_vp, _dp, _dp])
 _sig(_h, "blkh_gpu_sigma_arena_bytes", ctypes.c_long, [_vp])
 _sig(_h, "blkh_gpu_sigma_timed", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _lp])
+_sig(_h, "blkh_gpu_sigma_set_precision", ctypes.c_int, [_vp, ctypes.c_int])
 _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
      [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
-      ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp])
+      ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp])
+
+FP64, SPLIT_BF16 = 0, 1
+SWEEP_COLS = ("sweep", "forward", "M", "E0", "discarded", "seconds", "sigma_seconds", "sigma_flops",
+              "sigma_calls", "split_calls")
 
 
 class BlockError(RuntimeError):
@@ -149,14 +154,26 @@ class multiply_h:
     """GPU sigma functor for one superblock: ``v = H c`` (davidson.h:22-29).
 
     Construction flattens the superblock into the GEMM-pair work list and makes
-    every operator block resident in HBM; calls run the FP64 MFMA kernels."""
+    every operator block resident in HBM; calls run the fused MFMA kernels in
+    FP64 or, with ``precision=SPLIT_BF16``, as split BF16 slice products."""
 
-    def __init__(self, sb, device=0, w_budget=0):
+    def __init__(self, sb, device=0, precision=FP64):
         self.sb = sb
-        self._h = host_lib.blkh_gpu_sigma_create(sb._h, device, w_budget)
+        self._h = host_lib.blkh_gpu_sigma_create(sb._h, device, 0)
         if not self._h:
             raise _err()
         self.n = sb.wave_size
+        self.precision = precision
+
+    @property
+    def precision(self):
+        return self._precision
+
+    @precision.setter
+    def precision(self, mode):
+        if host_lib.blkh_gpu_sigma_set_precision(self._h, int(mode)) != 0:
+            raise _err()
+        self._precision = int(mode)
 
     flops = property(lambda s: host_lib.blkh_gpu_sigma_flops(s._h))
     pairs = property(lambda s: host_lib.blkh_gpu_sigma_pairs(s._h))
@@ -183,8 +200,9 @@ class multiply_h:
         return out
 
     def timed(self, c, warmup, reps):
-        """Device-resident repeated sigma: returns (wall_ms, [phase1, phase2, reduce] kernel ms,
-        [launches of each]) measured with HIP events on the plan stream."""
+        """Device-resident repeated sigma in the current precision: returns
+        (wall_ms, [fused, reduce, split] kernel ms, [launches of each]) measured
+        with HIP events on the plan stream."""
         c = np.ascontiguousarray(c, dtype=np.float64)
         wall = ctypes.c_double()
         km = np.zeros(3)
@@ -205,20 +223,22 @@ def _schedule_array(schedule):
 
 
 def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=None, max_sweeps=20,
-         sweep_tol=1e-8, device=0, verbose=0):
+         sweep_tol=1e-8, device=0, verbose=0, ladder=0.0):
     """Two-site spin-adapted DMRG (Sweep::do_one, sweep.C:341) with the Davidson
     sigma on the GPU.  schedule: rows (start_sweep, M, davidson_tol, noise).
-    Returns (energies, sweeps) with sweeps rows (sweep, forward, M, E0,
-    discarded, seconds, sigma_seconds, sigma_flops)."""
+    ladder > 0: split-precision sigma until the Davidson residual norm falls
+    below it, FP64 after (convergence is always decided on FP64 sigmas).
+    Returns (energies, sweeps) with sweeps rows named by SWEEP_COLS."""
     nelec = ints.nelec if nelec is None else nelec
     irrep = ints.isym if irrep is None else irrep
     schedule = schedule or [(0, 50, 1e-6, 1e-5), (2, 100, 1e-8, 1e-6), (6, 100, 1e-12, 0.0)]
     w = np.ascontiguousarray(weights if weights is not None else np.full(nroots, 1.0 / nroots), dtype=np.float64)
     sched = _schedule_array(schedule)
     e = np.zeros(nroots)
-    sw = np.zeros(8 * max_sweeps)
+    nc = len(SWEEP_COLS)
+    sw = np.zeros(nc * max_sweeps)
     n = host_lib.blkh_dmrg_gpu(ints._h, nelec, twoS, irrep, nroots, _ptr(w), len(schedule), _ptr(sched), max_sweeps,
-                               sweep_tol, device, verbose, _ptr(e), _ptr(sw))
+                               sweep_tol, float(ladder), device, verbose, _ptr(e), _ptr(sw))
     if n < 0:
         raise _err()
-    return e, sw[: 8 * n].reshape(n, 8)
+    return e, sw[: nc * n].reshape(n, nc)

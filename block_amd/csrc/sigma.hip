@@ -1,16 +1,18 @@
 // MI355X (gfx950) kernels of the Davidson sigma step and the C-ABI around them.
 //
-// One H.C is a list of sector GEMM pairs (the flattened two-operator
-// TensorMultiply of operatorfunctions.C:485-537):
-//   phase 1:  W_t   = alpha_t op(A_t) C_t            (one GEMM per item)
-//   phase 2:  V_blk += sum_c alpha_c S_c op(B_c)^T   (all contributions of a
-//                                                      V block, in list order)
-// Both phases are grouped GEMMs over ragged shapes, tiled 64x64 per 256-thread
-// workgroup; each wavefront owns a 32x32 quarter computed with the FP64
-// matrix-core instruction v_mfma_f64_16x16x4f64.  Operands are staged through
-// LDS in 16-deep K slices.  A phase-2 tile belongs to exactly one workgroup,
-// which walks that block's contributions in order, so V is reduced without
-// atomics and the result is bitwise reproducible.
+// One H.C is a list of GEMM pairs (the flattened two-operator TensorMultiply
+// of operatorfunctions.C:485-537), grouped by the V block they update:
+//   W = alpha_a op(A) C        (first GEMM, operatorfunctions.C:512-516)
+//   V += alpha W op(B)^T       (second GEMM, operatorfunctions.C:530)
+// k_sigma fuses both: a workgroup owns one V tile and walks a slice of that
+// block's pair list in order; for every pair it builds W in LDS chunk by
+// chunk (MFMA over the A.C contraction) and immediately contracts it with
+// B into the V accumulator, so W never travels to HBM.  Tiles are 64x64
+// (4 wavefronts) or 32x32 (1 wavefront), each wavefront owning a 32x32
+// quarter accumulated with v_mfma_f64_16x16x4f64 from LDS-staged 16-deep K
+// slices.  Long pair lists are cut into slices whose partial tiles k_reduce
+// adds in list order, so V is reduced without atomics and bitwise
+// reproducibly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -111,19 +113,20 @@ __device__ inline void wave_origin(int& wr, int& wc) {
   wc = (wave % WC) * 32;
 }
 
+// acc += A (rows of this wave, TK columns starting at ka) * B (TK rows starting
+// at kb, this wave's columns); A and B in LDS with pitches lda / ldb
 template <int WR, int WC>
-__device__ inline void mma_slice(Acc& acc, const double* As, const double* Bs) {
-  using S = Shape<WR, WC>;
+__device__ inline void mma_lds(Acc& acc, const double* A, int lda, int ka, const double* B, int ldb) {
   const int lane = threadIdx.x & 63;
   int wr, wc;
   wave_origin<WR, WC>(wr, wc);
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int kk = 0; kk < TK; kk += 4) {
-    double a0 = As[(wr + li) * S::LDA + kk + lk];
-    double a1 = As[(wr + 16 + li) * S::LDA + kk + lk];
-    double b0 = Bs[(kk + lk) * S::LDB + wc + li];
-    double b1 = Bs[(kk + lk) * S::LDB + wc + 16 + li];
+    double a0 = A[(wr + li) * lda + ka + kk + lk];
+    double a1 = A[(wr + 16 + li) * lda + ka + kk + lk];
+    double b0 = B[(kk + lk) * ldb + wc + li];
+    double b1 = B[(kk + lk) * ldb + wc + 16 + li];
     acc.r[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc.r[0][0], 0, 0, 0);
     acc.r[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc.r[0][1], 0, 0, 0);
     acc.r[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc.r[1][0], 0, 0, 0);
@@ -141,45 +144,9 @@ __device__ inline void acc_pos(int bi, int bj, int i, int& row, int& col) {
   col = wc + bj * 16 + (lane & 15);
 }
 
-template <int WR, int WC>
-__global__ __launch_bounds__(64 * WR * WC) void k_phase1(const bsg_gemm1_t* __restrict__ items,
-                                                         const TileRef* __restrict__ tiles, int ntiles,
-                                                         const double* __restrict__ arena,
-                                                         const double* __restrict__ c, double* __restrict__ w) {
-  using S = Shape<WR, WC>;
-  __shared__ double As[S::TM * S::LDA];
-  __shared__ double Bs[TK * S::LDB];
-  const int tid = blockIdx.x;
-  if (tid >= ntiles) return;
-  const TileRef tr = tiles[tid];
-  const bsg_gemm1_t it = items[tr.item];
-  const int row0 = tr.tm * S::TM, col0 = tr.tn * S::TN;
-  const double* A = arena + it.a_off;
-  const double* C = c + it.c_off;
-  Acc acc;
-  acc_zero(acc);
-  for (int k0 = 0; k0 < it.k; k0 += TK) {
-    load_a<S>(As, A, it.a_trans, it.m, it.k, row0, k0, it.alpha);
-    load_b<S>(Bs, C, 0, it.k, it.n, it.n, k0, col0);
-    __syncthreads();
-    mma_slice<WR, WC>(acc, As, Bs);
-    __syncthreads();
-  }
-  double* W = w + it.w_off;
-  for (int bi = 0; bi < 2; ++bi)
-    for (int bj = 0; bj < 2; ++bj)
-      for (int i = 0; i < 4; ++i) {
-        int r, cc;
-        acc_pos<WR, WC>(bi, bj, i, r, cc);
-        r += row0;
-        cc += col0;
-        if (r < it.m && cc < it.n) W[(size_t)r * it.n + cc] = acc.r[bi][bj][i];
-      }
-}
-
-// A phase-2 work unit: contributions [first, first+count) of one V block.
-// p_off < 0: the unit owns the whole contribution list and accumulates into V;
-// otherwise it writes its partial tile to P[p_off...] for k_reduce.
+// A work unit: pairs [first, first+count) of one V block.  p_off < 0: the
+// unit owns the whole pair list and accumulates into V; otherwise it writes
+// its partial tile to P[p_off...] for k_reduce.
 struct Seg {
   int32_t vb, first, count, pad;
   int64_t p_off;
@@ -192,46 +159,94 @@ struct RedItem {
 };
 
 template <int WR, int WC>
-__global__ __launch_bounds__(64 * WR * WC) void k_phase2(const bsg_vblock_t* __restrict__ vbs,
-                                                         const bsg_contrib_t* __restrict__ cts,
-                                                         const Seg* __restrict__ segs,
-                                                         const TileRef* __restrict__ tiles, int ntiles,
-                                                         const double* __restrict__ arena,
-                                                         const double* __restrict__ c,
-                                                         const double* __restrict__ w, double* __restrict__ v,
-                                                         double* __restrict__ part) {
+struct Fused {
   using S = Shape<WR, WC>;
-  __shared__ double As[S::TM * S::LDA];
-  __shared__ double Bs[TK * S::LDB];
+  static constexpr int TJ = S::TN;         // W chunk width (second contraction)
+  static constexpr int LDC = TJ + 2;       // pitch of the C slice
+  static constexpr int LDW = TJ + 1;       // pitch of the W chunk
+  static constexpr int AB = S::TM * S::LDA + TK * LDC;  // A and C slices (GEMM 1), B slice (GEMM 2)
+  static constexpr int AB2 = TK * S::LDB;
+  static constexpr int ABSZ = AB > AB2 ? AB : AB2;
+};
+
+template <int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void k_sigma(const bsg_vblock_t* __restrict__ vbs,
+                                                        const bsg_pair_t* __restrict__ prs,
+                                                        const Seg* __restrict__ segs,
+                                                        const TileRef* __restrict__ tiles, int ntiles,
+                                                        const double* __restrict__ arena,
+                                                        const double* __restrict__ c, double* __restrict__ v,
+                                                        double* __restrict__ part) {
+  using S = Shape<WR, WC>;
+  using F = Fused<WR, WC>;
+  __shared__ double Ab[F::ABSZ];
+  __shared__ double Ws[S::TM * F::LDW];
+  double* As = Ab;
+  double* Cs = Ab + S::TM * S::LDA;
+  double* Bs = Ab;
   const int tid = blockIdx.x;
   if (tid >= ntiles) return;
   const TileRef tr = tiles[tid];
   const Seg sg = segs[tr.item];
   const bsg_vblock_t vb = vbs[sg.vb];
   const int row0 = tr.tm * S::TM, col0 = tr.tn * S::TN;
-  Acc acc;
-  acc_zero(acc);
+  Acc accV;
+  acc_zero(accV);
   for (int q = 0; q < sg.count; ++q) {
-    const bsg_contrib_t ct = cts[sg.first + q];
-    const double* Sx = (ct.src_kind == 0 ? w : c) + ct.src_off;
-    if (ct.b_off < 0) {  // identity partner: V += alpha * S (k == n)
-      for (int bi = 0; bi < 2; ++bi)
-        for (int bj = 0; bj < 2; ++bj)
-          for (int i = 0; i < 4; ++i) {
-            int r, cc;
-            acc_pos<WR, WC>(bi, bj, i, r, cc);
-            r += row0;
-            cc += col0;
-            if (r < vb.m && cc < vb.n) acc.r[bi][bj][i] += ct.alpha * Sx[(size_t)r * ct.k + cc];
+    const bsg_pair_t pr = prs[sg.first + q];
+    const double* Cb = c + pr.c_off;
+    for (int j0 = 0; j0 < pr.k2; j0 += F::TJ) {
+      if (pr.b_off < 0 && j0 != col0) continue;  // identity B': only the diagonal chunk
+      // ---- W chunk (TM x TJ) = alpha_a op(A) C[:, j0:j0+TJ], times alpha, into Ws
+      if (pr.a_off < 0) {
+        for (int e = threadIdx.x; e < S::TM * F::TJ; e += S::NT) {
+          int i = e / F::TJ, j = e % F::TJ;
+          int gi = row0 + i, gj = j0 + j;
+          Ws[i * F::LDW + j] = (gi < vb.m && gj < pr.k2) ? pr.alpha * Cb[(size_t)gi * pr.k2 + gj] : 0.0;
+        }
+      } else {
+        const double* A = arena + pr.a_off;
+        Acc accW;
+        acc_zero(accW);
+        for (int k0 = 0; k0 < pr.k1; k0 += TK) {
+          load_a<S>(As, A, pr.a_trans, vb.m, pr.k1, row0, k0, pr.alpha_a);
+          // C slice: rows [k0, k0+TK), cols [j0, j0+TJ) of the k1 x k2 block
+          for (int e = threadIdx.x; e < TK * F::TJ; e += S::NT) {
+            int p = e / F::TJ, j = e % F::TJ;
+            int gp = k0 + p, gj = j0 + j;
+            Cs[p * F::LDC + j] = (gp < pr.k1 && gj < pr.k2) ? Cb[(size_t)gp * pr.k2 + gj] : 0.0;
           }
-      continue;
-    }
-    const double* B = arena + ct.b_off;
-    for (int k0 = 0; k0 < ct.k; k0 += TK) {
-      load_a<S>(As, Sx, 0, vb.m, ct.k, row0, k0, ct.alpha);
-      load_b<S>(Bs, B, ct.b_trans, ct.k, vb.n, vb.n, k0, col0);
+          __syncthreads();
+          mma_lds<WR, WC>(accW, As, S::LDA, 0, Cs, F::LDC);
+          __syncthreads();
+        }
+        for (int bi = 0; bi < 2; ++bi)
+          for (int bj = 0; bj < 2; ++bj)
+            for (int i = 0; i < 4; ++i) {
+              int r, cc;
+              acc_pos<WR, WC>(bi, bj, i, r, cc);
+              Ws[r * F::LDW + cc] = pr.alpha * accW.r[bi][bj][i];
+            }
+      }
       __syncthreads();
-      mma_slice<WR, WC>(acc, As, Bs);
+      // ---- V tile += W chunk * B'[j0:j0+TJ, col0:col0+TN]
+      if (pr.b_off < 0) {
+        for (int bi = 0; bi < 2; ++bi)
+          for (int bj = 0; bj < 2; ++bj)
+            for (int i = 0; i < 4; ++i) {
+              int r, cc;
+              acc_pos<WR, WC>(bi, bj, i, r, cc);
+              accV.r[bi][bj][i] += Ws[r * F::LDW + cc];
+            }
+      } else {
+        const double* B = arena + pr.b_off;
+        for (int kk0 = 0; kk0 < F::TJ && j0 + kk0 < pr.k2; kk0 += TK) {
+          load_b<S>(Bs, B, pr.b_trans, pr.k2, vb.n, vb.n, j0 + kk0, col0);
+          __syncthreads();
+          mma_lds<WR, WC>(accV, Ws, F::LDW, kk0, Bs, S::LDB);
+          __syncthreads();
+        }
+      }
       __syncthreads();
     }
   }
@@ -246,9 +261,9 @@ __global__ __launch_bounds__(64 * WR * WC) void k_phase2(const bsg_vblock_t* __r
         cc += col0;
         if (r < vb.m && cc < vb.n) {
           if (direct)
-            out[(size_t)r * vb.n + cc] += acc.r[bi][bj][i];
+            out[(size_t)r * vb.n + cc] += accV.r[bi][bj][i];
           else
-            out[(size_t)r * vb.n + cc] = acc.r[bi][bj][i];
+            out[(size_t)r * vb.n + cc] = accV.r[bi][bj][i];
         }
       }
 }
@@ -262,6 +277,257 @@ __global__ __launch_bounds__(256) void k_reduce(const RedItem* __restrict__ item
   double s = 0.0;
   for (int q = 0; q < it.nseg; ++q) s += part[it.p_off + (int64_t)q * it.mn + x];
   v[it.v_off + x] += s;
+}
+
+// ---------------------------------------------------------------- split precision
+// Every FP64 operand x is held as three BF16 slices x = x0 + x1 + x2 (x0 the
+// round-to-nearest BF16 of x, x1 of the remainder, x2 of the next remainder),
+// so |x - x0 - x1 - x2| <~ 2^-24 |x|.  BF16 carries FP32's 8-bit exponent, so
+// each slice is scaled by its own exponent and no block scale is needed.  A
+// product keeps the six slice products of order <= 2^-16 (a_i b_j, i+j <= 2),
+// summed smallest first into an FP32 accumulator per 32-deep K step
+// (v_mfma_f32_16x16x32_bf16, exact BF16 products), and every step's FP32
+// result is added into FP64 accumulators.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int SK = 32;       // K depth of one BF16 MFMA step
+constexpr int SKP = SK + 8;  // LDS pitch (BF16 elements) of the k-contiguous operand rows
+
+__device__ inline void split3(double x, __bf16& h0, __bf16& h1, __bf16& h2) {
+  h0 = (__bf16)(float)x;
+  double r = x - (double)(float)h0;
+  h1 = (__bf16)(float)r;
+  r -= (double)(float)h1;
+  h2 = (__bf16)(float)r;
+}
+
+__global__ __launch_bounds__(256) void k_split3(const double* __restrict__ x, int64_t n, __bf16* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    __bf16 h0, h1, h2;
+    split3(x[i], h0, h1, h2);
+    out[i] = h0;
+    out[n + i] = h1;
+    out[2 * n + i] = h2;
+  }
+}
+
+struct Acc64 {
+  double r[2][2][4];
+};
+__device__ inline void acc64_zero(Acc64& a) {
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 4; ++k) a.r[i][j][k] = 0.0;
+}
+
+// (row, col) inside the workgroup tile of element (bi, bj, i) of a
+// v_mfma_f32_16x16x32_bf16 accumulator: register i of lane l holds
+// (4*(l/16) + i, l%16) of its 16x16 block.
+template <int WR, int WC>
+__device__ inline void acc_pos_bf16(int bi, int bj, int i, int& row, int& col) {
+  const int lane = threadIdx.x & 63;
+  int wr, wc;
+  wave_origin<WR, WC>(wr, wc);
+  row = wr + bi * 16 + 4 * (lane >> 4) + i;
+  col = wc + bj * 16 + (lane & 15);
+}
+
+__device__ inline bf16x8 ld8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+// acc += A * B over one 32-deep K step.  A: three slices (stride as) of
+// k-contiguous rows (pitch ap) starting at column ka; B: three slices (stride
+// bs) of k-contiguous columns (pitch bp).  Operand layout of
+// v_mfma_f32_16x16x32_bf16: lane l supplies A(l%16, 8(l/16)..8(l/16)+7) and
+// B(8(l/16)..8(l/16)+7, l%16).
+template <int WR, int WC>
+__device__ inline void mma_split(Acc64& acc, const __bf16* A, int ap, int as, int ka, const __bf16* B, int bp, int bs) {
+  const int lane = threadIdx.x & 63;
+  int wr, wc;
+  wave_origin<WR, WC>(wr, wc);
+  const int li = lane & 15, lk = (lane >> 4) * 8;
+  bf16x8 a[3][2], b[3][2];
+#pragma unroll
+  for (int s = 0; s < 3; ++s)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      a[s][h] = ld8(A + s * as + (wr + h * 16 + li) * ap + ka + lk);
+      b[s][h] = ld8(B + s * bs + (wc + h * 16 + li) * bp + lk);
+    }
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      f32x4 x = {0.f, 0.f, 0.f, 0.f};
+      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][bi], b[0][bj], x, 0, 0, 0);
+      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][bi], b[1][bj], x, 0, 0, 0);
+      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][bi], b[2][bj], x, 0, 0, 0);
+      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][bi], b[0][bj], x, 0, 0, 0);
+      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][bi], b[1][bj], x, 0, 0, 0);
+      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][bi], b[0][bj], x, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc.r[bi][bj][i] += (double)x[i];
+    }
+}
+
+template <int WR, int WC>
+struct SplitLds {
+  static constexpr int TM = 32 * WR, TN = 32 * WC, NT = 64 * WR * WC, TJ = TN;
+  static constexpr int WP = TJ + 8;                           // pitch of the W slices
+  static constexpr int RA = (TM > TN ? TM : TN) * SKP;        // one slice of the A (or B) stage
+  static constexpr int RC = TJ * SKP;                         // one slice of the C stage
+  static constexpr int RW = TM * WP;                          // one slice of W
+};
+
+// Split-precision twin of k_sigma: same tiles, segments and pair order; the
+// operator arena comes as three BF16 slices (asl, slice stride an) and the
+// wavefunction as three slices (csl, stride cn) next to its FP64 copy (used
+// for identity-A pairs).  alpha_a and alpha are applied in FP64 to W.
+template <int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void k_sigma_split(const bsg_vblock_t* __restrict__ vbs,
+                                                              const bsg_pair_t* __restrict__ prs,
+                                                              const Seg* __restrict__ segs,
+                                                              const TileRef* __restrict__ tiles, int ntiles,
+                                                              const __bf16* __restrict__ asl, int64_t an,
+                                                              const double* __restrict__ c,
+                                                              const __bf16* __restrict__ csl, int64_t cn,
+                                                              double* __restrict__ v, double* __restrict__ part) {
+  using L = SplitLds<WR, WC>;
+  __shared__ __attribute__((aligned(16))) __bf16 As[3 * L::RA];
+  __shared__ __attribute__((aligned(16))) __bf16 Ct[3 * L::RC];
+  __shared__ __attribute__((aligned(16))) __bf16 Ws[3 * L::RW];
+  __bf16* Bt = As;
+  const int tid = blockIdx.x;
+  if (tid >= ntiles) return;
+  const TileRef tr = tiles[tid];
+  const Seg sg = segs[tr.item];
+  const bsg_vblock_t vb = vbs[sg.vb];
+  const int row0 = tr.tm * L::TM, col0 = tr.tn * L::TN;
+  Acc64 accV;
+  acc64_zero(accV);
+  for (int q = 0; q < sg.count; ++q) {
+    const bsg_pair_t pr = prs[sg.first + q];
+    for (int j0 = 0; j0 < pr.k2; j0 += L::TJ) {
+      if (pr.b_off < 0 && j0 != col0) continue;  // identity B': only the diagonal chunk
+      if (pr.a_off < 0) {
+        // W = alpha C (rows of the tile, columns j0..j0+TJ)
+        const double* Cb = c + pr.c_off;
+        if (pr.b_off < 0) {
+          for (int bi = 0; bi < 2; ++bi)
+            for (int bj = 0; bj < 2; ++bj)
+              for (int i = 0; i < 4; ++i) {
+                int r, cc;
+                acc_pos_bf16<WR, WC>(bi, bj, i, r, cc);
+                int gi = row0 + r, gj = j0 + cc;
+                if (gi < vb.m && gj < pr.k2) accV.r[bi][bj][i] += pr.alpha * Cb[(size_t)gi * pr.k2 + gj];
+              }
+          continue;
+        }
+        for (int e = threadIdx.x; e < L::TM * L::TJ; e += L::NT) {
+          int i = e / L::TJ, j = e % L::TJ;
+          int gi = row0 + i, gj = j0 + j;
+          double w = (gi < vb.m && gj < pr.k2) ? pr.alpha * Cb[(size_t)gi * pr.k2 + gj] : 0.0;
+          __bf16 h0, h1, h2;
+          split3(w, h0, h1, h2);
+          Ws[i * L::WP + j] = h0;
+          Ws[L::RW + i * L::WP + j] = h1;
+          Ws[2 * L::RW + i * L::WP + j] = h2;
+        }
+      } else {
+        Acc64 accW;
+        acc64_zero(accW);
+        for (int k0 = 0; k0 < pr.k1; k0 += SK) {
+          // op(A) rows [row0, row0+TM) x cols [k0, k0+SK), k-contiguous
+          if (!pr.a_trans) {
+            for (int e = threadIdx.x; e < L::TM * SK; e += L::NT) {
+              int i = e / SK, p = e % SK;
+              int gi = row0 + i, gp = k0 + p;
+              bool ok = gi < vb.m && gp < pr.k1;
+              int64_t x = pr.a_off + (int64_t)gi * pr.k1 + gp;
+              for (int s = 0; s < 3; ++s) As[s * L::RA + i * SKP + p] = ok ? asl[s * an + x] : (__bf16)0.0f;
+            }
+          } else {
+            for (int e = threadIdx.x; e < L::TM * SK; e += L::NT) {
+              int p = e / L::TM, i = e % L::TM;
+              int gi = row0 + i, gp = k0 + p;
+              bool ok = gi < vb.m && gp < pr.k1;
+              int64_t x = pr.a_off + (int64_t)gp * vb.m + gi;
+              for (int s = 0; s < 3; ++s) As[s * L::RA + i * SKP + p] = ok ? asl[s * an + x] : (__bf16)0.0f;
+            }
+          }
+          // C rows [k0, k0+SK) x cols [j0, j0+TJ), stored column-wise (k-contiguous)
+          for (int e = threadIdx.x; e < SK * L::TJ; e += L::NT) {
+            int p = e / L::TJ, j = e % L::TJ;
+            int gp = k0 + p, gj = j0 + j;
+            bool ok = gp < pr.k1 && gj < pr.k2;
+            int64_t x = pr.c_off + (int64_t)gp * pr.k2 + gj;
+            for (int s = 0; s < 3; ++s) Ct[s * L::RC + j * SKP + p] = ok ? csl[s * cn + x] : (__bf16)0.0f;
+          }
+          __syncthreads();
+          mma_split<WR, WC>(accW, As, SKP, L::RA, 0, Ct, SKP, L::RC);
+          __syncthreads();
+        }
+        const double sc = pr.alpha_a * pr.alpha;
+        if (pr.b_off < 0) {
+          for (int bi = 0; bi < 2; ++bi)
+            for (int bj = 0; bj < 2; ++bj)
+              for (int i = 0; i < 4; ++i) accV.r[bi][bj][i] += sc * accW.r[bi][bj][i];
+          continue;
+        }
+        for (int bi = 0; bi < 2; ++bi)
+          for (int bj = 0; bj < 2; ++bj)
+            for (int i = 0; i < 4; ++i) {
+              int r, cc;
+              acc_pos_bf16<WR, WC>(bi, bj, i, r, cc);
+              __bf16 h0, h1, h2;
+              split3(sc * accW.r[bi][bj][i], h0, h1, h2);
+              Ws[r * L::WP + cc] = h0;
+              Ws[L::RW + r * L::WP + cc] = h1;
+              Ws[2 * L::RW + r * L::WP + cc] = h2;
+            }
+      }
+      __syncthreads();
+      // V tile += W chunk * B'[j0:j0+TJ, col0:col0+TN]
+      const __bf16* B = asl + pr.b_off;
+      for (int kk0 = 0; kk0 < L::TJ && j0 + kk0 < pr.k2; kk0 += SK) {
+        if (!pr.b_trans) {  // B' stored k2 x n
+          for (int e = threadIdx.x; e < SK * L::TN; e += L::NT) {
+            int p = e / L::TN, n = e % L::TN;
+            int gp = j0 + kk0 + p, gj = col0 + n;
+            bool ok = gp < pr.k2 && gj < vb.n;
+            int64_t x = (int64_t)gp * vb.n + gj;
+            for (int s = 0; s < 3; ++s) Bt[s * L::RA + n * SKP + p] = ok ? B[s * an + x] : (__bf16)0.0f;
+          }
+        } else {  // B' stored n x k2
+          for (int e = threadIdx.x; e < SK * L::TN; e += L::NT) {
+            int n = e / SK, p = e % SK;
+            int gp = j0 + kk0 + p, gj = col0 + n;
+            bool ok = gp < pr.k2 && gj < vb.n;
+            int64_t x = (int64_t)gj * pr.k2 + gp;
+            for (int s = 0; s < 3; ++s) Bt[s * L::RA + n * SKP + p] = ok ? B[s * an + x] : (__bf16)0.0f;
+          }
+        }
+        __syncthreads();
+        mma_split<WR, WC>(accV, Ws, L::WP, L::RW, kk0, Bt, SKP, L::RA);
+        __syncthreads();
+      }
+    }
+  }
+  const bool direct = sg.p_off < 0;
+  double* out = direct ? v + vb.v_off : part + sg.p_off;
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
+      for (int i = 0; i < 4; ++i) {
+        int r, cc;
+        acc_pos_bf16<WR, WC>(bi, bj, i, r, cc);
+        r += row0;
+        cc += col0;
+        if (r < vb.m && cc < vb.n) {
+          if (direct)
+            out[(size_t)r * vb.n + cc] += accV.r[bi][bj][i];
+          else
+            out[(size_t)r * vb.n + cc] = accV.r[bi][bj][i];
+        }
+      }
 }
 
 // tile-shape choice: the shape with the smaller padded area (ties: 64x64)
@@ -350,14 +616,11 @@ struct bsg_context {
 };
 
 struct PlanChunk {
-  bsg_gemm1_t* d_g1 = nullptr;
-  TileRef* d_t1[2] = {nullptr, nullptr};  // per tile shape (0: 64x64, 1: 32x32)
-  int n_t1[2] = {0, 0};
   bsg_vblock_t* d_vb = nullptr;
-  bsg_contrib_t* d_ct = nullptr;
+  bsg_pair_t* d_pr = nullptr;
   Seg* d_seg = nullptr;
-  TileRef* d_t2[2] = {nullptr, nullptr};
-  int n_t2[2] = {0, 0};
+  TileRef* d_t[2] = {nullptr, nullptr};  // per tile shape (0: 64x64, 1: 32x32)
+  int n_t[2] = {0, 0};
   RedItem* d_red = nullptr;
   int n_red = 0;
 };
@@ -365,10 +628,15 @@ struct PlanChunk {
 struct bsg_plan {
   bsg_context* ctx = nullptr;
   std::vector<PlanChunk> chunks;
-  double* w = nullptr;
   double* part = nullptr;
   int64_t wave_size = 0;
+  int64_t arena_extent = 0;  // doubles of operator arena the pairs reach
   double flops = 0.0, flops1 = 0.0, flops2 = 0.0;
+  // split precision: BF16 slices of the arena (3 x an) and of one wavefunction
+  const double* split_src = nullptr;
+  __bf16* aslices = nullptr;
+  int64_t an = 0;
+  __bf16* cslices = nullptr;
 };
 
 extern "C" {
@@ -455,8 +723,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
   auto* p = new bsg_plan();
   p->ctx = ctx;
   p->wave_size = wave_size;
-  int64_t wmax = 0, pmax = 0;
-  const int KSEG = 512;  // contraction depth per phase-2 work unit
+  int64_t pmax = 0;
+  const int SEG_COST = 64;  // LDS slices per work unit before its pair list is split
   auto cleanup = [&](int code, const std::string& m) {
     bsg_plan_destroy(p);
     return fail(code, m);
@@ -464,47 +732,40 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
   for (int ci = 0; ci < n_chunks; ++ci) {
     const bsg_chunk_t& ch = chunks[ci];
     PlanChunk pc;
-    // validate and tile
-    std::vector<TileRef> t1[2], t2[2];
-    for (int i = 0; i < ch.n_gemm1; ++i) {
-      const auto& g = ch.gemm1[i];
-      if (g.m <= 0 || g.n <= 0 || g.k <= 0 || g.w_off < 0 || g.w_off + (int64_t)g.m * g.n > ch.w_size ||
-          g.c_off < 0 || g.c_off + (int64_t)g.k * g.n > wave_size || g.a_off < 0)
-        return cleanup(BSG_ERR_ARG, "bsg_plan_create: gemm1 item out of range");
-      const int sh = pick_shape(g.m, g.n), T = shape_tm(sh);
-      for (int tm = 0; tm < (g.m + T - 1) / T; ++tm)
-        for (int tn = 0; tn < (g.n + T - 1) / T; ++tn) t1[sh].push_back({i, tm, tn, 0});
-      p->flops += 2.0 * g.m * g.n * g.k;
-      p->flops1 += 2.0 * g.m * g.n * g.k;
-    }
+    std::vector<TileRef> tl[2];
     std::vector<Seg> segs;
     std::vector<RedItem> reds;
     int64_t poff = 0;
     for (int i = 0; i < ch.n_vblocks; ++i) {
       const auto& vb = ch.vblocks[i];
       if (vb.m <= 0 || vb.n <= 0 || vb.v_off < 0 || vb.v_off + (int64_t)vb.m * vb.n > wave_size || vb.first < 0 ||
-          vb.count < 0 || vb.first + vb.count > ch.n_contribs)
+          vb.count < 0 || vb.first + vb.count > ch.n_pairs)
         return cleanup(BSG_ERR_ARG, "bsg_plan_create: vblock out of range");
-      for (int q = 0; q < vb.count; ++q) {
-        const auto& ct = ch.contribs[vb.first + q];
-        int64_t lim = ct.src_kind == 0 ? ch.w_size : wave_size;
-        if (ct.k <= 0 || ct.src_off < 0 || ct.src_off + (int64_t)vb.m * ct.k > lim || (ct.b_off < 0 && ct.k != vb.n))
-          return cleanup(BSG_ERR_ARG, "bsg_plan_create: contribution out of range");
-        if (ct.b_off >= 0) {
-          p->flops += 2.0 * vb.m * vb.n * ct.k;
-          p->flops2 += 2.0 * vb.m * vb.n * ct.k;
-        }
-      }
-      // split the contribution list into units of about KSEG contraction depth
+      const int sh = pick_shape(vb.m, vb.n), T = shape_tm(sh), TJ = T;
+      // validate pairs, count flops, cut the list into units of ~SEG_COST slices
       std::vector<std::pair<int, int>> units;
-      int start = 0, depth = 0;
+      int start = 0, cost = 0;
       for (int q = 0; q < vb.count; ++q) {
-        const auto& ct = ch.contribs[vb.first + q];
-        depth += ct.b_off < 0 ? TK : ct.k;
-        if (depth >= KSEG && q + 1 < vb.count) {
+        const auto& pr = ch.pairs[vb.first + q];
+        if (pr.k1 <= 0 || pr.k2 <= 0 || pr.c_off < 0 || pr.c_off + (int64_t)pr.k1 * pr.k2 > wave_size ||
+            (pr.a_off < 0 && pr.k1 != vb.m) || (pr.b_off < 0 && pr.k2 != vb.n))
+          return cleanup(BSG_ERR_ARG, "bsg_plan_create: pair out of range");
+        if (pr.a_off >= 0) p->arena_extent = std::max(p->arena_extent, pr.a_off + (int64_t)vb.m * pr.k1);
+        if (pr.b_off >= 0) p->arena_extent = std::max(p->arena_extent, pr.b_off + (int64_t)vb.n * pr.k2);
+        if (pr.a_off >= 0) {
+          p->flops += 2.0 * vb.m * pr.k1 * pr.k2;
+          p->flops1 += 2.0 * vb.m * pr.k1 * pr.k2;
+        }
+        if (pr.b_off >= 0) {
+          p->flops += 2.0 * vb.m * pr.k2 * vb.n;
+          p->flops2 += 2.0 * vb.m * pr.k2 * vb.n;
+        }
+        const int chunks_j = pr.b_off < 0 ? 1 : (pr.k2 + TJ - 1) / TJ;
+        cost += chunks_j * ((pr.a_off < 0 ? 0 : (pr.k1 + TK - 1) / TK) + (pr.b_off < 0 ? 0 : TJ / TK) + 1);
+        if (cost >= SEG_COST && q + 1 < vb.count) {
           units.push_back({start, q + 1 - start});
           start = q + 1;
-          depth = 0;
+          cost = 0;
         }
       }
       units.push_back({start, vb.count - start});
@@ -519,9 +780,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         sg.p_off = nseg == 1 ? -1 : poff + (int64_t)u * mn;
         int si = (int)segs.size();
         segs.push_back(sg);
-        const int sh = pick_shape(vb.m, vb.n), T = shape_tm(sh);
         for (int tm = 0; tm < (vb.m + T - 1) / T; ++tm)
-          for (int tn = 0; tn < (vb.n + T - 1) / T; ++tn) t2[sh].push_back({si, tm, tn, 0});
+          for (int tn = 0; tn < (vb.n + T - 1) / T; ++tn) tl[sh].push_back({si, tm, tn, 0});
       }
       if (nseg > 1) {
         for (int64_t e0 = 0; e0 < mn; e0 += 256) {
@@ -537,7 +797,6 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         poff += nseg * mn;
       }
     }
-    wmax = std::max(wmax, ch.w_size);
     pmax = std::max(pmax, poff);
     auto up = [&](void** d, const void* h, size_t bytes) -> int {
       if (bytes == 0) return BSG_OK;
@@ -546,29 +805,20 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       return BSG_OK;
     };
     int rc = BSG_OK;
-    if ((rc = up((void**)&pc.d_g1, ch.gemm1, sizeof(bsg_gemm1_t) * ch.n_gemm1)) ||
-        (rc = up((void**)&pc.d_t1[0], t1[0].data(), sizeof(TileRef) * t1[0].size())) ||
-        (rc = up((void**)&pc.d_t1[1], t1[1].data(), sizeof(TileRef) * t1[1].size())) ||
-        (rc = up((void**)&pc.d_vb, ch.vblocks, sizeof(bsg_vblock_t) * ch.n_vblocks)) ||
-        (rc = up((void**)&pc.d_ct, ch.contribs, sizeof(bsg_contrib_t) * ch.n_contribs)) ||
+    if ((rc = up((void**)&pc.d_vb, ch.vblocks, sizeof(bsg_vblock_t) * ch.n_vblocks)) ||
+        (rc = up((void**)&pc.d_pr, ch.pairs, sizeof(bsg_pair_t) * ch.n_pairs)) ||
         (rc = up((void**)&pc.d_seg, segs.data(), sizeof(Seg) * segs.size())) ||
         (rc = up((void**)&pc.d_red, reds.data(), sizeof(RedItem) * reds.size())) ||
-        (rc = up((void**)&pc.d_t2[0], t2[0].data(), sizeof(TileRef) * t2[0].size())) ||
-        (rc = up((void**)&pc.d_t2[1], t2[1].data(), sizeof(TileRef) * t2[1].size()))) {
+        (rc = up((void**)&pc.d_t[0], tl[0].data(), sizeof(TileRef) * tl[0].size())) ||
+        (rc = up((void**)&pc.d_t[1], tl[1].data(), sizeof(TileRef) * tl[1].size()))) {
       p->chunks.push_back(pc);
       std::string m = g_err;
       return cleanup(rc, m);
     }
-    for (int sh = 0; sh < 2; ++sh) {
-      pc.n_t1[sh] = (int)t1[sh].size();
-      pc.n_t2[sh] = (int)t2[sh].size();
-    }
+    pc.n_t[0] = (int)tl[0].size();
+    pc.n_t[1] = (int)tl[1].size();
     pc.n_red = (int)reds.size();
     p->chunks.push_back(pc);
-  }
-  if (wmax > 0) {
-    hipError_t e = hipMalloc(&p->w, sizeof(double) * wmax);
-    if (e != hipSuccess) return cleanup(BSG_ERR_NOMEM, "bsg_plan_create: W scratch");
   }
   if (pmax > 0) {
     hipError_t e = hipMalloc(&p->part, sizeof(double) * pmax);
@@ -582,19 +832,43 @@ int bsg_plan_destroy(bsg_plan* p) {
   if (!p) return BSG_OK;
   if (p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
   for (auto& c : p->chunks) {
-    (void)hipFree(c.d_g1);
-    (void)hipFree(c.d_t1[0]);
-    (void)hipFree(c.d_t1[1]);
     (void)hipFree(c.d_vb);
-    (void)hipFree(c.d_ct);
+    (void)hipFree(c.d_pr);
     (void)hipFree(c.d_seg);
     (void)hipFree(c.d_red);
-    (void)hipFree(c.d_t2[0]);
-    (void)hipFree(c.d_t2[1]);
+    (void)hipFree(c.d_t[0]);
+    (void)hipFree(c.d_t[1]);
   }
-  (void)hipFree(p->w);
   (void)hipFree(p->part);
+  (void)hipFree(p->aslices);
+  (void)hipFree(p->cslices);
   delete p;
+  return BSG_OK;
+}
+
+int64_t bsg_plan_arena_extent(const bsg_plan* p) { return p ? p->arena_extent : 0; }
+
+int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* p, const double* arena, int64_t n) {
+  if (!ctx || !p || !arena || n < 0) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: bad args");
+  if (p->ctx != ctx) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: plan belongs to another context");
+  if (n < p->arena_extent) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: arena smaller than the plan's extent");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (p->an != n || !p->aslices) {
+    (void)hipFree(p->aslices);
+    p->aslices = nullptr;
+    p->an = 0;
+    hipError_t e = hipMalloc(&p->aslices, sizeof(__bf16) * 3 * std::max<int64_t>(1, n));
+    if (e != hipSuccess) return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: slice arena");
+    p->an = n;
+  }
+  if (!p->cslices) {
+    hipError_t e = hipMalloc(&p->cslices, sizeof(__bf16) * 3 * std::max<int64_t>(1, p->wave_size));
+    if (e != hipSuccess) return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: wavefunction slices");
+  }
+  if (n > 0) hipLaunchKernelGGL(k_split3, dim3(grid_for(n)), dim3(256), 0, ctx->stream, arena, n, p->aslices);
+  HIPCHK(hipGetLastError());
+  p->split_src = arena;
   return BSG_OK;
 }
 
@@ -609,29 +883,47 @@ int bsg_plan_phase_flops(const bsg_plan* p, double* f1, double* f2) {
 }  // extern "C"
 
 namespace {
-// Launch one chunk: phase 1 (both shapes), phase 2 (both shapes), ordered
-// reduction.  ev (optional) receives events before phase 1, phase 2, the
-// reduction and after it.
+// Launch one chunk: the fused kernel in both tile shapes, then the ordered
+// reduction.  ev (optional) receives events before the fused kernels, before
+// the reduction and after it.
 int enqueue_chunk(hipStream_t st, const bsg_plan* p, const PlanChunk& ch, const double* arena, const double* c,
-                  double* v, hipEvent_t* ev) {
+                  double* v, hipEvent_t* ev, bool split) {
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
-  if (ch.n_t1[0] > 0)
-    hipLaunchKernelGGL((k_phase1<2, 2>), dim3(ch.n_t1[0]), dim3(256), 0, st, ch.d_g1, ch.d_t1[0], ch.n_t1[0], arena, c,
-                       p->w);
-  if (ch.n_t1[1] > 0)
-    hipLaunchKernelGGL((k_phase1<1, 1>), dim3(ch.n_t1[1]), dim3(64), 0, st, ch.d_g1, ch.d_t1[1], ch.n_t1[1], arena, c,
-                       p->w);
+  if (!split) {
+    if (ch.n_t[0] > 0)
+      hipLaunchKernelGGL((k_sigma<2, 2>), dim3(ch.n_t[0]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_t[0],
+                         ch.n_t[0], arena, c, v, p->part);
+    if (ch.n_t[1] > 0)
+      hipLaunchKernelGGL((k_sigma<1, 1>), dim3(ch.n_t[1]), dim3(64), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_t[1],
+                         ch.n_t[1], arena, c, v, p->part);
+  } else {
+    if (ch.n_t[0] > 0)
+      hipLaunchKernelGGL((k_sigma_split<2, 2>), dim3(ch.n_t[0]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg,
+                         ch.d_t[0], ch.n_t[0], p->aslices, p->an, c, p->cslices, p->wave_size, v, p->part);
+    if (ch.n_t[1] > 0)
+      hipLaunchKernelGGL((k_sigma_split<1, 1>), dim3(ch.n_t[1]), dim3(64), 0, st, ch.d_vb, ch.d_pr, ch.d_seg,
+                         ch.d_t[1], ch.n_t[1], p->aslices, p->an, c, p->cslices, p->wave_size, v, p->part);
+  }
   if (ev) HIPCHK(hipEventRecord(ev[1], st));
-  if (ch.n_t2[0] > 0)
-    hipLaunchKernelGGL((k_phase2<2, 2>), dim3(ch.n_t2[0]), dim3(256), 0, st, ch.d_vb, ch.d_ct, ch.d_seg, ch.d_t2[0],
-                       ch.n_t2[0], arena, c, p->w, v, p->part);
-  if (ch.n_t2[1] > 0)
-    hipLaunchKernelGGL((k_phase2<1, 1>), dim3(ch.n_t2[1]), dim3(64), 0, st, ch.d_vb, ch.d_ct, ch.d_seg, ch.d_t2[1],
-                       ch.n_t2[1], arena, c, p->w, v, p->part);
-  if (ev) HIPCHK(hipEventRecord(ev[2], st));
   if (ch.n_red > 0) hipLaunchKernelGGL(k_reduce, dim3(ch.n_red), dim3(256), 0, st, ch.d_red, p->part, v);
-  if (ev) HIPCHK(hipEventRecord(ev[3], st));
+  if (ev) HIPCHK(hipEventRecord(ev[2], st));
   HIPCHK(hipGetLastError());
+  return BSG_OK;
+}
+
+// split precision: the wavefunction's BF16 slices for this application
+int enqueue_csplit(hipStream_t st, const bsg_plan* p, const double* c) {
+  if (p->wave_size > 0)
+    hipLaunchKernelGGL(k_split3, dim3(grid_for(p->wave_size)), dim3(256), 0, st, c, p->wave_size, p->cslices);
+  HIPCHK(hipGetLastError());
+  return BSG_OK;
+}
+
+int check_precision(const bsg_plan* p, const double* arena, int32_t precision) {
+  if (precision == BSG_FP64) return BSG_OK;
+  if (precision != BSG_SPLIT_BF16) return fail(BSG_ERR_ARG, "unknown precision mode");
+  if (!p->aslices || p->split_src != arena)
+    return fail(BSG_ERR_ARG, "split precision needs bsg_plan_split_arena on this arena first");
   return BSG_OK;
 }
 }  // namespace
@@ -642,33 +934,43 @@ int bsg_sigma_apply(bsg_context* ctx, bsg_plan* p, const double* arena, const do
                     int32_t precision) {
   if (!ctx || !p || !arena || !c || !v || nvec <= 0) return fail(BSG_ERR_ARG, "bsg_sigma_apply: bad args");
   if (p->ctx != ctx) return fail(BSG_ERR_ARG, "bsg_sigma_apply: plan belongs to another context");
-  if (precision != BSG_FP64) return fail(BSG_ERR_UNSUPPORTED, "bsg_sigma_apply: only BSG_FP64 is implemented");
+  int rc = check_precision(p, arena, precision);
+  if (rc) return rc;
+  const bool split = precision == BSG_SPLIT_BF16;
   for (int iv = 0; iv < nvec; ++iv) {
     const double* cv = c + (size_t)iv * p->wave_size;
     double* vv = v + (size_t)iv * p->wave_size;
-    for (auto& ch : p->chunks) {
-      int rc = enqueue_chunk(ctx->stream, p, ch, arena, cv, vv, nullptr);
-      if (rc) return rc;
-    }
+    if (split && (rc = enqueue_csplit(ctx->stream, p, cv))) return rc;
+    for (auto& ch : p->chunks)
+      if ((rc = enqueue_chunk(ctx->stream, p, ch, arena, cv, vv, nullptr, split))) return rc;
   }
   return BSG_OK;
 }
 
 int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const double* c, double* v, int32_t reps,
-                      double* wall_ms, double* kernel_ms, long* launches) {
+                      int32_t precision, double* wall_ms, double* kernel_ms, long* launches) {
   if (!ctx || !p || !arena || !c || !v || reps <= 0 || !wall_ms || !kernel_ms || !launches)
     return fail(BSG_ERR_ARG, "bsg_sigma_profile: bad args");
+  int rc = check_precision(p, arena, precision);
+  if (rc) return rc;
+  const bool split = precision == BSG_SPLIT_BF16;
   const size_t nch = p->chunks.size();
-  std::vector<hipEvent_t> ev((size_t)reps * nch * 4 + 2);
+  // per rep: [split start, split end] then 3 events per chunk
+  const size_t per = 2 + nch * 3;
+  std::vector<hipEvent_t> ev((size_t)reps * per + 2);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipEventRecord(ev[0], ctx->stream));
   size_t q = 1;
-  for (int r = 0; r < reps; ++r)
+  for (int r = 0; r < reps; ++r) {
+    HIPCHK(hipEventRecord(ev[q], ctx->stream));
+    if (split && (rc = enqueue_csplit(ctx->stream, p, c))) return rc;
+    HIPCHK(hipEventRecord(ev[q + 1], ctx->stream));
+    q += 2;
     for (auto& ch : p->chunks) {
-      int rc = enqueue_chunk(ctx->stream, p, ch, arena, c, v, &ev[q]);
-      if (rc) return rc;
-      q += 4;
+      if ((rc = enqueue_chunk(ctx->stream, p, ch, arena, c, v, &ev[q], split))) return rc;
+      q += 3;
     }
+  }
   HIPCHK(hipEventRecord(ev[q], ctx->stream));
   HIPCHK(hipEventSynchronize(ev[q]));
   float ms = 0.f;
@@ -679,19 +981,25 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
     launches[i] = 0;
   }
   size_t k = 1;
-  for (int r = 0; r < reps; ++r)
+  for (int r = 0; r < reps; ++r) {
+    if (split) {
+      HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      kernel_ms[2] += ms;
+      launches[2] += 1;
+    }
+    k += 2;
     for (auto& ch : p->chunks) {
-      const int present[3] = {(ch.n_t1[0] > 0) + (ch.n_t1[1] > 0), (ch.n_t2[0] > 0) + (ch.n_t2[1] > 0),
-                              ch.n_red > 0};
-      for (int i = 0; i < 3; ++i) {
+      const int present[2] = {(ch.n_t[0] > 0) + (ch.n_t[1] > 0), ch.n_red > 0};
+      for (int i = 0; i < 2; ++i) {
         HIPCHK(hipEventElapsedTime(&ms, ev[k + i], ev[k + i + 1]));
         if (present[i]) {
           kernel_ms[i] += ms;
           launches[i] += present[i];
         }
       }
-      k += 4;
+      k += 3;
     }
+  }
   for (auto& e : ev) HIPCHK(hipEventDestroy(e));
   return BSG_OK;
 }

@@ -40,6 +40,10 @@ struct GpuFunctor : DavidsonFunctor {
   GpuSigma sig;
   GpuFunctor(GpuContext& g, const SuperBlock& sb) : sig(g, sb) {}
   void operator()(const double* c, double* v) override { sig.apply(c, v); }
+  bool set_precision(int mode) override {
+    sig.set_precision(mode);
+    return true;
+  }
 };
 }  // namespace
 
@@ -109,12 +113,12 @@ int blkh_diag_cpu(void* h, double* out) {
   });
 }
 
-void* blkh_gpu_sigma_create(void* sb, int device, long w_budget) {
+void* blkh_gpu_sigma_create(void* sb, int device, long reserved) {
   return guarded_ptr([&]() -> void* {
     auto* h = new GpuSigmaHandle();
     try {
       h->ctx.reset(new GpuContext(device));
-      h->sig.reset(new GpuSigma(*h->ctx, *static_cast<SuperBlock*>(sb), w_budget > 0 ? w_budget : (int64_t)1 << 27));
+      h->sig.reset(new GpuSigma(*h->ctx, *static_cast<SuperBlock*>(sb)));
     } catch (...) {
       delete h;
       throw;
@@ -140,11 +144,18 @@ double blkh_gpu_sigma_flops(void* h) { return static_cast<GpuSigmaHandle*>(h)->s
 int blkh_gpu_sigma_phase_flops(void* h, double* f1, double* f2) {
   return bsg_plan_phase_flops(static_cast<GpuSigmaHandle*>(h)->sig->plan(), f1, f2);
 }
+int blkh_gpu_sigma_set_precision(void* h, int mode) {
+  return guarded([&]() {
+    static_cast<GpuSigmaHandle*>(h)->sig->set_precision(mode);
+    return 0;
+  });
+}
 long blkh_gpu_sigma_pairs(void* h) { return (long)static_cast<GpuSigmaHandle*>(h)->sig->n_pairs(); }
 long blkh_gpu_sigma_arena_bytes(void* h) { return (long)static_cast<GpuSigmaHandle*>(h)->sig->arena_bytes(); }
 // device-resident loop for timing: c is uploaded once, v accumulates `reps` sigmas.
-// kernel_ms[0..1] receive the summed phase-1 / phase-2 kernel times measured
-// with HIP events on the plan's stream, wall_ms the whole loop.
+// kernel_ms[0..2] receive the summed fused-kernel / reduction / wavefunction
+// split times measured with HIP events on the plan's stream, wall_ms the
+// whole loop; the handle's current precision is used.
 int blkh_gpu_sigma_timed(void* h, const double* c, int warmup, int reps, double* wall_ms, double* kernel_ms,
                          long* launches) {
   return guarded([&]() {
@@ -155,20 +166,22 @@ int blkh_gpu_sigma_timed(void* h, const double* c, int warmup, int reps, double*
     bsg_check(bsg_memset_zero(s.ctx(), s.dev_v(), bytes), "v clear");
     for (int i = 0; i < warmup; ++i) s.apply_device(s.dev_c(), s.dev_v());
     bsg_check(bsg_synchronize(s.ctx()), "sync");
-    bsg_check(bsg_sigma_profile(s.ctx(), s.plan(), s.dev_arena(), s.dev_c(), s.dev_v(), reps, wall_ms, kernel_ms,
-                                launches),
+    bsg_check(bsg_sigma_profile(s.ctx(), s.plan(), s.dev_arena(), s.dev_c(), s.dev_v(), reps, s.precision(), wall_ms,
+                                kernel_ms, launches),
               "profile");
     return 0;
   });
 }
 
 // Two-site DMRG with the sigma on the GPU.  schedule: n_sched rows of
-// (start_sweep, M, davidson_tol, noise).  energies_out[nroots];
-// sweep_out[max_sweeps * 8]: sweep, forward, M, E0, discarded, seconds,
-// sigma_seconds, sigma_flops.  Returns the number of sweeps run.
+// (start_sweep, M, davidson_tol, noise).  ladder > 0: split-precision sigma
+// until the Davidson residual norm falls below it, FP64 after.
+// energies_out[nroots]; sweep_out[max_sweeps * 10]: sweep, forward, M, E0,
+// discarded, seconds, sigma_seconds, sigma_flops, sigma calls, split calls.
+// Returns the number of sweeps run.
 int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
-                  const double* sched, int max_sweeps, double sweep_tol, int device, int verbose, double* energies_out,
-                  double* sweep_out) {
+                  const double* sched, int max_sweeps, double sweep_tol, double ladder, int device, int verbose,
+                  double* energies_out, double* sweep_out) {
   return guarded([&]() {
     const Integrals& I = *static_cast<Integrals*>(ints);
     DmrgParams P;
@@ -182,23 +195,13 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
     P.max_sweeps = max_sweeps;
     P.sweep_tol = sweep_tol;
     P.verbose = verbose;
+    P.ladder = ladder;
     GpuContext ctx(device);
     auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {
       return std::unique_ptr<DavidsonFunctor>(new GpuFunctor(ctx, sb));
     });
     for (int r = 0; r < nroots; ++r) energies_out[r] = res.energies[r];
-    for (size_t i = 0; i < res.sweeps.size(); ++i) {
-      const auto& s = res.sweeps[i];
-      double* o = sweep_out + 8 * i;
-      o[0] = s.sweep;
-      o[1] = s.forward;
-      o[2] = s.M;
-      o[3] = s.energies[0];
-      o[4] = s.max_discarded;
-      o[5] = s.seconds;
-      o[6] = s.sigma_seconds;
-      o[7] = s.sigma_flops;
-    }
+    write_sweep_records(res, sweep_out);
     return (int)res.sweeps.size();
   });
 }

@@ -42,8 +42,9 @@ void olsen_precondition(std::vector<double>& r, const std::vector<double>& c0, d
 
 void block_davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_diag, double normtol,
                     DavidsonFunctor& h, bool useprecond, int deflation_min, int deflation_max, DavidsonStats* st,
-                    int max_iter) {
+                    int max_iter, double ladder) {
   const double levelshift = 0.0;
+  bool split = ladder > 0.0 && h.set_precision(1);
   const int nroots = (int)b.size();
   // orthonormalise the guesses (linear.C:190-200)
   for (int i = 0; i < nroots; ++i) {
@@ -61,7 +62,10 @@ void block_davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_
     for (size_t i = sigma.size(); i < b.size(); ++i) {
       sigma.emplace_back(b[i].size(), 0.0);
       h(b[i].data(), sigma[i].data());
-      if (st) st->sigma_calls++;
+      if (st) {
+        st->sigma_calls++;
+        if (split) st->sigma_calls_split++;
+      }
     }
     if (st) st->sigma_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
@@ -97,6 +101,16 @@ void block_davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_
     r = sigma[converged_roots];
     daxpy(-eval[converged_roots], b[converged_roots], r);
     double rnorm = ddot(r, r);
+    if (split && (rnorm < ladder * ladder || rnorm < normtol)) {
+      // ladder step: continue from the current Ritz vectors with FP64 sigmas
+      h.set_precision(0);
+      split = false;
+      if (st) st->ladder_switches++;
+      b.resize(nroots);
+      sigma.clear();
+      converged_roots = 0;
+      continue;
+    }
     if (useprecond) olsen_precondition(r, b[converged_roots], eval[converged_roots], diag0, levelshift);
     if (rnorm < normtol || iter >= max_iter) {
       ++converged_roots;

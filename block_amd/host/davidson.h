@@ -12,11 +12,16 @@ namespace blk {
 struct DavidsonFunctor {
   virtual ~DavidsonFunctor() = default;
   virtual void operator()(const double* c, double* v) = 0;
+  // Arithmetic of the following sigmas: 0 = FP64, 1 = split BF16.  Returns
+  // false when the functor has no such mode (the default: FP64 only).
+  virtual bool set_precision(int mode) { return mode == 0; }
 };
 
 struct DavidsonStats {
   int iterations = 0;
   int sigma_calls = 0;
+  int sigma_calls_split = 0;  // of which in split precision
+  int ladder_switches = 0;
   double sigma_seconds = 0.0;  // the reference's hmultiply timer (linear.C:220,259)
 };
 
@@ -26,9 +31,17 @@ struct DavidsonStats {
 // convergence on the squared residual norm, deflation and re-orthogonalised
 // expansion.  On return b holds the Ritz vectors and h_diag[0..nroots) the
 // eigenvalues.
+//
+// Precision ladder (ladder > 0 and a functor with split precision): sigmas
+// start in split precision; once the residual norm of the current root falls
+// below `ladder`, or it meets normtol while still in split precision, the
+// solver keeps the current Ritz vectors, drops the rest of the subspace and
+// continues with FP64 sigmas, so convergence is always decided on FP64
+// sigmas.  Subspace algebra, orthogonalisation and residuals are FP64
+// throughout.
 void block_davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_diag, double normtol,
                     DavidsonFunctor& h, bool useprecond, int deflation_min, int deflation_max, DavidsonStats* st,
-                    int max_iter = 1000);
+                    int max_iter = 1000, double ladder = 0.0);
 
 // vector kernels of BaseOperator.C:240-288 / linear.C:27-60 on host vectors
 double ddot(const std::vector<double>& x, const std::vector<double>& y);

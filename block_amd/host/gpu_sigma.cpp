@@ -14,7 +14,7 @@ GpuContext::~GpuContext() {
   if (ctx) bsg_free(ctx);
 }
 
-WorkList compile_worklist(const SuperBlock& sb, int64_t w_budget) {
+WorkList compile_worklist(const SuperBlock& sb) {
   WorkList wl;
   const WaveLayout& W = *sb.wave;
   const StateInfo &ls = *W.L, &rs = *W.R;
@@ -30,28 +30,7 @@ WorkList compile_worklist(const SuperBlock& sb, int64_t w_budget) {
         wl.arena_size += (int64_t)o->data.size();
       }
 
-  std::vector<std::vector<bsg_contrib_t>> per_v(W.blocks.size());
-  std::vector<bsg_gemm1_t> g1;
-  int64_t wcur = 0;
-  auto flush = [&]() {
-    std::vector<bsg_vblock_t> vbs;
-    std::vector<bsg_contrib_t> cts;
-    for (size_t iv = 0; iv < per_v.size(); ++iv) {
-      if (per_v[iv].empty()) continue;
-      const auto& b = W.blocks[iv];
-      vbs.push_back({b.off, b.rows, b.cols, (int32_t)cts.size(), (int32_t)per_v[iv].size()});
-      cts.insert(cts.end(), per_v[iv].begin(), per_v[iv].end());
-      per_v[iv].clear();
-    }
-    if (vbs.empty() && g1.empty()) return;
-    wl.g1.push_back(std::move(g1));
-    wl.vb.push_back(std::move(vbs));
-    wl.ct.push_back(std::move(cts));
-    wl.wsize.push_back(wcur);
-    g1.clear();
-    wcur = 0;
-  };
-
+  std::vector<std::vector<bsg_pair_t>> per_v(W.blocks.size());
   for (const auto& t : sb.terms) {
     const bool aid = (t.a.op == idL), bid = (t.b.op == idR);
     const int64_t aoff = wl.arena_off[t.a.op], boff = wl.arena_off[t.b.op];
@@ -61,27 +40,6 @@ WorkList compile_worklist(const SuperBlock& sb, int64_t w_budget) {
       for (int l = 0; l < ls.nsec(); ++l) {
         int ia = t.a.find(l, lp);
         if (ia < 0) continue;
-        const int dl = ls.dim[l], dlp = ls.dim[lp], drp = rs.dim[rp];
-        int32_t src_kind;
-        int64_t src_off;
-        if (aid) {
-          src_kind = 1;
-          src_off = cb.off;
-        } else {
-          src_kind = 0;
-          src_off = wcur;
-          bsg_gemm1_t g;
-          g.a_off = aoff + t.a.op->pat->blocks[ia].off;
-          g.c_off = cb.off;
-          g.w_off = wcur;
-          g.m = dl;
-          g.n = drp;
-          g.k = dlp;
-          g.a_trans = t.a.dag ? 1 : 0;
-          g.alpha = t.a.scale(l, lp);
-          g1.push_back(g);
-          wcur += (int64_t)dl * drp;
-        }
         for (int r = 0; r < rs.nsec(); ++r) {
           int iv = W.find(l, r);
           if (iv < 0) continue;
@@ -91,22 +49,34 @@ WorkList compile_worklist(const SuperBlock& sb, int64_t w_budget) {
           if (t.b.fermion() && (ls.q[lp].n & 1)) f = -f;
           f *= t.b.scale(r, rp);
           if (f == 0.0) continue;
-          bsg_contrib_t ct;
-          ct.src_off = src_off;
-          ct.src_kind = src_kind;
-          ct.b_off = bid ? -1 : boff + t.b.op->pat->blocks[ib].off;
-          ct.b_trans = t.b.dag ? 0 : 1;
-          ct.k = drp;
-          ct.pad = 0;
-          ct.alpha = f;
-          per_v[iv].push_back(ct);
-          ++wl.n_pairs;
+          bsg_pair_t p;
+          p.a_off = aid ? -1 : aoff + t.a.op->pat->blocks[ia].off;
+          p.c_off = cb.off;
+          p.b_off = bid ? -1 : boff + t.b.op->pat->blocks[ib].off;
+          p.k1 = ls.dim[lp];
+          p.k2 = rs.dim[rp];
+          p.a_trans = t.a.dag ? 1 : 0;
+          p.b_trans = t.b.dag ? 0 : 1;
+          p.alpha_a = aid ? 1.0 : t.a.scale(l, lp);
+          p.alpha = f;
+          per_v[iv].push_back(p);
         }
       }
     }
-    if (wcur >= w_budget) flush();
   }
-  flush();
+  // every operator block a pair names must lie inside the arena
+  for (size_t iv = 0; iv < per_v.size(); ++iv)
+    for (const auto& p : per_v[iv]) {
+      const int64_t m = W.blocks[iv].rows, n = W.blocks[iv].cols;
+      if ((p.a_off >= 0 && p.a_off + m * p.k1 > wl.arena_size) || (p.b_off >= 0 && p.b_off + n * p.k2 > wl.arena_size))
+        throw std::runtime_error("compile_worklist: operator block outside the arena");
+    }
+  for (size_t iv = 0; iv < per_v.size(); ++iv) {
+    if (per_v[iv].empty()) continue;
+    const auto& b = W.blocks[iv];
+    wl.vb.push_back({b.off, b.rows, b.cols, (int32_t)wl.pairs.size(), (int32_t)per_v[iv].size()});
+    wl.pairs.insert(wl.pairs.end(), per_v[iv].begin(), per_v[iv].end());
+  }
 
   // diagonal terms (SpinBlock::diagonalH)
   for (const auto& t : sb.terms) {
@@ -133,10 +103,10 @@ WorkList compile_worklist(const SuperBlock& sb, int64_t w_budget) {
   return wl;
 }
 
-GpuSigma::GpuSigma(GpuContext& g, const SuperBlock& sb, int64_t w_budget, int max_vec) : g_(g) {
+GpuSigma::GpuSigma(GpuContext& g, const SuperBlock& sb, int max_vec) : g_(g) {
   wave_size_ = sb.wave->size;
-  WorkList wl = compile_worklist(sb, w_budget);
-  n_pairs_ = wl.n_pairs;
+  WorkList wl = compile_worklist(sb);
+  n_pairs_ = wl.pairs.size();
   arena_size_ = wl.arena_size;
   diag_terms_ = std::move(wl.diag);
   bsg_check(bsg_malloc(g_.ctx, sizeof(double) * std::max<int64_t>(1, arena_size_), (void**)&d_arena_), "arena");
@@ -147,17 +117,12 @@ GpuSigma::GpuSigma(GpuContext& g, const SuperBlock& sb, int64_t w_budget, int ma
   size_t wbytes = sizeof(double) * std::max<long>(1, wave_size_) * std::max(1, max_vec);
   bsg_check(bsg_malloc(g_.ctx, wbytes, (void**)&d_c_), "c buffer");
   bsg_check(bsg_malloc(g_.ctx, wbytes, (void**)&d_v_), "v buffer");
-  std::vector<bsg_chunk_t> chunks(wl.g1.size());
-  for (size_t i = 0; i < chunks.size(); ++i) {
-    chunks[i].gemm1 = wl.g1[i].data();
-    chunks[i].n_gemm1 = (int32_t)wl.g1[i].size();
-    chunks[i].vblocks = wl.vb[i].data();
-    chunks[i].n_vblocks = (int32_t)wl.vb[i].size();
-    chunks[i].contribs = wl.ct[i].data();
-    chunks[i].n_contribs = (int32_t)wl.ct[i].size();
-    chunks[i].w_size = wl.wsize[i];
-  }
-  bsg_check(bsg_plan_create(g_.ctx, chunks.data(), (int32_t)chunks.size(), wave_size_, &plan_), "plan");
+  bsg_chunk_t chunk;
+  chunk.vblocks = wl.vb.data();
+  chunk.n_vblocks = (int32_t)wl.vb.size();
+  chunk.pairs = wl.pairs.data();
+  chunk.n_pairs = (int32_t)wl.pairs.size();
+  bsg_check(bsg_plan_create(g_.ctx, &chunk, 1, wave_size_, &plan_), "plan");
   flops_ = bsg_plan_flops(plan_);
 }
 
@@ -168,8 +133,17 @@ GpuSigma::~GpuSigma() {
   if (d_v_) bsg_mfree(g_.ctx, d_v_);
 }
 
+void GpuSigma::set_precision(int mode) {
+  if (mode != BSG_FP64 && mode != BSG_SPLIT_BF16) throw std::runtime_error("GpuSigma: unknown precision mode");
+  if (mode == BSG_SPLIT_BF16 && !split_ready_) {
+    bsg_check(bsg_plan_split_arena(g_.ctx, plan_, d_arena_, arena_size_), "split arena");
+    split_ready_ = true;
+  }
+  precision_ = mode;
+}
+
 void GpuSigma::apply_device(const double* dc, double* dv, int nvec) {
-  bsg_check(bsg_sigma_apply(g_.ctx, plan_, d_arena_, dc, dv, nvec, BSG_FP64), "sigma");
+  bsg_check(bsg_sigma_apply(g_.ctx, plan_, d_arena_, dc, dv, nvec, precision_), "sigma");
 }
 
 void GpuSigma::apply(const double* c_host, double* v_host) {

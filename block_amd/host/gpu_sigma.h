@@ -24,21 +24,18 @@ struct GpuContext {
 };
 
 struct WorkList {
-  std::vector<std::vector<bsg_gemm1_t>> g1;
-  std::vector<std::vector<bsg_vblock_t>> vb;
-  std::vector<std::vector<bsg_contrib_t>> ct;
-  std::vector<int64_t> wsize;
+  std::vector<bsg_vblock_t> vb;    // V blocks with their pair ranges
+  std::vector<bsg_pair_t> pairs;   // GEMM pairs grouped by V block, in term order
   std::vector<bsg_diag_t> diag;
   std::unordered_map<const SparseOp*, int64_t> arena_off;
   int64_t arena_size = 0;
-  size_t n_pairs = 0;  // number of (GEMM1, GEMM2) sector pairs
 };
-// Flatten the term list; W scratch per chunk is kept under w_budget doubles.
-WorkList compile_worklist(const SuperBlock& sb, int64_t w_budget);
+// Flatten the term list into GEMM pairs grouped by the V block they update.
+WorkList compile_worklist(const SuperBlock& sb);
 
 class GpuSigma {
  public:
-  GpuSigma(GpuContext& g, const SuperBlock& sb, int64_t w_budget = (int64_t)1 << 27, int max_vec = 1);
+  GpuSigma(GpuContext& g, const SuperBlock& sb, int max_vec = 1);
   ~GpuSigma();
   GpuSigma(const GpuSigma&) = delete;
   GpuSigma& operator=(const GpuSigma&) = delete;
@@ -47,6 +44,10 @@ class GpuSigma {
   double flops() const { return flops_; }
   size_t n_pairs() const { return n_pairs_; }
   size_t arena_bytes() const { return (size_t)arena_size_ * 8; }
+  // arithmetic of the following applications: BSG_FP64 or BSG_SPLIT_BF16
+  // (the BF16 slices of the arena are built on first use)
+  void set_precision(int mode);
+  int precision() const { return precision_; }
   // host vectors: v = H c (overwrites v)
   void apply(const double* c_host, double* v_host);
   // device vectors: v += H c
@@ -68,6 +69,8 @@ class GpuSigma {
   double* d_c_ = nullptr;
   double* d_v_ = nullptr;
   bsg_plan* plan_ = nullptr;
+  int precision_ = BSG_FP64;
+  bool split_ready_ = false;
   std::vector<bsg_diag_t> diag_terms_;
 };
 

@@ -213,7 +213,7 @@ Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, c
   }
   out.e = diag;
   block_davidson(out.b, out.e, tol, *hfun, true, P.deflation_min, std::max(P.deflation_max, P.nroots + 2), &out.st,
-                 P.davidson_max_iter);
+                 P.davidson_max_iter, P.ladder);
   out.e.resize(P.nroots);
   return out;
 }
@@ -374,6 +374,7 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       Solved sol = solve_superblock(sb, P, S->davidson_tol, make_sigma, rng);
       rec.sigma_seconds += sol.st.sigma_seconds;
       rec.sigma_calls += sol.st.sigma_calls;
+      rec.sigma_calls_split += sol.st.sigma_calls_split;
       rec.sigma_flops += sol.st.sigma_calls * sigma_flops(sb);
       for (int r = 0; r < P.nroots; ++r) rec.energies[r] = std::min(rec.energies[r], sol.e[r]);
       auto rho = density_matrix(sb, sol, weights, true, S->noise, rng);
@@ -418,6 +419,23 @@ SuperBlock dmrg_snapshot(const Integrals& I, const DmrgParams& P, const SigmaFac
   std::vector<BlockPtr> fwd(k), bwd(k);
   std::mt19937_64 rng(P.seed);
   return infinite_warmup(I, P, make_sigma, fwd, bwd, rng);
+}
+
+void write_sweep_records(const DmrgResult& res, double* out) {
+  for (size_t i = 0; i < res.sweeps.size(); ++i) {
+    const auto& s = res.sweeps[i];
+    double* o = out + kSweepCols * i;
+    o[0] = s.sweep;
+    o[1] = s.forward;
+    o[2] = s.M;
+    o[3] = s.energies[0];
+    o[4] = s.max_discarded;
+    o[5] = s.seconds;
+    o[6] = s.sigma_seconds;
+    o[7] = s.sigma_flops;
+    o[8] = (double)s.sigma_calls;
+    o[9] = (double)s.sigma_calls_split;
+  }
 }
 
 }  // namespace blk

@@ -29,6 +29,7 @@ struct DmrgParams {
   int warmup_M = 0;         // 0: first schedule M
   int deflation_min = 2, deflation_max = 20;
   int davidson_max_iter = 200;
+  double ladder = 0.0;  // > 0: split-precision sigma until the residual norm falls below this (FP64 after)
   uint64_t seed = 12345;
   int verbose = 0;
 };
@@ -42,14 +43,22 @@ struct SweepRecord {
   double seconds = 0.0;
   double sigma_seconds = 0.0;
   long sigma_calls = 0;
+  long sigma_calls_split = 0;
   double sigma_flops = 0.0;
 };
+
 
 struct DmrgResult {
   std::vector<SweepRecord> sweeps;
   std::vector<double> energies;  // final per-root energies
   bool converged = false;
 };
+
+// Columns of the per-sweep records the C entry points write.
+constexpr int kSweepCols = 10;
+// sweep, forward, M, E0, max discarded weight, seconds, sigma seconds, sigma
+// flops, sigma calls, split-precision sigma calls
+void write_sweep_records(const DmrgResult& res, double* out);
 
 using SigmaFactory = std::function<std::unique_ptr<DavidsonFunctor>(const SuperBlock&)>;
 

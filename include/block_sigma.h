@@ -42,43 +42,33 @@ enum { BSG_FP64 = 0, BSG_SPLIT_BF16 = 1 };
 typedef struct bsg_context bsg_context;
 typedef struct bsg_plan bsg_plan;
 
-/* Phase-1 item: W[w_off] (m x n) = alpha * op(A) (m x k) * C[c_off] (k x n).
- * A lives in the operator arena at a_off, stored m x k row-major (a_trans=0)
- * or k x m row-major (a_trans=1).  C is a wavefunction block, k x n row-major. */
+/* One GEMM pair of the flattened TensorMultiply (operatorfunctions.C:485-537)
+ * contributing to a V block (m x n):
+ *   W = alpha_a * op(A) (m x k1) * C (k1 x k2),   V += alpha * W * B' (k2 x n)
+ * A lives in the operator arena at a_off, stored m x k1 row-major (a_trans=0)
+ * or k1 x m (a_trans=1); a_off < 0 means A is the identity (W = C, k1 == m).
+ * C is the wavefunction block at c_off, k1 x k2 row-major.  B' = op(B)^T from
+ * the arena at b_off, stored k2 x n (b_trans=0) or n x k2 (b_trans=1);
+ * b_off < 0 means B' is the identity (k2 == n).  W never leaves the chip. */
 typedef struct {
-  int64_t a_off, c_off, w_off;
-  int32_t m, n, k, a_trans;
-  double alpha;
-} bsg_gemm1_t;
+  int64_t a_off, c_off, b_off;
+  int32_t k1, k2, a_trans, b_trans;
+  double alpha_a, alpha;
+} bsg_pair_t;
 
-/* Phase-2 contribution to one V block (m x n):
- *   V += alpha * S (m x k) * B' (k x n)
- * S is W scratch (src_kind 0) or a block of the input C (src_kind 1), row-major
- * with leading dimension k.  B' = op(B)^T from the arena at b_off, stored
- * n x k row-major (b_trans=1) or k x n row-major (b_trans=0); b_off < 0 means
- * B' is the identity (then k == n). */
-typedef struct {
-  int64_t src_off, b_off;
-  int32_t src_kind, b_trans, k, pad;
-  double alpha;
-} bsg_contrib_t;
-
-/* V block at v_off (m x n row-major) with contributions [first, first+count),
- * summed in list order (deterministic). */
+/* V block at v_off (m x n row-major) with pairs [first, first+count), summed
+ * in list order (deterministic). */
 typedef struct {
   int64_t v_off;
   int32_t m, n, first, count;
 } bsg_vblock_t;
 
-/* One chunk of the work list: phase 1 then phase 2. */
+/* One chunk of the work list. */
 typedef struct {
-  const bsg_gemm1_t* gemm1;
-  int32_t n_gemm1;
   const bsg_vblock_t* vblocks;
   int32_t n_vblocks;
-  const bsg_contrib_t* contribs;
-  int32_t n_contribs;
-  int64_t w_size; /* doubles of W scratch this chunk needs */
+  const bsg_pair_t* pairs;
+  int32_t n_pairs;
 } bsg_chunk_t;
 
 /* Diagonal term: diag[v_off + i*n + j] += alpha * A[a_off + i*(m+1)] * B[b_off + j*(n+1)]
@@ -103,26 +93,42 @@ int bsg_memcpy_d2h(bsg_context* ctx, void* dst, const void* src, size_t bytes);
 int bsg_memset_zero(bsg_context* ctx, void* dst, size_t bytes);
 
 /* Build a device plan from host work-list chunks (copied; the host arrays may
- * be freed afterwards).  The plan allocates the W scratch of its largest chunk. */
+ * be freed afterwards).  The plan owns the partial-tile scratch of the
+ * ordered split reduction. */
 int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunks, int64_t wave_size,
                     bsg_plan** plan);
 int bsg_plan_destroy(bsg_plan* plan);
 /* Flops (algorithmic, GEMM pairs) of one application, for reporting. */
 double bsg_plan_flops(const bsg_plan* plan);
-/* The same split by kernel: phase 1 (W = op(A) C) and phase 2 (V += S op(B)^T). */
+/* The same split by GEMM: first (W = op(A) C) and second (V += W op(B)^T). */
 int bsg_plan_phase_flops(const bsg_plan* plan, double* phase1, double* phase2);
 
+/* Doubles of operator arena the plan's pairs reach (the arena passed to
+ * bsg_sigma_apply must hold at least this many). */
+int64_t bsg_plan_arena_extent(const bsg_plan* plan);
+
+/* Split precision set-up: the plan builds (and owns) three BF16 slices of
+ * every arena element, x = x0 + x1 + x2 with |x - x0 - x1 - x2| <~ 2^-24 |x|,
+ * plus a slice buffer for one wavefunction.  Call once after the arena is
+ * uploaded and again whenever its contents change; n >= the plan's extent.
+ * No counterpart in the reference (its sigma is FP64 only). */
+int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* plan, const double* arena, int64_t n);
+
 /* v += H c for nvec vectors stored back to back (stride wave_size doubles).
- * arena: device operator arena; c, v: device pointers.  precision: BSG_FP64. */
+ * arena: device operator arena; c, v: device pointers.
+ * precision: BSG_FP64, or BSG_SPLIT_BF16 (BF16 slice products on MFMA with
+ * FP32 accumulation per 32-deep step and FP64 sums; needs
+ * bsg_plan_split_arena on the same arena first; ~1e-7 relative error). */
 int bsg_sigma_apply(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t nvec,
                     int32_t precision);
 
-/* Timing helper: runs `reps` sigma applications (FP64) on the context stream
- * with HIP events around every kernel launch.  wall_ms: whole loop;
- * kernel_ms[0..2]: summed time of the phase-1, phase-2 and partial-tile
- * reduction kernels; launches[0..2]: number of launches of each. */
+/* Timing helper: runs `reps` sigma applications on the context stream with
+ * HIP events around every kernel launch.  wall_ms: whole loop;
+ * kernel_ms[0..2]: summed time of the fused GEMM-pair kernels, of the
+ * partial-tile reduction and of the wavefunction split (split precision
+ * only); launches[0..2]: number of launches of each. */
 int bsg_sigma_profile(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t reps,
-                      double* wall_ms, double* kernel_ms, long* launches);
+                      int32_t precision, double* wall_ms, double* kernel_ms, long* launches);
 
 /* diag += diagonal terms (device pointers). */
 int bsg_diag_apply(bsg_context* ctx, const bsg_diag_t* terms_host, int32_t n_terms, const double* arena,

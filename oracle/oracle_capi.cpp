@@ -63,38 +63,40 @@ int orc_diag(void* sb, double* out) {
 int orc_exec_worklist(void* sbp, const double* c, double* v) {
   return guarded([&]() {
     const SuperBlock& sb = *static_cast<SuperBlock*>(sbp);
-    WorkList wl = compile_worklist(sb, (int64_t)1 << 22);
+    WorkList wl = compile_worklist(sb);
     std::vector<double> arena(wl.arena_size);
     for (const auto& kv : wl.arena_off) std::memcpy(arena.data() + kv.second, kv.first->data.data(), 8 * kv.first->data.size());
     const long N = sb.wave->size;
     std::fill(v, v + N, 0.0);
-    for (size_t ch = 0; ch < wl.g1.size(); ++ch) {
-      std::vector<double> W(wl.wsize[ch], 0.0);
-      for (const auto& g : wl.g1[ch])
-        for (int i = 0; i < g.m; ++i)
-          for (int p = 0; p < g.k; ++p) {
-            double a = g.alpha * (g.a_trans ? arena[g.a_off + (size_t)p * g.m + i] : arena[g.a_off + (size_t)i * g.k + p]);
-            for (int j = 0; j < g.n; ++j) W[g.w_off + (size_t)i * g.n + j] += a * c[g.c_off + (size_t)p * g.n + j];
+    std::vector<double> W;
+    for (const auto& vb : wl.vb)
+      for (int q = 0; q < vb.count; ++q) {
+        const auto& p = wl.pairs[vb.first + q];
+        // W = alpha_a op(A) C   (m x k2)
+        W.assign((size_t)vb.m * p.k2, 0.0);
+        for (int i = 0; i < vb.m; ++i)
+          for (int s = 0; s < p.k1; ++s) {
+            double a = p.a_off < 0 ? (i == s ? 1.0 : 0.0)
+                                   : p.alpha_a * (p.a_trans ? arena[p.a_off + (size_t)s * vb.m + i]
+                                                            : arena[p.a_off + (size_t)i * p.k1 + s]);
+            if (a == 0.0) continue;
+            for (int j = 0; j < p.k2; ++j) W[(size_t)i * p.k2 + j] += a * c[p.c_off + (size_t)s * p.k2 + j];
           }
-      for (const auto& vb : wl.vb[ch])
-        for (int q = 0; q < vb.count; ++q) {
-          const auto& ct = wl.ct[ch][vb.first + q];
-          const double* S = (ct.src_kind == 0 ? W.data() : c) + ct.src_off;
-          for (int i = 0; i < vb.m; ++i)
-            for (int j = 0; j < vb.n; ++j) {
-              double s = 0.0;
-              if (ct.b_off < 0) {
-                s = S[(size_t)i * ct.k + j];
-              } else {
-                for (int p = 0; p < ct.k; ++p) {
-                  double b = ct.b_trans ? arena[ct.b_off + (size_t)j * ct.k + p] : arena[ct.b_off + (size_t)p * vb.n + j];
-                  s += S[(size_t)i * ct.k + p] * b;
-                }
+        // V += alpha W B'
+        for (int i = 0; i < vb.m; ++i)
+          for (int j = 0; j < vb.n; ++j) {
+            double s = 0.0;
+            if (p.b_off < 0) {
+              s = W[(size_t)i * p.k2 + j];
+            } else {
+              for (int t = 0; t < p.k2; ++t) {
+                double b = p.b_trans ? arena[p.b_off + (size_t)j * p.k2 + t] : arena[p.b_off + (size_t)t * vb.n + j];
+                s += W[(size_t)i * p.k2 + t] * b;
               }
-              v[vb.v_off + (size_t)i * vb.n + j] += ct.alpha * s;
             }
-        }
-    }
+            v[vb.v_off + (size_t)i * vb.n + j] += p.alpha * s;
+          }
+      }
     return 0;
   });
 }
@@ -131,7 +133,7 @@ void* orc_exact_superblock(void* ints, int nelec, int twoS, int irrep, int nL) {
 }
 
 // the sweep driver with the oracle sigma (CPU reference run); same outputs as
-// blkh_dmrg_gpu
+// blkh_dmrg_gpu (sweep_out rows of 10)
 int orc_dmrg_cpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
                  const double* sched, int max_sweeps, double sweep_tol, int nthreads, int verbose,
                  double* energies_out, double* sweep_out) {
@@ -152,18 +154,7 @@ int orc_dmrg_cpu(void* ints, int nelec, int twoS, int irrep, int nroots, const d
       return std::unique_ptr<DavidsonFunctor>(new OracleFunctor(sb, nthreads));
     });
     for (int r = 0; r < nroots; ++r) energies_out[r] = res.energies[r];
-    for (size_t i = 0; i < res.sweeps.size(); ++i) {
-      const auto& s = res.sweeps[i];
-      double* o = sweep_out + 8 * i;
-      o[0] = s.sweep;
-      o[1] = s.forward;
-      o[2] = s.M;
-      o[3] = s.energies[0];
-      o[4] = s.max_discarded;
-      o[5] = s.seconds;
-      o[6] = s.sigma_seconds;
-      o[7] = s.sigma_flops;
-    }
+    write_sweep_records(res, sweep_out);
     return (int)res.sweeps.size();
   });
 }

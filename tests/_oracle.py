@@ -98,7 +98,8 @@ def dmrg_cpu(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, sched
     w = np.ascontiguousarray(weights if weights is not None else np.full(nroots, 1.0 / nroots), dtype=np.float64)
     sched = block_amd._schedule_array(schedule)
     e = np.zeros(nroots)
-    sw = np.zeros(8 * max_sweeps)
+    nc = len(block_amd.SWEEP_COLS)
+    sw = np.zeros(nc * max_sweeps)
     n = _chk(lib.orc_dmrg_cpu(ints._h, nelec, twoS, irrep, nroots, _p(w), len(schedule), _p(sched), max_sweeps,
                               sweep_tol, nthreads, verbose, _p(e), _p(sw)))
-    return e, sw[: 8 * n].reshape(n, 8)
+    return e, sw[: nc * n].reshape(n, nc)

@@ -29,3 +29,18 @@ def test_gpu_dmrg_hubbard_chain12():
     assert e[0] == pytest.approx(-6.526243384454524, abs=1e-6)
     e_cpu, _ = _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=14, sweep_tol=1e-9)
     assert abs(e[0] - e_cpu[0]) <= 1e-10
+
+
+def test_gpu_dmrg_ladder_chain12():
+    """Precision ladder: split-precision sigmas until the residual norm is
+    below 1e-5, FP64 after; the final energy matches the FP64 sweep to 1e-8."""
+    print("[gpu] dmrg hubbard chain 12 ladder", flush=True)
+    ints = config_fcidump("hubbard12")
+    e64, sw64 = block_amd.dmrg(ints, schedule=SCHED, max_sweeps=14, sweep_tol=1e-9)
+    e, sw = block_amd.dmrg(ints, schedule=SCHED, max_sweeps=14, sweep_tol=1e-9, ladder=1e-5)
+    split_calls = sw[:, block_amd.SWEEP_COLS.index("split_calls")].sum()
+    calls = sw[:, block_amd.SWEEP_COLS.index("sigma_calls")].sum()
+    print(f"[gpu] ladder dE={e[0] - e64[0]:.3e} split calls {split_calls:.0f} of {calls:.0f}", flush=True)
+    assert split_calls > 0
+    assert abs(e[0] - e64[0]) <= 1e-8
+    assert e[0] == pytest.approx(-6.526243384454524, abs=1e-6)

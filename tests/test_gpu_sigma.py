@@ -45,3 +45,30 @@ def test_sigma_parity(name, make):
     # diagonal
     d = h.diagonal()
     np.testing.assert_allclose(d, _oracle.diagonal(sb), rtol=0, atol=1e-12 * max(1.0, np.abs(d).max()))
+
+
+SPLIT_TOL = 2e-6  # three BF16 slices: ~2^-24 per element, FP32 sums of 32-deep steps
+
+
+@pytest.mark.parametrize("name,make", _cases(), ids=[c[0] for c in _cases()])
+def test_split_sigma(name, make):
+    """Split-precision sigma against the FP64 oracle; switching back restores
+    the FP64 result bit for bit."""
+    print(f"[gpu] split sigma {name}", flush=True)
+    sb = make()
+    h = block_amd.multiply_h(sb)
+    c = rng_vec(sb.wave_size, 11)
+    ref = _oracle.sigma(sb, c, nthreads=8)
+    f64 = h(c)
+    h.precision = block_amd.SPLIT_BF16
+    got = h(c)
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"[gpu] {name} split rel={rel:.3e}", flush=True)
+    assert rel <= SPLIT_TOL
+    assert rel > 0.0  # the BF16 path really ran
+    assert np.array_equal(h(c), got)
+    # Rayleigh quotient error of the split sigma
+    cn = c / np.linalg.norm(c)
+    assert abs(cn @ h(cn) - cn @ (ref / np.linalg.norm(c))) <= SPLIT_TOL * np.linalg.norm(ref) / np.linalg.norm(c)
+    h.precision = block_amd.FP64
+    assert np.array_equal(h(c), f64)
