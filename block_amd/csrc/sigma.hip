@@ -59,89 +59,11 @@ struct Shape {
   static constexpr int LDB = TN + 2;  // LDS row pitch of the B slice
 };
 
-struct Acc {
-  dbl4 r[2][2];
-};
-
-__device__ inline void acc_zero(Acc& a) {
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) a.r[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
-}
-
-// Stage op(X) rows [row0,row0+TM) x cols [k0,k0+TK) into As (zero outside m x k).
-// trans=0: X is m x k row-major (ld = k); trans=1: X is k x m row-major (ld = m).
-template <class S>
-__device__ inline void load_a(double* As, const double* X, int trans, int m, int k, int row0, int k0, double alpha) {
-  if (!trans) {
-    for (int e = threadIdx.x; e < S::TM * TK; e += S::NT) {
-      int i = e / TK, p = e % TK;
-      int gi = row0 + i, gp = k0 + p;
-      As[i * S::LDA + p] = (gi < m && gp < k) ? alpha * X[(size_t)gi * k + gp] : 0.0;
-    }
-  } else {
-    for (int e = threadIdx.x; e < S::TM * TK; e += S::NT) {
-      int p = e / S::TM, i = e % S::TM;
-      int gi = row0 + i, gp = k0 + p;
-      As[i * S::LDA + p] = (gi < m && gp < k) ? alpha * X[(size_t)gp * m + gi] : 0.0;
-    }
-  }
-}
-
-// Stage B' rows [k0,k0+TK) x cols [col0,col0+TN) into Bs (zero outside k x n).
-// trans=0: B' stored k x n row-major (ld = ldb); trans=1: stored n x k (ld = k).
-template <class S>
-__device__ inline void load_b(double* Bs, const double* Y, int trans, int k, int n, int ldb, int k0, int col0) {
-  if (!trans) {
-    for (int e = threadIdx.x; e < TK * S::TN; e += S::NT) {
-      int p = e / S::TN, j = e % S::TN;
-      int gp = k0 + p, gj = col0 + j;
-      Bs[p * S::LDB + j] = (gp < k && gj < n) ? Y[(size_t)gp * ldb + gj] : 0.0;
-    }
-  } else {
-    for (int e = threadIdx.x; e < TK * S::TN; e += S::NT) {
-      int j = e / TK, p = e % TK;
-      int gp = k0 + p, gj = col0 + j;
-      Bs[p * S::LDB + j] = (gp < k && gj < n) ? Y[(size_t)gj * k + gp] : 0.0;
-    }
-  }
-}
-
 template <int WR, int WC>
 __device__ inline void wave_origin(int& wr, int& wc) {
   const int wave = threadIdx.x >> 6;
   wr = (wave / WC) * 32;
   wc = (wave % WC) * 32;
-}
-
-// acc += A (rows of this wave, TK columns starting at ka) * B (TK rows starting
-// at kb, this wave's columns); A and B in LDS with pitches lda / ldb
-template <int WR, int WC>
-__device__ inline void mma_lds(Acc& acc, const double* A, int lda, int ka, const double* B, int ldb) {
-  const int lane = threadIdx.x & 63;
-  int wr, wc;
-  wave_origin<WR, WC>(wr, wc);
-  const int li = lane & 15, lk = lane >> 4;
-#pragma unroll
-  for (int kk = 0; kk < TK; kk += 4) {
-    double a0 = A[(wr + li) * lda + ka + kk + lk];
-    double a1 = A[(wr + 16 + li) * lda + ka + kk + lk];
-    double b0 = B[(kk + lk) * ldb + wc + li];
-    double b1 = B[(kk + lk) * ldb + wc + 16 + li];
-    acc.r[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc.r[0][0], 0, 0, 0);
-    acc.r[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc.r[0][1], 0, 0, 0);
-    acc.r[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc.r[1][0], 0, 0, 0);
-    acc.r[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc.r[1][1], 0, 0, 0);
-  }
-}
-
-// (row, col) inside the workgroup tile of accumulator element (bi, bj, i)
-template <int WR, int WC>
-__device__ inline void acc_pos(int bi, int bj, int i, int& row, int& col) {
-  const int lane = threadIdx.x & 63;
-  int wr, wc;
-  wave_origin<WR, WC>(wr, wc);
-  row = wr + bi * 16 + (lane >> 4) + 4 * i;
-  col = wc + bj * 16 + (lane & 15);
 }
 
 // A work unit: pairs [first, first+count) of one V block.  p_off < 0: the
@@ -158,116 +80,6 @@ struct RedItem {
   int32_t nseg, mn, e0, len;
 };
 
-template <int WR, int WC>
-struct Fused {
-  using S = Shape<WR, WC>;
-  static constexpr int TJ = S::TN;         // W chunk width (second contraction)
-  static constexpr int LDC = TJ + 2;       // pitch of the C slice
-  static constexpr int LDW = TJ + 1;       // pitch of the W chunk
-  static constexpr int AB = S::TM * S::LDA + TK * LDC;  // A and C slices (GEMM 1), B slice (GEMM 2)
-  static constexpr int AB2 = TK * S::LDB;
-  static constexpr int ABSZ = AB > AB2 ? AB : AB2;
-};
-
-template <int WR, int WC>
-__global__ __launch_bounds__(64 * WR * WC) void k_sigma(const bsg_vblock_t* __restrict__ vbs,
-                                                        const bsg_pair_t* __restrict__ prs,
-                                                        const Seg* __restrict__ segs,
-                                                        const TileRef* __restrict__ tiles, int ntiles,
-                                                        const double* __restrict__ arena,
-                                                        const double* __restrict__ c, double* __restrict__ v,
-                                                        double* __restrict__ part) {
-  using S = Shape<WR, WC>;
-  using F = Fused<WR, WC>;
-  __shared__ double Ab[F::ABSZ];
-  __shared__ double Ws[S::TM * F::LDW];
-  double* As = Ab;
-  double* Cs = Ab + S::TM * S::LDA;
-  double* Bs = Ab;
-  const int tid = blockIdx.x;
-  if (tid >= ntiles) return;
-  const TileRef tr = tiles[tid];
-  const Seg sg = segs[tr.item];
-  const bsg_vblock_t vb = vbs[sg.vb];
-  const int row0 = tr.tm * S::TM, col0 = tr.tn * S::TN;
-  Acc accV;
-  acc_zero(accV);
-  for (int q = 0; q < sg.count; ++q) {
-    const bsg_pair_t pr = prs[sg.first + q];
-    const double* Cb = c + pr.c_off;
-    for (int j0 = 0; j0 < pr.k2; j0 += F::TJ) {
-      if (pr.b_off < 0 && j0 != col0) continue;  // identity B': only the diagonal chunk
-      // ---- W chunk (TM x TJ) = alpha_a op(A) C[:, j0:j0+TJ], times alpha, into Ws
-      if (pr.a_off < 0) {
-        for (int e = threadIdx.x; e < S::TM * F::TJ; e += S::NT) {
-          int i = e / F::TJ, j = e % F::TJ;
-          int gi = row0 + i, gj = j0 + j;
-          Ws[i * F::LDW + j] = (gi < vb.m && gj < pr.k2) ? pr.alpha * Cb[(size_t)gi * pr.k2 + gj] : 0.0;
-        }
-      } else {
-        const double* A = arena + pr.a_off;
-        Acc accW;
-        acc_zero(accW);
-        for (int k0 = 0; k0 < pr.k1; k0 += TK) {
-          load_a<S>(As, A, pr.a_trans, vb.m, pr.k1, row0, k0, pr.alpha_a);
-          // C slice: rows [k0, k0+TK), cols [j0, j0+TJ) of the k1 x k2 block
-          for (int e = threadIdx.x; e < TK * F::TJ; e += S::NT) {
-            int p = e / F::TJ, j = e % F::TJ;
-            int gp = k0 + p, gj = j0 + j;
-            Cs[p * F::LDC + j] = (gp < pr.k1 && gj < pr.k2) ? Cb[(size_t)gp * pr.k2 + gj] : 0.0;
-          }
-          __syncthreads();
-          mma_lds<WR, WC>(accW, As, S::LDA, 0, Cs, F::LDC);
-          __syncthreads();
-        }
-        for (int bi = 0; bi < 2; ++bi)
-          for (int bj = 0; bj < 2; ++bj)
-            for (int i = 0; i < 4; ++i) {
-              int r, cc;
-              acc_pos<WR, WC>(bi, bj, i, r, cc);
-              Ws[r * F::LDW + cc] = pr.alpha * accW.r[bi][bj][i];
-            }
-      }
-      __syncthreads();
-      // ---- V tile += W chunk * B'[j0:j0+TJ, col0:col0+TN]
-      if (pr.b_off < 0) {
-        for (int bi = 0; bi < 2; ++bi)
-          for (int bj = 0; bj < 2; ++bj)
-            for (int i = 0; i < 4; ++i) {
-              int r, cc;
-              acc_pos<WR, WC>(bi, bj, i, r, cc);
-              accV.r[bi][bj][i] += Ws[r * F::LDW + cc];
-            }
-      } else {
-        const double* B = arena + pr.b_off;
-        for (int kk0 = 0; kk0 < F::TJ && j0 + kk0 < pr.k2; kk0 += TK) {
-          load_b<S>(Bs, B, pr.b_trans, pr.k2, vb.n, vb.n, j0 + kk0, col0);
-          __syncthreads();
-          mma_lds<WR, WC>(accV, Ws, F::LDW, kk0, Bs, S::LDB);
-          __syncthreads();
-        }
-      }
-      __syncthreads();
-    }
-  }
-  const bool direct = sg.p_off < 0;
-  double* out = direct ? v + vb.v_off : part + sg.p_off;
-  for (int bi = 0; bi < 2; ++bi)
-    for (int bj = 0; bj < 2; ++bj)
-      for (int i = 0; i < 4; ++i) {
-        int r, cc;
-        acc_pos<WR, WC>(bi, bj, i, r, cc);
-        r += row0;
-        cc += col0;
-        if (r < vb.m && cc < vb.n) {
-          if (direct)
-            out[(size_t)r * vb.n + cc] += accV.r[bi][bj][i];
-          else
-            out[(size_t)r * vb.n + cc] = accV.r[bi][bj][i];
-        }
-      }
-}
-
 __global__ __launch_bounds__(256) void k_reduce(const RedItem* __restrict__ items, const double* __restrict__ part,
                                                 double* __restrict__ v) {
   const RedItem it = items[blockIdx.x];
@@ -277,6 +89,299 @@ __global__ __launch_bounds__(256) void k_reduce(const RedItem* __restrict__ item
   double s = 0.0;
   for (int q = 0; q < it.nseg; ++q) s += part[it.p_off + (int64_t)q * it.mn + x];
   v[it.v_off + x] += s;
+}
+
+// ---------------------------------------------------------------- FP64 grouped kernel
+// One launch covers every V block.  A workgroup of 4 wavefronts owns a V
+// panel tile: rows [row0, row0 + TM) and one column group of width JC (the
+// whole block when n <= JC), with the waves arranged WR x WC (WR WC = 4),
+// each wave owning NR 16-row blocks and PB = 2 16-wide column blocks (wc,
+// wc + WC).  Per tile geometry (WR, NR) is chosen by the host per V block:
+// TM = 16 NR WR <= 64, JC = 32 WC.  Two waves per SIMD (register budget 256).  Each W block is formed once per tile and
+// padding is 16-granular.
+//   GEMM 1: W chunk (TM x JC of W) = alpha_a op(A) C: op(A) staged through LDS
+//           in 32-deep K chunks (double-buffered, shared by the waves of a
+//           wave row), C column blocks straight into MFMA B operands; the
+//           next chunk's loads are issued before the current chunk's MFMAs.
+//   GEMM 2: V += alpha W B': the W chunk in LDS is the MFMA A operand, B'
+//           column blocks go straight into B operands, 32 K deep per batch.
+// Operand loads are raw buffer loads: masked lanes use an out-of-range offset
+// and read zero, so no select follows a load.
+constexpr int PB = 2;      // column blocks per wave
+constexpr int PKC = 32;    // K depth of one staged op(A) chunk
+constexpr int GTM = 64;    // largest tile height
+constexpr int GP = GTM + 1;        // LDS pitch of k-major rows (doubles)
+constexpr int GWS = 16 * PB * 4 * 33;  // W chunk doubles: max over geometries of JC (TM + 1)
+constexpr uint32_t OOB = 0x80000000u;
+
+__device__ inline __amdgpu_buffer_rsrc_t block_rsrc(const double* p, int64_t n) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * 8), 0x00020000);
+}
+__device__ inline double bload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+// voff: per-lane byte offset (OOB for a masked lane); soff: uniform byte offset
+__device__ inline double bload2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// acc[r][t] += A(row blocks of this wave, 4-deep K step at k) * b[t] for the
+// first NV column blocks and the first nr row blocks; A k-major in LDS.
+template <int NV>
+__device__ inline void mma_step(int nr, dbl4 (&acc)[2][PB], const double* Al, int P, int arow, int k, int li, int lk,
+                                const double (&b)[PB]) {
+  const double a0 = Al[(k + lk) * P + arow + li];
+  double a1 = 0.0;
+  if (nr > 1) a1 = Al[(k + lk) * P + arow + 16 + li];
+#pragma unroll
+  for (int t = 0; t < NV; ++t) {
+    acc[0][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b[t], acc[0][t], 0, 0, 0);
+    if (nr > 1) acc[1][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b[t], acc[1][t], 0, 0, 0);
+  }
+}
+// 4 K steps (16 deep) from k0; steps at or past kend skipped
+template <int NV>
+__device__ inline void mma_run_n(int nr, dbl4 (&acc)[2][PB], const double* Al, int P, int arow, int k0, int kend,
+                                 int li, int lk, const double (&b)[4][PB]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    if (k0 + 4 * s < kend) mma_step<NV>(nr, acc, Al, P, arow, k0 + 4 * s, li, lk, b[s]);
+}
+__device__ inline void mma_run(int nv, int nr, dbl4 (&acc)[2][PB], const double* Al, int P, int arow, int k0,
+                               int kend, int li, int lk, const double (&b)[4][PB]) {
+  // column slots past nv hold zero B operands; they run without a branch
+  if (nv > 0) mma_run_n<PB>(nr, acc, Al, P, arow, k0, kend, li, lk, b);
+}
+
+// Tile geometry code in TileRef::pad: bits 0-1 log2(WR), bit 2 (NR - 1).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sigma(const bsg_vblock_t* __restrict__ vbs,
+                                               const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
+                                               const TileRef* __restrict__ tiles, int ntiles,
+                                               const double* __restrict__ arena, const double* __restrict__ c,
+                                               double* __restrict__ v, double* __restrict__ part) {
+  constexpr int NT = 256;
+  // GEMM 1 uses two op(A) chunk buffers, GEMM 2 the W chunk; they share the
+  // LDS (barriers separate the phases).  Both k-major: X[k * P + row].
+  constexpr int LDSD = 2 * PKC * GP > GWS ? 2 * PKC * GP : GWS;
+  __shared__ double lds[LDSD];
+  double* const As0 = lds;
+  double* const As1 = lds + PKC * GP;
+  double* const Ws = lds;
+  const int tid = blockIdx.x;
+  if (tid >= ntiles) return;
+  const TileRef tr = tiles[tid];
+  const Seg sg = segs[tr.item];
+  const bsg_vblock_t vb = vbs[sg.vb];
+  const int lwr = tr.pad & 3, nr = ((tr.pad >> 2) & 1) + 1;
+  const int WR = 1 << lwr, WC = 4 >> lwr;
+  const int TM = 16 * nr * WR, JC = 16 * PB * WC, P = TM + 1;
+  const int row0 = tr.tm * TM, colg = tr.tn * JC;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wr = w / WC, wc = w % WC;
+  const int arow = wr * nr * 16;  // first panel row of this wave
+  const int li = lane & 15, lk = lane >> 4;
+  // row blocks of this wave inside the block (0, 1 or 2)
+  int nrw = (vb.m - row0 - arow + 15) / 16;
+  nrw = nrw < 0 ? 0 : (nrw > nr ? nr : nrw);
+  int nvb = 0;  // V column blocks of this wave that exist
+  for (int t = 0; t < PB; ++t)
+    if (colg + (wc + WC * t) * 16 < vb.n) nvb = t + 1;
+  if (nrw == 0) nvb = 0;
+  dbl4 accV[2][PB];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int t = 0; t < PB; ++t) accV[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  int abuf = 0;
+  for (int q = 0; q < sg.count; ++q) {
+    const bsg_pair_t pr = prs[sg.first + q];
+    const __amdgpu_buffer_rsrc_t rC = block_rsrc(c + pr.c_off, (int64_t)pr.k1 * pr.k2);
+    for (int j0 = 0; j0 < pr.k2; j0 += JC) {
+      if (pr.b_off < 0 && j0 != colg) continue;  // identity B': only the matching chunk
+      const int jw = min(JC, pr.k2 - j0);        // W columns in this chunk
+      int nwb = 0;                               // W column blocks of this wave in the chunk
+      for (int t = 0; t < PB; ++t)
+        if ((wc + WC * t) * 16 < jw) nwb = t + 1;
+      if (nrw == 0) nwb = 0;
+      if (pr.a_off < 0) {
+        // identity A: W = alpha C (panel rows of C)
+        if (pr.b_off < 0) {
+#pragma unroll
+          for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int t = 0; t < PB; ++t) {
+              const int col = colg + (wc + WC * t) * 16 + li;
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                const int row = row0 + arow + r * 16 + lk + 4 * i;
+                const bool ok = r < nr && row < vb.m && col < vb.n;
+                accV[r][t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
+              }
+            }
+          continue;
+        }
+        for (int e = threadIdx.x; e < TM * JC; e += NT) {
+          const int j = e % JC, i = e / JC;
+          const int gi = row0 + i;
+          const bool ok = gi < vb.m && j < jw;
+          Ws[j * P + i] = pr.alpha * bload(rC, ok ? (uint32_t)(gi * pr.k2 + j0 + j) * 8u : OOB);
+        }
+      } else {
+        const __amdgpu_buffer_rsrc_t rA = block_rsrc(arena + pr.a_off, (int64_t)vb.m * pr.k1);
+        dbl4 accW[2][PB];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int t = 0; t < PB; ++t) accW[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+        // Pipeline over 32-deep chunks, each consumed in two 16-deep halves:
+        // the op(A) elements of a chunk and the C operands of its first half
+        // are loaded during the previous chunk's second half, the C operands
+        // of the second half during the first half.  Per-lane offsets are
+        // loop-invariant; the K position goes in the uniform offset.  Loads
+        // past the end of a block read zero (buffer range check); op(A)
+        // elements loaded past k1 inside the block only meet zero C rows.
+        constexpr int AE = GTM * PKC / NT;
+        uint32_t aoff[AE];
+#pragma unroll
+        for (int u = 0; u < AE; ++u) {
+          const int e = threadIdx.x + u * NT;
+          const int i = pr.a_trans ? e % TM : e / PKC, p = pr.a_trans ? e / TM : e % PKC;
+          const int gi = row0 + i;
+          aoff[u] = e < TM * PKC ? (uint32_t)(pr.a_trans ? p * vb.m + gi : gi * pr.k1 + p) * 8u : OOB;
+        }
+        const uint32_t astep = (uint32_t)(pr.a_trans ? vb.m : 1) * 8u;  // bytes per K index
+        uint32_t coff[PB];
+#pragma unroll
+        for (int t = 0; t < PB; ++t) {
+          const int col = j0 + (wc + WC * t) * 16 + li;
+          coff[t] = (t < nwb && col < pr.k2) ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
+        }
+        const uint32_t crow = (uint32_t)pr.k2 * 8u;  // bytes per C row
+        double an[AE], bn[4][PB];
+        auto load_a = [&](int k0) {
+#pragma unroll
+          for (int u = 0; u < AE; ++u)
+            if (u * NT < TM * PKC) an[u] = bload2(rA, aoff[u], (uint32_t)k0 * astep);
+        };
+        auto load_c = [&](int k0, int kh) {  // C rows k0 + kh .. k0 + kh + 15
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+            for (int t = 0; t < PB; ++t) bn[s4][t] = bload2(rC, coff[t], (uint32_t)(k0 + kh + 4 * s4) * crow);
+        };
+        load_a(0);
+        load_c(0, 0);
+        for (int k0 = 0; k0 < pr.k1; k0 += PKC) {
+          const int kc = min(PKC, pr.k1 - k0);
+          double* Ab = abuf ? As1 : As0;
+          abuf ^= 1;
+#pragma unroll
+          for (int u = 0; u < AE; ++u) {
+            const int e = threadIdx.x + u * NT;
+            if (u * NT < TM * PKC && e < TM * PKC) {
+              const int i = pr.a_trans ? e % TM : e / PKC, p = pr.a_trans ? e / TM : e % PKC;
+              Ab[p * P + i] = pr.alpha_a * an[u];
+            }
+          }
+          double bc[4][PB];
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+            for (int t = 0; t < PB; ++t) bc[s4][t] = bn[s4][t];
+          __syncthreads();
+          if (kc > 16) load_c(k0, 16);
+          mma_run(nwb, nrw, accW, Ab, P, arow, 0, kc, li, lk, bc);
+          if (kc > 16) {
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+              for (int t = 0; t < PB; ++t) bc[s4][t] = bn[s4][t];
+          }
+          if (k0 + PKC < pr.k1) {
+            load_a(k0 + PKC);
+            load_c(k0 + PKC, 0);
+          }
+          if (kc > 16) mma_run(nwb, nrw, accW, Ab, P, arow, 16, kc, li, lk, bc);
+        }
+        __syncthreads();  // op(A) buffers done before W (same LDS) is written
+        if (pr.b_off < 0) {
+          // identity B': the W blocks of this wave are its V blocks
+#pragma unroll
+          for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int t = 0; t < PB; ++t)
+#pragma unroll
+              for (int i = 0; i < 4; ++i) accV[r][t][i] += pr.alpha * accW[r][t][i];
+          continue;
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int t = 0; t < PB; ++t) {
+            const int cb = wc + WC * t;
+            if (r < nr)
+#pragma unroll
+              for (int i = 0; i < 4; ++i)
+                Ws[(cb * 16 + li) * P + arow + r * 16 + lk + 4 * i] = pr.alpha * accW[r][t][i];
+          }
+      }
+      __syncthreads();
+      // GEMM 2: V[tile] += W chunk * B'[j0:j0+jw, group columns]; W columns
+      // past jw up to the next multiple of 4 are zero.  B' operands of the
+      // next 32-deep batch are loaded while the current one runs.
+      {
+        const __amdgpu_buffer_rsrc_t rB = block_rsrc(arena + pr.b_off, (int64_t)vb.n * pr.k2);
+        // B'(k, col) at k * ks + col * cs inside the block
+        const int ks = pr.b_trans ? 1 : vb.n, cs = pr.b_trans ? pr.k2 : 1;
+        // per-lane offsets loop-invariant, K position in the uniform offset;
+        // rows past k2 read zero or meet zero W columns
+        uint32_t boff[PB];
+#pragma unroll
+        for (int t = 0; t < PB; ++t) {
+          const int col = colg + (wc + WC * t) * 16 + li;
+          boff[t] = (t < nvb && col < vb.n) ? (uint32_t)(lk * ks + col * cs) * 8u : OOB;
+        }
+        const uint32_t bstep = (uint32_t)ks * 8u;
+        double bn[4][PB];
+        auto load_b = [&](int kg) {
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+            for (int t = 0; t < PB; ++t) bn[s4][t] = bload2(rB, boff[t], (uint32_t)(j0 + kg + 4 * s4) * bstep);
+        };
+        load_b(0);
+        for (int kg = 0; kg < jw; kg += 16) {
+          double bb[4][PB];
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+            for (int t = 0; t < PB; ++t) bb[s4][t] = bn[s4][t];
+          if (kg + 16 < jw) load_b(kg + 16);
+          mma_run(nvb, nrw, accV, Ws, P, arow, kg, jw, li, lk, bb);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const bool direct = sg.p_off < 0;
+  double* out = direct ? v + vb.v_off : part + sg.p_off;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int t = 0; t < PB; ++t) {
+      const int col = colg + (wc + WC * t) * 16 + li;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = row0 + arow + r * 16 + lk + 4 * i;
+        if (r < nr && row < vb.m && col < vb.n) {
+          if (direct)
+            out[(size_t)row * vb.n + col] += accV[r][t][i];
+          else
+            out[(size_t)row * vb.n + col] = accV[r][t][i];
+        }
+      }
+    }
 }
 
 // ---------------------------------------------------------------- split precision
@@ -538,6 +643,58 @@ inline int pick_shape(int m, int n) {
 }
 inline int shape_tm(int s) { return s ? 32 : 64; }
 
+// Tile geometry of the FP64 grouped kernel for an m x n V block: waves WR x
+// WC (WR WC = 4) and 16-row blocks per wave NR, TM = 16 NR WR <= 64 rows and
+// JC = 16 PB WC columns per tile.  Chosen to minimise the MFMA issue time of
+// the block's tiles (per-wave MFMAs per pair, summed over tiles), which
+// counts row padding, idle column slots and GEMM 1 repeated per column group.
+struct Geom {
+  int lwr, nr, tm, jc, code;
+};
+inline long geom_cost(const Geom& g, long m, long n, long k) {
+  auto cdiv = [](long a, long b) { return (a + b - 1) / b; };
+  const int wc = 4 >> g.lwr;
+  const long panels = cdiv(m, g.tm), groups = cdiv(n, g.jc);
+  const long wblk = cdiv(cdiv(k, 16), wc);          // W column blocks per wave (all chunks)
+  const long vblk = std::min<long>(PB, cdiv(std::min<long>(n, g.jc), 16 * wc) > 0 ? cdiv(std::min<long>(n, g.jc), 16 * wc) : 1);
+  return panels * groups * g.nr * (wblk + vblk) * cdiv(k, 4);
+}
+inline Geom make_geom(int lwr, int nr) {
+  Geom g;
+  g.lwr = lwr;
+  g.nr = nr;
+  g.tm = 16 * nr * (1 << lwr);
+  g.jc = 16 * PB * (4 >> lwr);
+  g.code = lwr | ((nr - 1) << 2);
+  return g;
+}
+inline Geom pick_geom(int m, int n) {
+  // typical contraction depth: the block's own dimensions
+  const long k = std::max(m, n);
+  Geom best = make_geom(0, 1);
+  long bc = -1;
+  for (int lwr = 0; lwr <= 2; ++lwr)
+    for (int nr = 1; nr <= 2; ++nr) {
+      const Geom g = make_geom(lwr, nr);
+      if (g.tm > GTM) continue;
+      const long cst = geom_cost(g, m, n, k);
+      if (bc < 0 || cst < bc || (cst == bc && g.tm > best.tm)) {
+        bc = cst;
+        best = g;
+      }
+    }
+  return best;
+}
+// MFMA instructions per wavefront of one pair in a tile (segment cost)
+inline long pair_cost(const Geom& g, int n, const bsg_pair_t& pr) {
+  auto cdiv = [](long a, long b) { return (a + b - 1) / b; };
+  const int wc = 4 >> g.lwr;
+  const long nch = pr.b_off < 0 ? 1 : cdiv(pr.k2, g.jc);
+  const long g1 = pr.a_off < 0 ? 0 : g.nr * cdiv(cdiv(pr.k2, 16), wc) * cdiv(pr.k1, 4);
+  const long g2 = pr.b_off < 0 ? 0 : g.nr * std::min<long>(PB, cdiv(cdiv(std::min<long>(n, g.jc), 16), wc)) * cdiv(std::min<long>(pr.k2, g.jc), 4) * nch;
+  return g1 + g2 + 16 * nch;
+}
+
 // ---------------------------------------------------------------- vector kernels
 constexpr int RB = 256, RMAXB = 1024;
 
@@ -619,8 +776,10 @@ struct PlanChunk {
   bsg_vblock_t* d_vb = nullptr;
   bsg_pair_t* d_pr = nullptr;
   Seg* d_seg = nullptr;
-  TileRef* d_t[2] = {nullptr, nullptr};  // per tile shape (0: 64x64, 1: 32x32)
+  TileRef* d_t[2] = {nullptr, nullptr};  // split-precision tiles per shape (0: 64x64, 1: 32x32)
   int n_t[2] = {0, 0};
+  TileRef* d_g = nullptr;  // FP64 grouped-kernel tiles (costliest first)
+  int n_g = 0;
   RedItem* d_red = nullptr;
   int n_red = 0;
 };
@@ -724,7 +883,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
   p->ctx = ctx;
   p->wave_size = wave_size;
   int64_t pmax = 0;
-  const int SEG_COST = 64;  // LDS slices per work unit before its pair list is split
+  const long SEG_COST = 2048;  // MFMAs per wavefront of a work unit before its pair list is split
   auto cleanup = [&](int code, const std::string& m) {
     bsg_plan_destroy(p);
     return fail(code, m);
@@ -733,6 +892,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     const bsg_chunk_t& ch = chunks[ci];
     PlanChunk pc;
     std::vector<TileRef> tl[2];
+    std::vector<std::pair<long, TileRef>> gt;  // (cost, tile)
     std::vector<Seg> segs;
     std::vector<RedItem> reds;
     int64_t poff = 0;
@@ -741,10 +901,12 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       if (vb.m <= 0 || vb.n <= 0 || vb.v_off < 0 || vb.v_off + (int64_t)vb.m * vb.n > wave_size || vb.first < 0 ||
           vb.count < 0 || vb.first + vb.count > ch.n_pairs)
         return cleanup(BSG_ERR_ARG, "bsg_plan_create: vblock out of range");
-      const int sh = pick_shape(vb.m, vb.n), T = shape_tm(sh), TJ = T;
-      // validate pairs, count flops, cut the list into units of ~SEG_COST slices
+      const int sh = pick_shape(vb.m, vb.n), T = shape_tm(sh);
+      const Geom gm = pick_geom(vb.m, vb.n);
+      // validate pairs, count flops, cut the list into units of ~SEG_COST MFMAs per wave
       std::vector<std::pair<int, int>> units;
-      int start = 0, cost = 0;
+      int start = 0;
+      long cost = 0;
       for (int q = 0; q < vb.count; ++q) {
         const auto& pr = ch.pairs[vb.first + q];
         if (pr.k1 <= 0 || pr.k2 <= 0 || pr.c_off < 0 || pr.c_off + (int64_t)pr.k1 * pr.k2 > wave_size ||
@@ -760,8 +922,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           p->flops += 2.0 * vb.m * pr.k2 * vb.n;
           p->flops2 += 2.0 * vb.m * pr.k2 * vb.n;
         }
-        const int chunks_j = pr.b_off < 0 ? 1 : (pr.k2 + TJ - 1) / TJ;
-        cost += chunks_j * ((pr.a_off < 0 ? 0 : (pr.k1 + TK - 1) / TK) + (pr.b_off < 0 ? 0 : TJ / TK) + 1);
+        cost += pair_cost(gm, vb.n, pr);
         if (cost >= SEG_COST && q + 1 < vb.count) {
           units.push_back({start, q + 1 - start});
           start = q + 1;
@@ -782,6 +943,10 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         segs.push_back(sg);
         for (int tm = 0; tm < (vb.m + T - 1) / T; ++tm)
           for (int tn = 0; tn < (vb.n + T - 1) / T; ++tn) tl[sh].push_back({si, tm, tn, 0});
+        long ucost = 0;
+        for (int q = 0; q < sg.count; ++q) ucost += pair_cost(gm, vb.n, ch.pairs[sg.first + q]);
+        for (int tm = 0; tm < (vb.m + gm.tm - 1) / gm.tm; ++tm)
+          for (int tn = 0; tn < (vb.n + gm.jc - 1) / gm.jc; ++tn) gt.push_back({ucost, {si, tm, tn, gm.code}});
       }
       if (nseg > 1) {
         for (int64_t e0 = 0; e0 < mn; e0 += 256) {
@@ -815,6 +980,19 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       std::string m = g_err;
       return cleanup(rc, m);
     }
+    // costliest tiles first, so the long ones start early and the tail is short
+    std::stable_sort(gt.begin(), gt.end(), [](const std::pair<long, TileRef>& x, const std::pair<long, TileRef>& y) {
+      return x.first > y.first;
+    });
+    std::vector<TileRef> gtiles(gt.size());
+    for (size_t i = 0; i < gt.size(); ++i) gtiles[i] = gt[i].second;
+    rc = up((void**)&pc.d_g, gtiles.data(), sizeof(TileRef) * gtiles.size());
+    pc.n_g = (int)gtiles.size();
+    if (rc) {
+      p->chunks.push_back(pc);
+      std::string m = g_err;
+      return cleanup(rc, m);
+    }
     pc.n_t[0] = (int)tl[0].size();
     pc.n_t[1] = (int)tl[1].size();
     pc.n_red = (int)reds.size();
@@ -838,6 +1016,7 @@ int bsg_plan_destroy(bsg_plan* p) {
     (void)hipFree(c.d_red);
     (void)hipFree(c.d_t[0]);
     (void)hipFree(c.d_t[1]);
+    (void)hipFree(c.d_g);
   }
   (void)hipFree(p->part);
   (void)hipFree(p->aslices);
@@ -890,12 +1069,9 @@ int enqueue_chunk(hipStream_t st, const bsg_plan* p, const PlanChunk& ch, const 
                   double* v, hipEvent_t* ev, bool split) {
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   if (!split) {
-    if (ch.n_t[0] > 0)
-      hipLaunchKernelGGL((k_sigma<2, 2>), dim3(ch.n_t[0]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_t[0],
-                         ch.n_t[0], arena, c, v, p->part);
-    if (ch.n_t[1] > 0)
-      hipLaunchKernelGGL((k_sigma<1, 1>), dim3(ch.n_t[1]), dim3(64), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_t[1],
-                         ch.n_t[1], arena, c, v, p->part);
+    if (ch.n_g > 0)
+      hipLaunchKernelGGL(k_sigma, dim3(ch.n_g), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g, ch.n_g, arena, c,
+                         v, p->part);
   } else {
     if (ch.n_t[0] > 0)
       hipLaunchKernelGGL((k_sigma_split<2, 2>), dim3(ch.n_t[0]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg,
@@ -989,7 +1165,12 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
     }
     k += 2;
     for (auto& ch : p->chunks) {
-      const int present[2] = {(ch.n_t[0] > 0) + (ch.n_t[1] > 0), ch.n_red > 0};
+      int nf = 0;
+      if (split)
+        nf = (ch.n_t[0] > 0) + (ch.n_t[1] > 0);
+      else
+        nf = ch.n_g > 0;
+      const int present[2] = {nf, ch.n_red > 0};
       for (int i = 0; i < 2; ++i) {
         HIPCHK(hipEventElapsedTime(&ms, ev[k + i], ev[k + i + 1]));
         if (present[i]) {
