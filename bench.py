@@ -59,6 +59,33 @@ def cpu_baseline(sb, seconds_budget):
             "sample": f"{n} full H.C of the same superblock ({sb.flops:.3e} flop each), {total:.1f} s"}
 
 
+def copy_calibration(n, reps):
+    """A plain device copy of n doubles (bsg_copy), `reps` times: the known
+    traffic (8n bytes read, 8n written per launch) that scales the HBM
+    counters of a profile run."""
+    lib = block_amd.sigma_lib
+    vp = ctypes.c_void_p
+    lib.bsg_init.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+    lib.bsg_malloc.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(vp)]
+    lib.bsg_copy.argtypes = [vp, vp, vp, ctypes.c_int64]
+    lib.bsg_synchronize.argtypes = [vp]
+    lib.bsg_mfree.argtypes = [vp, vp]
+    lib.bsg_free.argtypes = [vp]
+    ctx, x, y = vp(), vp(), vp()
+    assert lib.bsg_init(ctypes.byref(ctx), 0) == 0
+    assert lib.bsg_malloc(ctx, 8 * n, ctypes.byref(x)) == 0
+    assert lib.bsg_malloc(ctx, 8 * n, ctypes.byref(y)) == 0
+    for _ in range(reps):
+        assert lib.bsg_copy(ctx, x, y, n) == 0
+    assert lib.bsg_synchronize(ctx) == 0
+    lib.bsg_mfree(ctx, x)
+    lib.bsg_mfree(ctx, y)
+    lib.bsg_free(ctx)
+    print(json.dumps({"copy_calibration": {"doubles": n, "bytes_read": 8 * n, "bytes_written": 8 * n,
+                                           "launches": reps}}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,7 +97,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision timing")
     ap.add_argument("--wilson", action="store_true", help="Wilson-warmed blocks instead of the DMRG warm-up")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
+                    help="only run a device copy of N doubles (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration, 5)
     if args.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         return 2

@@ -126,31 +126,33 @@ __device__ inline double bload2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_
 }
 
 // acc[r][t] += A(row blocks of this wave, 4-deep K step at k) * b[t] for the
-// first NV column blocks and the first nr row blocks; A k-major in LDS.
+// first nv column blocks and the first nr row blocks; A k-major in LDS.
 template <int NV>
-__device__ inline void mma_step(int nr, dbl4 (&acc)[2][PB], const double* Al, int P, int arow, int k, int li, int lk,
+__device__ inline void mma_step(int nv, int nr, dbl4 (&acc)[2][PB], const double* Al, int P, int arow, int k, int li, int lk,
                                 const double (&b)[PB]) {
   const double a0 = Al[(k + lk) * P + arow + li];
   double a1 = 0.0;
   if (nr > 1) a1 = Al[(k + lk) * P + arow + 16 + li];
 #pragma unroll
   for (int t = 0; t < NV; ++t) {
-    acc[0][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b[t], acc[0][t], 0, 0, 0);
-    if (nr > 1) acc[1][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b[t], acc[1][t], 0, 0, 0);
+    if (t < nv) {
+      acc[0][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b[t], acc[0][t], 0, 0, 0);
+      if (nr > 1) acc[1][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b[t], acc[1][t], 0, 0, 0);
+    }
   }
 }
 // 4 K steps (16 deep) from k0; steps at or past kend skipped
 template <int NV>
-__device__ inline void mma_run_n(int nr, dbl4 (&acc)[2][PB], const double* Al, int P, int arow, int k0, int kend,
+__device__ inline void mma_run_n(int nv, int nr, dbl4 (&acc)[2][PB], const double* Al, int P, int arow, int k0, int kend,
                                  int li, int lk, const double (&b)[4][PB]) {
 #pragma unroll
   for (int s = 0; s < 4; ++s)
-    if (k0 + 4 * s < kend) mma_step<NV>(nr, acc, Al, P, arow, k0 + 4 * s, li, lk, b[s]);
+    if (k0 + 4 * s < kend) mma_step<NV>(nv, nr, acc, Al, P, arow, k0 + 4 * s, li, lk, b[s]);
 }
 __device__ inline void mma_run(int nv, int nr, dbl4 (&acc)[2][PB], const double* Al, int P, int arow, int k0,
                                int kend, int li, int lk, const double (&b)[4][PB]) {
-  // column slots past nv hold zero B operands; they run without a branch
-  if (nv > 0) mma_run_n<PB>(nr, acc, Al, P, arow, k0, kend, li, lk, b);
+  // column slots past nv are skipped (uniform branches)
+  if (nv > 0) mma_run_n<PB>(nv, nr, acc, Al, P, arow, k0, kend, li, lk, b);
 }
 
 // Tile geometry code in TileRef::pad: bits 0-1 log2(WR), bit 2 (NR - 1).
@@ -170,6 +172,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int tid = blockIdx.x;
   if (tid >= ntiles) return;
   const TileRef tr = tiles[tid];
+  if (tr.item < 0) return;  // padding of the XCD-interleaved order
   const Seg sg = segs[tr.item];
   const bsg_vblock_t vb = vbs[sg.vb];
   const int lwr = tr.pad & 3, nr = ((tr.pad >> 2) & 1) + 1;
@@ -695,6 +698,39 @@ inline long pair_cost(const Geom& g, int n, const bsg_pair_t& pr) {
   return g1 + g2 + 16 * nch;
 }
 
+// Launch order of the grouped kernel's tiles.  Workgroup i runs on XCD
+// i mod 8, each XCD with its own L2: the tiles of one work unit share their
+// C and B' blocks, so a unit's tiles go to one XCD, units are spread over
+// the XCDs by cost (costliest first, to the least loaded XCD), each XCD's
+// queue runs costliest units first, and the queues are interleaved
+// (padding tiles with item -1 keep the positions aligned).
+constexpr int NXCD = 8;
+struct GUnit {
+  long cost = 0;
+  std::vector<TileRef> tiles;
+};
+inline std::vector<TileRef> xcd_order(std::vector<GUnit>& units) {
+  std::vector<int> ord(units.size());
+  for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+    return units[a].cost * (long)units[a].tiles.size() > units[b].cost * (long)units[b].tiles.size();
+  });
+  std::vector<std::vector<TileRef>> q(NXCD);
+  std::vector<long> load(NXCD, 0);
+  for (int u : ord) {
+    int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += units[u].cost * (long)units[u].tiles.size();
+    q[x].insert(q[x].end(), units[u].tiles.begin(), units[u].tiles.end());
+  }
+  size_t len = 0;
+  for (auto& v : q) len = std::max(len, v.size());
+  std::vector<TileRef> out;
+  out.reserve(len * NXCD);
+  for (size_t i = 0; i < len; ++i)
+    for (int x = 0; x < NXCD; ++x) out.push_back(i < q[x].size() ? q[x][i] : TileRef{-1, 0, 0, 0});
+  return out;
+}
+
 // ---------------------------------------------------------------- vector kernels
 constexpr int RB = 256, RMAXB = 1024;
 
@@ -892,7 +928,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     const bsg_chunk_t& ch = chunks[ci];
     PlanChunk pc;
     std::vector<TileRef> tl[2];
-    std::vector<std::pair<long, TileRef>> gt;  // (cost, tile)
+    std::vector<GUnit> gunits;  // the tiles of each work unit
     std::vector<Seg> segs;
     std::vector<RedItem> reds;
     int64_t poff = 0;
@@ -945,8 +981,11 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           for (int tn = 0; tn < (vb.n + T - 1) / T; ++tn) tl[sh].push_back({si, tm, tn, 0});
         long ucost = 0;
         for (int q = 0; q < sg.count; ++q) ucost += pair_cost(gm, vb.n, ch.pairs[sg.first + q]);
+        GUnit gu;
+        gu.cost = ucost;
         for (int tm = 0; tm < (vb.m + gm.tm - 1) / gm.tm; ++tm)
-          for (int tn = 0; tn < (vb.n + gm.jc - 1) / gm.jc; ++tn) gt.push_back({ucost, {si, tm, tn, gm.code}});
+          for (int tn = 0; tn < (vb.n + gm.jc - 1) / gm.jc; ++tn) gu.tiles.push_back({si, tm, tn, gm.code});
+        gunits.push_back(std::move(gu));
       }
       if (nseg > 1) {
         for (int64_t e0 = 0; e0 < mn; e0 += 256) {
@@ -980,12 +1019,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       std::string m = g_err;
       return cleanup(rc, m);
     }
-    // costliest tiles first, so the long ones start early and the tail is short
-    std::stable_sort(gt.begin(), gt.end(), [](const std::pair<long, TileRef>& x, const std::pair<long, TileRef>& y) {
-      return x.first > y.first;
-    });
-    std::vector<TileRef> gtiles(gt.size());
-    for (size_t i = 0; i < gt.size(); ++i) gtiles[i] = gt[i].second;
+    std::vector<TileRef> gtiles = xcd_order(gunits);
     rc = up((void**)&pc.d_g, gtiles.data(), sizeof(TileRef) * gtiles.size());
     pc.n_g = (int)gtiles.size();
     if (rc) {
@@ -1227,6 +1261,13 @@ int bsg_axpy(bsg_context* ctx, double a, const double* x, double* y, int64_t n) 
   if (!ctx || !x || !y || n < 0) return fail(BSG_ERR_ARG, "bsg_axpy: bad args");
   if (n == 0) return BSG_OK;
   hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(256), 0, ctx->stream, a, x, y, n);
+  HIPCHK(hipGetLastError());
+  return BSG_OK;
+}
+int bsg_copy(bsg_context* ctx, const double* x, double* y, int64_t n) {
+  if (!ctx || !x || !y || n < 0) return fail(BSG_ERR_ARG, "bsg_copy: bad args");
+  if (n == 0) return BSG_OK;
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, ctx->stream, x, y, n);
   HIPCHK(hipGetLastError());
   return BSG_OK;
 }

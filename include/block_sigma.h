@@ -138,6 +138,8 @@ int bsg_diag_apply(bsg_context* ctx, const bsg_diag_t* terms_host, int32_t n_ter
 int bsg_dot(bsg_context* ctx, const double* x, const double* y, int64_t n, double* result_host);
 int bsg_axpy(bsg_context* ctx, double a, const double* x, double* y, int64_t n);
 int bsg_scal(bsg_context* ctx, double a, double* x, int64_t n);
+/* y = x (the Wavefunction copies of linear.C:241-244) */
+int bsg_copy(bsg_context* ctx, const double* x, double* y, int64_t n);
 /* x[i] /= (e - diag[i] + shift) where |e - diag[i]| > 1e-12 (linear.C:27-47) */
 int bsg_precondition(bsg_context* ctx, double* x, const double* diag, double e, double shift, int64_t n);
 /* Olsen step (linear.C:50-60): r -= (<C~,r>/<C,C~>) C with C~ = precond(C),

@@ -10,7 +10,7 @@ import block_amd
 def test_exports_every_declared_symbol():
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     hdr = open(os.path.join(root, "include", "block_sigma.h")).read()
-    names = sorted(set(re.findall(r"^\s*(?:int|double|const char\*)\s+(bsg_\w+)\s*\(", hdr, re.M)))
+    names = sorted(set(re.findall(r"^\s*(?:int|int64_t|double|const char\*)\s+(bsg_\w+)\s*\(", hdr, re.M)))
     assert len(names) >= 18
     lib = ctypes.CDLL(os.path.join(root, "block_amd", "libblock_sigma.so"))
     for n in names:
