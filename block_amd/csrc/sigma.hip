@@ -387,6 +387,240 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
+// ---------------------------------------------------------------- FP64 small-block kernel
+// V blocks of at most 32 x 32 (2 x 2 blocks of 16) are latency-bound in the
+// grouped kernel: their pairs are tiny and every pair pays the barriers and
+// memory round trips of a cooperative workgroup.  Here each wavefront owns a
+// whole small V block for one work unit and walks its pairs alone -- no LDS,
+// no barriers -- and a workgroup's 4 waves run 4 unrelated units.  A pair is
+// fused in registers by forming W^T = C^T op(A)^T instead of W: register i of
+// a v_mfma_f64_16x16x4f64 result holds (lane/16 + 4 i, lane%16), i.e. for W^T
+// the element W(lane%16, lane/16 + 4 i), which is exactly the A operand of
+// the second product's K step i.  Identity-B pairs use the direct W = op(A) C
+// orientation, whose result layout matches the V accumulators.
+// SB: 16-blocks per dimension of the V blocks an instantiation takes (the
+// plan uses SB = 2, blocks up to 32 x 32, two waves per SIMD).
+template <int SB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_small(
+    const bsg_vblock_t* __restrict__ vbs,
+                                               const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
+                                               const int4* __restrict__ units, int nunits,
+                                               const double* __restrict__ arena, const double* __restrict__ c,
+                                               double* __restrict__ v, double* __restrict__ part) {
+  const int u = blockIdx.x;
+  if (u >= nunits) return;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int4 un = units[u];
+  const int si = w == 0 ? un.x : (w == 1 ? un.y : (w == 2 ? un.z : un.w));
+  if (si < 0) return;  // no unit for this wave (waves never synchronise here)
+  const Seg sg = segs[si];
+  const bsg_vblock_t vb = vbs[sg.vb];
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int nrb = (vb.m + 15) / 16, nnb = (vb.n + 15) / 16;
+  dbl4 accV[SB][SB];
+#pragma unroll
+  for (int r = 0; r < SB; ++r)
+#pragma unroll
+    for (int t = 0; t < SB; ++t) accV[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q < sg.count; ++q) {
+    const bsg_pair_t pr = prs[sg.first + q];
+    const __amdgpu_buffer_rsrc_t rC = block_rsrc(c + pr.c_off, (int64_t)pr.k1 * pr.k2);
+    const uint32_t crow = (uint32_t)pr.k2 * 8u;
+    if (pr.a_off < 0) {
+      if (pr.b_off < 0) {  // identity A and B': V += alpha C
+#pragma unroll
+        for (int r = 0; r < SB; ++r)
+#pragma unroll
+          for (int t = 0; t < SB; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int row = r * 16 + lk + 4 * i, col = t * 16 + li;
+              const bool ok = row < vb.m && col < vb.n;
+              accV[r][t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
+            }
+        continue;
+      }
+      // identity A: the A operand of GEMM 2 is alpha C(row, k) directly
+      const __amdgpu_buffer_rsrc_t rB = block_rsrc(arena + pr.b_off, (int64_t)vb.n * pr.k2);
+      const int ks = pr.b_trans ? 1 : vb.n, cs = pr.b_trans ? pr.k2 : 1;
+      for (int k0 = 0; k0 < pr.k2; k0 += 4) {
+        const int k = k0 + lk;
+        double a[SB], b[SB];
+#pragma unroll
+        for (int r = 0; r < SB; ++r) {
+          const bool ok = r * 16 + li < vb.m && k < pr.k2;
+          a[r] = pr.alpha * bload(rC, ok ? (uint32_t)((r * 16 + li) * pr.k2 + k) * 8u : OOB);
+        }
+#pragma unroll
+        for (int t = 0; t < SB; ++t) {
+          const bool ok = t * 16 + li < vb.n && k < pr.k2;
+          b[t] = bload(rB, ok ? (uint32_t)(k * ks + (t * 16 + li) * cs) * 8u : OOB);
+        }
+#pragma unroll
+        for (int r = 0; r < SB; ++r)
+#pragma unroll
+          for (int t = 0; t < SB; ++t)
+            if (r < nrb && t < nnb) accV[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[t], accV[r][t], 0, 0, 0);
+      }
+      continue;
+    }
+    const __amdgpu_buffer_rsrc_t rA = block_rsrc(arena + pr.a_off, (int64_t)vb.m * pr.k1);
+    // op(A)(row, k) at row * ar + k * ak inside the block; rows past m masked,
+    // k past k1 meets zero C rows (out of the C block's range).  The C block
+    // of a pair may be larger than the V block: K runs in chunks of KS 4-deep
+    // steps, all loads of a chunk requested before its first MFMA.
+    constexpr int KS = 8;
+    const int nk1 = (pr.k1 + 3) / 4;
+    const int ar = pr.a_trans ? 1 : pr.k1, ak = pr.a_trans ? vb.m : 1;
+    uint32_t aoff[SB];
+#pragma unroll
+    for (int r = 0; r < SB; ++r)
+      aoff[r] = (r < nrb && r * 16 + li < vb.m) ? (uint32_t)((r * 16 + li) * ar + lk * ak) * 8u : OOB;
+    const double aw = pr.alpha_a * pr.alpha;
+    if (pr.b_off < 0) {
+      // identity B': V(:, block t) += alpha alpha_a op(A) C(:, block t)
+#pragma unroll
+      for (int t = 0; t < SB; ++t) {
+        if (t >= nnb) continue;
+        const int col = t * 16 + li;
+        const uint32_t coff = col < pr.k2 ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
+        dbl4 accW[SB];
+#pragma unroll
+        for (int r = 0; r < SB; ++r) accW[r] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int s0 = 0; s0 < nk1; s0 += KS) {
+          double a[SB][KS], cb[KS];
+#pragma unroll
+          for (int s4 = 0; s4 < KS; ++s4) {
+            const bool ok = s0 + s4 < nk1;
+            cb[s4] = ok ? bload2(rC, coff, (uint32_t)(4 * (s0 + s4)) * crow) : 0.0;
+#pragma unroll
+            for (int r = 0; r < SB; ++r) a[r][s4] = ok ? bload2(rA, aoff[r], (uint32_t)((s0 + s4) * ak) * 32u) : 0.0;
+          }
+#pragma unroll
+          for (int s4 = 0; s4 < KS; ++s4)
+            if (s0 + s4 < nk1)
+#pragma unroll
+              for (int r = 0; r < SB; ++r)
+                if (r < nrb) accW[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r][s4], cb[s4], accW[r], 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < SB; ++r)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) accV[r][t][i] += aw * accW[r][i];
+      }
+      continue;
+    }
+    const __amdgpu_buffer_rsrc_t rB = block_rsrc(arena + pr.b_off, (int64_t)vb.n * pr.k2);
+    const int ks = pr.b_trans ? 1 : vb.n, cs = pr.b_trans ? pr.k2 : 1;
+    uint32_t boff[SB];
+#pragma unroll
+    for (int t = 0; t < SB; ++t)
+      boff[t] = (t < nnb && t * 16 + li < vb.n) ? (uint32_t)(lk * ks + (t * 16 + li) * cs) * 8u : OOB;
+    const int njb = (pr.k2 + 15) / 16;
+    if (pr.k1 <= 16 * SB && pr.k2 <= 16 * SB) {
+      // the common case, C block no larger than the V block bound: every
+      // operand of the pair is requested before the first MFMA
+      constexpr int KF = 4 * SB;  // 4-deep K steps
+      double a[SB][KF], ct[SB][KF], b[SB][4][SB];
+#pragma unroll
+      for (int s4 = 0; s4 < KF; ++s4)
+#pragma unroll
+        for (int r = 0; r < SB; ++r) a[r][s4] = s4 < nk1 ? bload2(rA, aoff[r], (uint32_t)(s4 * ak) * 32u) : 0.0;
+#pragma unroll
+      for (int jb = 0; jb < SB; ++jb) {
+        const int col = jb * 16 + li;
+        const uint32_t coff = (jb < njb && col < pr.k2) ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
+#pragma unroll
+        for (int s4 = 0; s4 < KF; ++s4) ct[jb][s4] = (jb < njb && s4 < nk1) ? bload2(rC, coff, (uint32_t)(4 * s4) * crow) : 0.0;
+#pragma unroll
+        for (int t = 0; t < SB; ++t)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            b[jb][i][t] = jb < njb ? bload2(rB, boff[t], (uint32_t)((jb * 4 + i) * ks) * 32u) : 0.0;
+      }
+#pragma unroll
+      for (int jb = 0; jb < SB; ++jb) {
+        if (jb >= njb) continue;
+        dbl4 accT[SB];
+#pragma unroll
+        for (int r = 0; r < SB; ++r) accT[r] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < KF; ++s4)
+          if (s4 < nk1)
+#pragma unroll
+            for (int r = 0; r < SB; ++r)
+              if (r < nrb) accT[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ct[jb][s4], a[r][s4], accT[r], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int r = 0; r < SB; ++r)
+#pragma unroll
+            for (int t = 0; t < SB; ++t)
+              if (r < nrb && t < nnb)
+                accV[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aw * accT[r][i], b[jb][i][t], accV[r][t], 0, 0, 0);
+      }
+      continue;
+    }
+    for (int jb = 0; jb < njb; ++jb) {
+      // operands of W^T block row jb and of its GEMM 2 steps
+      const int col = jb * 16 + li;
+      const uint32_t coff = col < pr.k2 ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
+      double b[4][SB];
+#pragma unroll
+      for (int t = 0; t < SB; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[i][t] = bload2(rB, boff[t], (uint32_t)((jb * 4 + i) * ks) * 32u);
+      // W^T block row jb (zero for W columns past k2)
+      dbl4 accT[SB];
+#pragma unroll
+      for (int r = 0; r < SB; ++r) accT[r] = dbl4{0.0, 0.0, 0.0, 0.0};
+      for (int s0 = 0; s0 < nk1; s0 += KS) {
+        double a[SB][KS], ct[KS];
+#pragma unroll
+        for (int s4 = 0; s4 < KS; ++s4) {
+          const bool ok = s0 + s4 < nk1;
+          ct[s4] = ok ? bload2(rC, coff, (uint32_t)(4 * (s0 + s4)) * crow) : 0.0;
+#pragma unroll
+          for (int r = 0; r < SB; ++r) a[r][s4] = ok ? bload2(rA, aoff[r], (uint32_t)((s0 + s4) * ak) * 32u) : 0.0;
+        }
+#pragma unroll
+        for (int s4 = 0; s4 < KS; ++s4)
+          if (s0 + s4 < nk1)
+#pragma unroll
+            for (int r = 0; r < SB; ++r)
+              if (r < nrb) accT[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ct[s4], a[r][s4], accT[r], 0, 0, 0);
+      }
+      // GEMM 2 over K = jb*16 + 4 i + lane/16, A operand = register i of accT
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < SB; ++r)
+#pragma unroll
+          for (int t = 0; t < SB; ++t)
+            if (r < nrb && t < nnb)
+              accV[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aw * accT[r][i], b[i][t], accV[r][t], 0, 0, 0);
+    }
+  }
+  const bool direct = sg.p_off < 0;
+  double* out = direct ? v + vb.v_off : part + sg.p_off;
+#pragma unroll
+  for (int r = 0; r < SB; ++r)
+#pragma unroll
+    for (int t = 0; t < SB; ++t) {
+      const int col = t * 16 + li;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = r * 16 + lk + 4 * i;
+        if (row < vb.m && col < vb.n) {
+          if (direct)
+            out[(size_t)row * vb.n + col] += accV[r][t][i];
+          else
+            out[(size_t)row * vb.n + col] = accV[r][t][i];
+        }
+      }
+    }
+}
+
 // ---------------------------------------------------------------- split precision
 // Every FP64 operand x is held as three BF16 slices x = x0 + x1 + x2 (x0 the
 // round-to-nearest BF16 of x, x1 of the remainder, x2 of the next remainder),
@@ -688,6 +922,15 @@ inline Geom pick_geom(int m, int n) {
     }
   return best;
 }
+// MFMA instructions of one pair in the small-block kernel plus a latency
+// allowance (segment cost; the wave works alone)
+inline long small_cost(int m, int n, const bsg_pair_t& pr) {
+  auto cdiv = [](long a, long b) { return (a + b - 1) / b; };
+  const long nrb = cdiv(m, 16), nnb = cdiv(n, 16), njb = cdiv(pr.k2, 16);
+  const long g1 = pr.a_off < 0 ? 0 : (pr.b_off < 0 ? nnb : njb) * cdiv(pr.k1, 4) * nrb;
+  const long g2 = pr.b_off < 0 ? 0 : njb * 4 * nrb * nnb;
+  return g1 + g2 + 24;
+}
 // MFMA instructions per wavefront of one pair in a tile (segment cost)
 inline long pair_cost(const Geom& g, int n, const bsg_pair_t& pr) {
   auto cdiv = [](long a, long b) { return (a + b - 1) / b; };
@@ -816,6 +1059,8 @@ struct PlanChunk {
   int n_t[2] = {0, 0};
   TileRef* d_g = nullptr;  // FP64 grouped-kernel tiles (costliest first)
   int n_g = 0;
+  int4* d_s[2] = {nullptr, nullptr};  // FP64 small-block units, 4 per workgroup (SB = 2, 3)
+  int n_s[2] = {0, 0};
   RedItem* d_red = nullptr;
   int n_red = 0;
 };
@@ -919,7 +1164,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
   p->ctx = ctx;
   p->wave_size = wave_size;
   int64_t pmax = 0;
-  const long SEG_COST = 2048;  // MFMAs per wavefront of a work unit before its pair list is split
+  const long SEG_COST = 2048;        // MFMAs per wavefront of a work unit before its pair list is split
+  const long SEG_COST_SMALL = 1024;  // the same for the per-wave small-block units
   auto cleanup = [&](int code, const std::string& m) {
     bsg_plan_destroy(p);
     return fail(code, m);
@@ -929,6 +1175,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     PlanChunk pc;
     std::vector<TileRef> tl[2];
     std::vector<GUnit> gunits;  // the tiles of each work unit
+    std::vector<bsg_pair_t> perm;  // pairs in device order (grouped per V block as given)
+    std::vector<std::pair<long, int>> sunits[2];  // (cost, segment) of small-block units
     std::vector<Seg> segs;
     std::vector<RedItem> reds;
     int64_t poff = 0;
@@ -939,10 +1187,11 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         return cleanup(BSG_ERR_ARG, "bsg_plan_create: vblock out of range");
       const int sh = pick_shape(vb.m, vb.n), T = shape_tm(sh);
       const Geom gm = pick_geom(vb.m, vb.n);
-      // validate pairs, count flops, cut the list into units of ~SEG_COST MFMAs per wave
-      std::vector<std::pair<int, int>> units;
-      int start = 0;
-      long cost = 0;
+      // small blocks (up to 32 x 32) go to the per-wave kernel, except pairs
+      // whose C block is larger than that (or whose A is the identity and B
+      // is not): those run in the grouped kernel with its K pipeline
+      const bool small_vb = std::max(vb.m, vb.n) <= 32;
+      std::vector<int> fast, slow;  // pair indices (absolute) in term order
       for (int q = 0; q < vb.count; ++q) {
         const auto& pr = ch.pairs[vb.first + q];
         if (pr.k1 <= 0 || pr.k2 <= 0 || pr.c_off < 0 || pr.c_off + (int64_t)pr.k1 * pr.k2 > wave_size ||
@@ -958,20 +1207,42 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           p->flops += 2.0 * vb.m * pr.k2 * vb.n;
           p->flops2 += 2.0 * vb.m * pr.k2 * vb.n;
         }
-        cost += pair_cost(gm, vb.n, pr);
-        if (cost >= SEG_COST && q + 1 < vb.count) {
-          units.push_back({start, q + 1 - start});
-          start = q + 1;
-          cost = 0;
-        }
+        const bool f = small_vb && ((pr.a_off >= 0 && pr.k1 <= 32 && pr.k2 <= 32) || (pr.a_off < 0 && pr.b_off < 0));
+        (f ? fast : slow).push_back(vb.first + q);
       }
-      units.push_back({start, vb.count - start});
+      // the block's pairs in device order: the small-kernel ones, then the rest
+      const int base = (int)perm.size();
+      for (int x : fast) perm.push_back(ch.pairs[x]);
+      for (int x : slow) perm.push_back(ch.pairs[x]);
+      // cut each list into work units of about SEG_COST(_SMALL) MFMAs per wave
+      std::vector<std::pair<int, int>> units;  // (first, count) in device order
+      std::vector<char> usmall;
+      auto cut = [&](int first, int n, bool sm) {
+        int start = 0;
+        long cost = 0;
+        for (int q = 0; q < n; ++q) {
+          const auto& pr = perm[first + q];
+          cost += sm ? small_cost(vb.m, vb.n, pr) : pair_cost(gm, vb.n, pr);
+          if (cost >= (sm ? SEG_COST_SMALL : SEG_COST) && q + 1 < n) {
+            units.push_back({first + start, q + 1 - start});
+            usmall.push_back(sm);
+            start = q + 1;
+            cost = 0;
+          }
+        }
+        if (n > start) {
+          units.push_back({first + start, n - start});
+          usmall.push_back(sm);
+        }
+      };
+      cut(base, (int)fast.size(), true);
+      cut(base + (int)fast.size(), (int)slow.size(), false);
       const int nseg = (int)units.size();
       const int64_t mn = (int64_t)vb.m * vb.n;
       for (int u = 0; u < nseg; ++u) {
         Seg sg;
         sg.vb = i;
-        sg.first = vb.first + units[u].first;
+        sg.first = units[u].first;
         sg.count = units[u].second;
         sg.pad = 0;
         sg.p_off = nseg == 1 ? -1 : poff + (int64_t)u * mn;
@@ -979,8 +1250,14 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         segs.push_back(sg);
         for (int tm = 0; tm < (vb.m + T - 1) / T; ++tm)
           for (int tn = 0; tn < (vb.n + T - 1) / T; ++tn) tl[sh].push_back({si, tm, tn, 0});
+        if (usmall[u]) {
+          long sc = 0;
+          for (int q = 0; q < sg.count; ++q) sc += small_cost(vb.m, vb.n, perm[sg.first + q]);
+          sunits[0].push_back({sc, si});
+          continue;
+        }
         long ucost = 0;
-        for (int q = 0; q < sg.count; ++q) ucost += pair_cost(gm, vb.n, ch.pairs[sg.first + q]);
+        for (int q = 0; q < sg.count; ++q) ucost += pair_cost(gm, vb.n, perm[sg.first + q]);
         GUnit gu;
         gu.cost = ucost;
         for (int tm = 0; tm < (vb.m + gm.tm - 1) / gm.tm; ++tm)
@@ -1010,7 +1287,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     };
     int rc = BSG_OK;
     if ((rc = up((void**)&pc.d_vb, ch.vblocks, sizeof(bsg_vblock_t) * ch.n_vblocks)) ||
-        (rc = up((void**)&pc.d_pr, ch.pairs, sizeof(bsg_pair_t) * ch.n_pairs)) ||
+        (rc = up((void**)&pc.d_pr, perm.data(), sizeof(bsg_pair_t) * perm.size())) ||
         (rc = up((void**)&pc.d_seg, segs.data(), sizeof(Seg) * segs.size())) ||
         (rc = up((void**)&pc.d_red, reds.data(), sizeof(RedItem) * reds.size())) ||
         (rc = up((void**)&pc.d_t[0], tl[0].data(), sizeof(TileRef) * tl[0].size())) ||
@@ -1020,6 +1297,24 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       return cleanup(rc, m);
     }
     std::vector<TileRef> gtiles = xcd_order(gunits);
+    // small-block units, costliest first, four per workgroup
+    std::vector<int4> sq[2];
+    for (int k = 0; k < 2; ++k) {
+      std::stable_sort(sunits[k].begin(), sunits[k].end(),
+                       [](const std::pair<long, int>& x, const std::pair<long, int>& y) { return x.first > y.first; });
+      for (size_t i = 0; i < sunits[k].size(); i += 4) {
+        int e[4];
+        for (int j = 0; j < 4; ++j) e[j] = i + j < sunits[k].size() ? sunits[k][i + j].second : -1;
+        sq[k].push_back(make_int4(e[0], e[1], e[2], e[3]));
+      }
+      if ((rc = up((void**)&pc.d_s[k], sq[k].data(), sizeof(int4) * sq[k].size()))) break;
+      pc.n_s[k] = (int)sq[k].size();
+    }
+    if (rc) {
+      p->chunks.push_back(pc);
+      std::string m = g_err;
+      return cleanup(rc, m);
+    }
     rc = up((void**)&pc.d_g, gtiles.data(), sizeof(TileRef) * gtiles.size());
     pc.n_g = (int)gtiles.size();
     if (rc) {
@@ -1051,6 +1346,8 @@ int bsg_plan_destroy(bsg_plan* p) {
     (void)hipFree(c.d_t[0]);
     (void)hipFree(c.d_t[1]);
     (void)hipFree(c.d_g);
+    (void)hipFree(c.d_s[0]);
+    (void)hipFree(c.d_s[1]);
   }
   (void)hipFree(p->part);
   (void)hipFree(p->aslices);
@@ -1106,6 +1403,9 @@ int enqueue_chunk(hipStream_t st, const bsg_plan* p, const PlanChunk& ch, const 
     if (ch.n_g > 0)
       hipLaunchKernelGGL(k_sigma, dim3(ch.n_g), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g, ch.n_g, arena, c,
                          v, p->part);
+    if (ch.n_s[0] > 0)
+      hipLaunchKernelGGL(k_small<2>, dim3(ch.n_s[0]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_s[0],
+                         ch.n_s[0], arena, c, v, p->part);
   } else {
     if (ch.n_t[0] > 0)
       hipLaunchKernelGGL((k_sigma_split<2, 2>), dim3(ch.n_t[0]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg,
@@ -1203,7 +1503,7 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
       if (split)
         nf = (ch.n_t[0] > 0) + (ch.n_t[1] > 0);
       else
-        nf = ch.n_g > 0;
+        nf = (ch.n_g > 0) + (ch.n_s[0] > 0) + (ch.n_s[1] > 0);
       const int present[2] = {nf, ch.n_red > 0};
       for (int i = 0; i < 2; ++i) {
         HIPCHK(hipEventElapsedTime(&ms, ev[k + i], ev[k + i + 1]));
