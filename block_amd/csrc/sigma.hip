@@ -1048,6 +1048,10 @@ int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(204
 struct bsg_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream (default priority) on which the small-block kernel runs
+  // beside the grouped kernel; fork/join events tie it to `stream`
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   double* red = nullptr;  // RMAXB partials + 1 result
 };
 
@@ -1102,6 +1106,16 @@ int bsg_init(bsg_context** ctx, int device) {
     delete c;
     return fail(BSG_ERR_NOMEM, "bsg_init: reduction buffer");
   }
+  if (hipStreamCreate(&c->side) != hipSuccess || hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
+    (void)hipFree(c->red);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(BSG_ERR_HIP, "bsg_init: side stream");
+  }
   *ctx = c;
   return BSG_OK;
 }
@@ -1111,6 +1125,10 @@ int bsg_free(bsg_context* ctx) {
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipFree(ctx->red));
+  HIPCHK(hipStreamSynchronize(ctx->side));
+  HIPCHK(hipEventDestroy(ctx->fork));
+  HIPCHK(hipEventDestroy(ctx->join));
+  HIPCHK(hipStreamDestroy(ctx->side));
   HIPCHK(hipStreamDestroy(ctx->stream));
   delete ctx;
   return BSG_OK;
@@ -1396,16 +1414,25 @@ namespace {
 // Launch one chunk: the fused kernel in both tile shapes, then the ordered
 // reduction.  ev (optional) receives events before the fused kernels, before
 // the reduction and after it.
-int enqueue_chunk(hipStream_t st, const bsg_plan* p, const PlanChunk& ch, const double* arena, const double* c,
+int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, const double* arena, const double* c,
                   double* v, hipEvent_t* ev, bool split) {
+  hipStream_t st = ctx->stream;
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   if (!split) {
+    // the small-block kernel runs on the side stream beside the grouped kernel
+    // (they write disjoint V blocks and partial tiles); the reduction waits
+    // for both
+    if (ch.n_s[0] > 0) {
+      HIPCHK(hipEventRecord(ctx->fork, st));
+      HIPCHK(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+      hipLaunchKernelGGL(k_small<2>, dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_s[0],
+                         ch.n_s[0], arena, c, v, p->part);
+      HIPCHK(hipEventRecord(ctx->join, ctx->side));
+    }
     if (ch.n_g > 0)
       hipLaunchKernelGGL(k_sigma, dim3(ch.n_g), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g, ch.n_g, arena, c,
                          v, p->part);
-    if (ch.n_s[0] > 0)
-      hipLaunchKernelGGL(k_small<2>, dim3(ch.n_s[0]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_s[0],
-                         ch.n_s[0], arena, c, v, p->part);
+    if (ch.n_s[0] > 0) HIPCHK(hipStreamWaitEvent(st, ctx->join, 0));
   } else {
     if (ch.n_t[0] > 0)
       hipLaunchKernelGGL((k_sigma_split<2, 2>), dim3(ch.n_t[0]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg,
@@ -1452,7 +1479,7 @@ int bsg_sigma_apply(bsg_context* ctx, bsg_plan* p, const double* arena, const do
     double* vv = v + (size_t)iv * p->wave_size;
     if (split && (rc = enqueue_csplit(ctx->stream, p, cv))) return rc;
     for (auto& ch : p->chunks)
-      if ((rc = enqueue_chunk(ctx->stream, p, ch, arena, cv, vv, nullptr, split))) return rc;
+      if ((rc = enqueue_chunk(ctx, p, ch, arena, cv, vv, nullptr, split))) return rc;
   }
   return BSG_OK;
 }
@@ -1477,7 +1504,7 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
     HIPCHK(hipEventRecord(ev[q + 1], ctx->stream));
     q += 2;
     for (auto& ch : p->chunks) {
-      if ((rc = enqueue_chunk(ctx->stream, p, ch, arena, c, v, &ev[q], split))) return rc;
+      if ((rc = enqueue_chunk(ctx, p, ch, arena, c, v, &ev[q], split))) return rc;
       q += 3;
     }
   }
