@@ -71,7 +71,7 @@ _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
 
 FP64, SPLIT_BF16 = 0, 1
 SWEEP_COLS = ("sweep", "forward", "M", "E0", "discarded", "seconds", "sigma_seconds", "sigma_flops",
-              "sigma_calls", "split_calls")
+              "sigma_calls", "split_calls", "build_seconds", "davidson_seconds", "renorm_seconds")
 
 
 class BlockError(RuntimeError):

@@ -1021,6 +1021,72 @@ __global__ void k_copy(const double* __restrict__ x, double* __restrict__ y, int
     y[i] = x[i];
 }
 
+// Gram matrix of two vector sets (m <= GMAX vectors of length n, stride ld):
+// per-block partial sums G[i m + j] = sum_x X_i[x] Y_j[x] over the block's
+// chunks, then an ordered sum over blocks (deterministic).
+constexpr int GMAX = 32, GCH = 64, GBLK = 256;
+__global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__ X, const double* __restrict__ Y, int m,
+                                                      int64_t n, int64_t ld, double* __restrict__ part) {
+  __shared__ double xs[GMAX][GCH + 1], ys[GMAX][GCH + 1];
+  double acc[GMAX * GMAX / 256];
+  const int npair = m * m;
+#pragma unroll
+  for (int u = 0; u < GMAX * GMAX / 256; ++u) acc[u] = 0.0;
+  for (int64_t x0 = (int64_t)blockIdx.x * GCH; x0 < n; x0 += (int64_t)gridDim.x * GCH) {
+    for (int e = threadIdx.x; e < m * GCH; e += 256) {
+      const int i = e / GCH, x = e % GCH;
+      const bool ok = x0 + x < n;
+      xs[i][x] = ok ? X[(int64_t)i * ld + x0 + x] : 0.0;
+      ys[i][x] = ok ? Y[(int64_t)i * ld + x0 + x] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < GMAX * GMAX / 256; ++u) {
+      const int pidx = threadIdx.x + 256 * u;
+      if (pidx < npair) {
+        const int i = pidx / m, j = pidx % m;
+        double s = 0.0;
+        for (int x = 0; x < GCH; ++x) s += xs[i][x] * ys[j][x];
+        acc[u] += s;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < GMAX * GMAX / 256; ++u) {
+    const int pidx = threadIdx.x + 256 * u;
+    if (pidx < npair) part[(int64_t)blockIdx.x * npair + pidx] = acc[u];
+  }
+}
+__global__ __launch_bounds__(256) void k_gram_sum(const double* __restrict__ part, int nblk, int npair,
+                                                  double* __restrict__ out) {
+  const int pidx = blockIdx.x * 256 + threadIdx.x;
+  if (pidx >= npair) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * npair + pidx];
+  out[pidx] = s;
+}
+// Y_j[x] = sum_i coef[i mo + j] X_i[x] for j < mo (m, mo <= GMAX)
+__global__ __launch_bounds__(256) void k_lincomb(const double* __restrict__ X, int m, int64_t n, int64_t ld,
+                                                 const double* __restrict__ coef, int mo, double* __restrict__ Y,
+                                                 int64_t ldy) {
+  __shared__ double cs[GMAX * GMAX];
+  for (int e = threadIdx.x; e < m * mo; e += 256) cs[e] = coef[e];
+  __syncthreads();
+  for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (int64_t)gridDim.x * 256) {
+    double xv[GMAX];
+#pragma unroll
+    for (int i = 0; i < GMAX; ++i) xv[i] = i < m ? X[(int64_t)i * ld + x] : 0.0;
+    for (int j = 0; j < mo; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < GMAX; ++i)
+        if (i < m) s += xv[i] * cs[i * mo + j];
+      Y[(int64_t)j * ldy + x] = s;
+    }
+  }
+}
+
 struct DiagGroup {
   int32_t first, count;
 };
@@ -1053,6 +1119,7 @@ struct bsg_context {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   double* red = nullptr;  // RMAXB partials + 1 result
+  double* gpart = nullptr;  // Gram partials (GBLK GMAX^2) + result (GMAX^2) + coefficients (GMAX^2)
 };
 
 struct PlanChunk {
@@ -1106,11 +1173,18 @@ int bsg_init(bsg_context** ctx, int device) {
     delete c;
     return fail(BSG_ERR_NOMEM, "bsg_init: reduction buffer");
   }
+  if (hipMalloc(&c->gpart, sizeof(double) * (GBLK + 2) * GMAX * GMAX) != hipSuccess) {
+    (void)hipFree(c->red);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(BSG_ERR_NOMEM, "bsg_init: Gram scratch");
+  }
   if (hipStreamCreate(&c->side) != hipSuccess || hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
+    (void)hipFree(c->gpart);
     (void)hipFree(c->red);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1126,6 +1200,7 @@ int bsg_free(bsg_context* ctx) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipFree(ctx->red));
   HIPCHK(hipStreamSynchronize(ctx->side));
+  HIPCHK(hipFree(ctx->gpart));
   HIPCHK(hipEventDestroy(ctx->fork));
   HIPCHK(hipEventDestroy(ctx->join));
   HIPCHK(hipStreamDestroy(ctx->side));
@@ -1589,6 +1664,29 @@ int bsg_axpy(bsg_context* ctx, double a, const double* x, double* y, int64_t n) 
   if (n == 0) return BSG_OK;
   hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(256), 0, ctx->stream, a, x, y, n);
   HIPCHK(hipGetLastError());
+  return BSG_OK;
+}
+int bsg_gram(bsg_context* ctx, const double* X, const double* Y, int32_t m, int64_t n, int64_t ld, double* out) {
+  if (!ctx || !X || !Y || !out || m <= 0 || m > GMAX || n < 0 || ld < n) return fail(BSG_ERR_ARG, "bsg_gram: bad args");
+  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(GBLK, (n + GCH - 1) / GCH));
+  double* part = ctx->gpart;
+  double* res = ctx->gpart + (size_t)GBLK * GMAX * GMAX;
+  hipLaunchKernelGGL(k_gram_partial, dim3(nblk), dim3(256), 0, ctx->stream, X, Y, m, n, ld, part);
+  hipLaunchKernelGGL(k_gram_sum, dim3((m * m + 255) / 256), dim3(256), 0, ctx->stream, part, nblk, m * m, res);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, res, sizeof(double) * m * m, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BSG_OK;
+}
+int bsg_lincomb(bsg_context* ctx, const double* X, int32_t m, int64_t n, int64_t ld, const double* coef, int32_t mo,
+                double* Y, int64_t ldy) {
+  if (!ctx || !X || !coef || !Y || m <= 0 || m > GMAX || mo <= 0 || mo > GMAX || n < 0 || ld < n || ldy < n)
+    return fail(BSG_ERR_ARG, "bsg_lincomb: bad args");
+  double* cd = ctx->gpart + (size_t)(GBLK + 1) * GMAX * GMAX;
+  HIPCHK(hipMemcpyAsync(cd, coef, sizeof(double) * m * mo, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n)), dim3(256), 0, ctx->stream, X, m, n, ld, cd, mo, Y, ldy);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // coef is a host buffer the caller may reuse
   return BSG_OK;
 }
 int bsg_copy(bsg_context* ctx, const double* x, double* y, int64_t n) {

@@ -44,6 +44,15 @@ struct GpuFunctor : DavidsonFunctor {
     sig.set_precision(mode);
     return true;
   }
+  bool diagonal(std::vector<double>& d) override {
+    d = sig.diagonal();
+    return true;
+  }
+  bool davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_diag, double normtol, bool useprecond,
+                int deflation_min, int deflation_max, DavidsonStats* st, int max_iter, double ladder) override {
+    block_davidson_device(sig, b, h_diag, normtol, useprecond, deflation_min, deflation_max, st, max_iter, ladder);
+    return true;
+  }
 };
 }  // namespace
 

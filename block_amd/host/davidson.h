@@ -15,6 +15,16 @@ struct DavidsonFunctor {
   // Arithmetic of the following sigmas: 0 = FP64, 1 = split BF16.  Returns
   // false when the functor has no such mode (the default: FP64 only).
   virtual bool set_precision(int mode) { return mode == 0; }
+  // diag(H) in wavefunction order into d, when the functor computes it
+  // itself (SpinBlock::diagonalH); false: the caller computes it
+  virtual bool diagonal(std::vector<double>& d) { return false; }
+  // runs the whole Davidson solve itself (device-resident vectors) with the
+  // arguments of block_davidson; false: the caller runs block_davidson
+  virtual bool davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_diag, double normtol,
+                        bool useprecond, int deflation_min, int deflation_max, struct DavidsonStats* st, int max_iter,
+                        double ladder) {
+    return false;
+  }
 };
 
 struct DavidsonStats {
@@ -23,6 +33,8 @@ struct DavidsonStats {
   int sigma_calls_split = 0;  // of which in split precision
   int ladder_switches = 0;
   double sigma_seconds = 0.0;  // the reference's hmultiply timer (linear.C:220,259)
+  double setup_seconds = 0.0;  // sigma functor construction (work list, operator upload)
+  double diag_seconds = 0.0;   // diagonal of H (preconditioner)
 };
 
 // Restatement of block_davidson with its control flow: Gram-Schmidt of the

@@ -30,29 +30,38 @@ WorkList compile_worklist(const SuperBlock& sb) {
         wl.arena_size += (int64_t)o->data.size();
       }
 
+  // pairs per V block (l, r), in term order and, within a term, in C block
+  // order; V blocks are independent, so they are built in parallel
+  std::vector<std::vector<int>> cblocks(ls.nsec());  // C blocks by left sector
+  for (size_t ic = 0; ic < W.blocks.size(); ++ic) cblocks[W.blocks[ic].l].push_back((int)ic);
+  std::vector<int64_t> toff_a(sb.terms.size()), toff_b(sb.terms.size());
+  for (size_t it = 0; it < sb.terms.size(); ++it) {
+    toff_a[it] = wl.arena_off.at(sb.terms[it].a.op);
+    toff_b[it] = wl.arena_off.at(sb.terms[it].b.op);
+  }
   std::vector<std::vector<bsg_pair_t>> per_v(W.blocks.size());
-  for (const auto& t : sb.terms) {
-    const bool aid = (t.a.op == idL), bid = (t.b.op == idR);
-    const int64_t aoff = wl.arena_off[t.a.op], boff = wl.arena_off[t.b.op];
-    for (size_t ic = 0; ic < W.blocks.size(); ++ic) {  // c block (lQ', rQ')
-      const auto& cb = W.blocks[ic];
-      const int lp = cb.l, rp = cb.r;
-      for (int l = 0; l < ls.nsec(); ++l) {
-        int ia = t.a.find(l, lp);
+#pragma omp parallel for schedule(dynamic)
+  for (long iv = 0; iv < (long)W.blocks.size(); ++iv) {
+    const int l = W.blocks[iv].l, r = W.blocks[iv].r;
+    for (size_t it = 0; it < sb.terms.size(); ++it) {
+      const auto& t = sb.terms[it];
+      const bool aid = (t.a.op == idL), bid = (t.b.op == idR);
+      for (int lp = 0; lp < ls.nsec(); ++lp) {
+        const int ia = t.a.find(l, lp);
         if (ia < 0) continue;
-        for (int r = 0; r < rs.nsec(); ++r) {
-          int iv = W.find(l, r);
-          if (iv < 0) continue;
-          int ib = t.b.find(r, rp);
+        for (int ic : cblocks[lp]) {
+          const auto& cb = W.blocks[ic];
+          const int rp = cb.r;
+          const int ib = t.b.find(r, rp);
           if (ib < 0) continue;
           double f = t.coef * wninej(ls.q[lp].s2, rs.q[rp].s2, St, t.a.k2(), t.b.k2(), 0, ls.q[l].s2, rs.q[r].s2, St);
           if (t.b.fermion() && (ls.q[lp].n & 1)) f = -f;
           f *= t.b.scale(r, rp);
           if (f == 0.0) continue;
           bsg_pair_t p;
-          p.a_off = aid ? -1 : aoff + t.a.op->pat->blocks[ia].off;
+          p.a_off = aid ? -1 : toff_a[it] + t.a.op->pat->blocks[ia].off;
           p.c_off = cb.off;
-          p.b_off = bid ? -1 : boff + t.b.op->pat->blocks[ib].off;
+          p.b_off = bid ? -1 : toff_b[it] + t.b.op->pat->blocks[ib].off;
           p.k1 = ls.dim[lp];
           p.k2 = rs.dim[rp];
           p.a_trans = t.a.dag ? 1 : 0;

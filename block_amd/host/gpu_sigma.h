@@ -8,6 +8,7 @@
 #include <unordered_map>
 
 #include "../../include/block_sigma.h"
+#include "davidson.h"
 #include "superblock.h"
 
 namespace blk {
@@ -73,5 +74,11 @@ class GpuSigma {
   bool split_ready_ = false;
   std::vector<bsg_diag_t> diag_terms_;
 };
+
+// Linear::block_davidson on device vectors with the sigma of h (same control
+// flow and outputs as blk::block_davidson; b, h_diag host vectors in/out).
+void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std::vector<double>& h_diag,
+                           double normtol, bool useprecond, int deflation_min, int deflation_max, DavidsonStats* st,
+                           int max_iter, double ladder);
 
 }  // namespace blk

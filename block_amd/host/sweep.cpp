@@ -199,8 +199,12 @@ Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, c
   std::normal_distribution<double> gauss(0.0, 1.0);
   const long N = sb.wave->size;
   if (N < P.nroots) throw std::runtime_error("run_dmrg: superblock smaller than the number of roots");
+  double t0 = now_s();
   auto hfun = make_sigma(sb);
-  std::vector<double> diag = diagonal_h(sb);
+  double t1 = now_s();
+  std::vector<double> diag;
+  if (!hfun->diagonal(diag)) diag = diagonal_h(sb);
+  double t2 = now_s();
   // guess: lowest diagonal elements with a little seeded noise
   std::vector<long> ord(N);
   std::iota(ord.begin(), ord.end(), 0);
@@ -212,9 +216,12 @@ Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, c
     out.b[r][ord[r]] += 1.0;
   }
   out.e = diag;
-  block_davidson(out.b, out.e, tol, *hfun, true, P.deflation_min, std::max(P.deflation_max, P.nroots + 2), &out.st,
-                 P.davidson_max_iter, P.ladder);
+  const int dmax = std::max(P.deflation_max, P.nroots + 2);
+  if (!hfun->davidson(out.b, out.e, tol, true, P.deflation_min, dmax, &out.st, P.davidson_max_iter, P.ladder))
+    block_davidson(out.b, out.e, tol, *hfun, true, P.deflation_min, dmax, &out.st, P.davidson_max_iter, P.ladder);
   out.e.resize(P.nroots);
+  out.st.setup_seconds = t1 - t0;
+  out.st.diag_seconds = t2 - t1;
   return out;
 }
 
@@ -367,11 +374,18 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       BuildSpec sr;
       sr.normal = false;
       sr.comp = true;
+      double ts = now_s();
       BlockPtr L = build_sum_block(I, *sys, *dot1, target, nR, sl, scr);
       BlockPtr R = build_sum_block(I, *env, *dot2, target, nL, sr, scr);
       SuperBlock sb = make_superblock(I, L, R, target);
       const long N = sb.wave->size;
+      rec.build_seconds += now_s() - ts;
+      ts = now_s();
       Solved sol = solve_superblock(sb, P, S->davidson_tol, make_sigma, rng);
+      rec.davidson_seconds += now_s() - ts - sol.st.sigma_seconds;
+      rec.setup_seconds += sol.st.setup_seconds;
+      rec.diag_seconds += sol.st.diag_seconds;
+      ts = now_s();
       rec.sigma_seconds += sol.st.sigma_seconds;
       rec.sigma_calls += sol.st.sigma_calls;
       rec.sigma_calls_split += sol.st.sigma_calls_split;
@@ -381,6 +395,7 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       double disc = 0.0;
       BlockPtr nsys = truncate_block(I, *L, *sys, *dot1, rho, S->M, scr, &disc);
       rec.max_discarded = std::max(rec.max_discarded, disc);
+      rec.renorm_seconds += now_s() - ts;
       if (forward)
         fwd[l + 1] = nsys;
       else
@@ -397,8 +412,12 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
     rec.seconds = now_s() - t0;
     res.sweeps.push_back(rec);
     if (P.verbose)
-      std::fprintf(stderr, "sweep %d %s M=%d E=%.12f disc=%.2e t=%.2fs\n", sweep, forward ? "fwd" : "bwd", S->M,
-                   rec.energies[0], rec.max_discarded, rec.seconds);
+      std::fprintf(stderr,
+                   "sweep %d %s M=%d E=%.12f disc=%.2e t=%.2fs (build %.2f, sigma %.2f in %ld calls, davidson %.2f "
+                   "[setup %.2f, diag %.2f], renorm %.2f)\n",
+                   sweep, forward ? "fwd" : "bwd", S->M, rec.energies[0], rec.max_discarded, rec.seconds,
+                   rec.build_seconds, rec.sigma_seconds, rec.sigma_calls, rec.davidson_seconds, rec.setup_seconds,
+                   rec.diag_seconds, rec.renorm_seconds);
     res.energies = rec.energies;
     // convergence between consecutive full sweeps once the schedule is exhausted
     if (!forward) {
@@ -435,6 +454,9 @@ void write_sweep_records(const DmrgResult& res, double* out) {
     o[7] = s.sigma_flops;
     o[8] = (double)s.sigma_calls;
     o[9] = (double)s.sigma_calls_split;
+    o[10] = s.build_seconds;
+    o[11] = s.davidson_seconds;
+    o[12] = s.renorm_seconds;
   }
 }
 

@@ -45,6 +45,11 @@ struct SweepRecord {
   long sigma_calls = 0;
   long sigma_calls_split = 0;
   double sigma_flops = 0.0;
+  double build_seconds = 0.0;     // L and R block builds and the superblock term list
+  double davidson_seconds = 0.0;  // solve minus its sigma calls (functor set-up, vectors, guesses)
+  double renorm_seconds = 0.0;    // density matrix, rotation and renormalised operators
+  double setup_seconds = 0.0;     // of davidson_seconds: sigma functor construction
+  double diag_seconds = 0.0;      // of davidson_seconds: diagonal of H
 };
 
 
@@ -55,9 +60,10 @@ struct DmrgResult {
 };
 
 // Columns of the per-sweep records the C entry points write.
-constexpr int kSweepCols = 10;
+constexpr int kSweepCols = 13;
 // sweep, forward, M, E0, max discarded weight, seconds, sigma seconds, sigma
-// flops, sigma calls, split-precision sigma calls
+// flops, sigma calls, split-precision sigma calls, build seconds, other
+// Davidson seconds, renormalisation seconds
 void write_sweep_records(const DmrgResult& res, double* out);
 
 using SigmaFactory = std::function<std::unique_ptr<DavidsonFunctor>(const SuperBlock&)>;

@@ -140,6 +140,13 @@ int bsg_axpy(bsg_context* ctx, double a, const double* x, double* y, int64_t n);
 int bsg_scal(bsg_context* ctx, double a, double* x, int64_t n);
 /* y = x (the Wavefunction copies of linear.C:241-244) */
 int bsg_copy(bsg_context* ctx, const double* x, double* y, int64_t n);
+/* Subspace matrix of the block Davidson (linear.C:265-270): out_host[i*m+j]
+ * = <X_i, Y_j> for m <= 32 device vectors of length n stored with stride ld. */
+int bsg_gram(bsg_context* ctx, const double* X, const double* Y, int32_t m, int64_t n, int64_t ld, double* out_host);
+/* Ritz rotation (linear.C:279-294): Y_j = sum_i coef_host[i*mo+j] X_i for
+ * j < mo; X, Y device vector sets (strides ld, ldy; must not overlap). */
+int bsg_lincomb(bsg_context* ctx, const double* X, int32_t m, int64_t n, int64_t ld, const double* coef_host,
+                int32_t mo, double* Y, int64_t ldy);
 /* x[i] /= (e - diag[i] + shift) where |e - diag[i]| > 1e-12 (linear.C:27-47) */
 int bsg_precondition(bsg_context* ctx, double* x, const double* diag, double e, double shift, int64_t n);
 /* Olsen step (linear.C:50-60): r -= (<C~,r>/<C,C~>) C with C~ = precond(C),
