@@ -44,27 +44,7 @@ struct TileRef {
   int32_t item, tm, tn, pad;
 };
 
-// Tile shapes: a workgroup of WR x WC wavefronts owns a (32 WR) x (32 WC)
-// output tile and every wavefront a 32x32 quarter of it, i.e. 2x2 blocks of
-// 16x16 accumulated with v_mfma_f64_16x16x4f64.  Two shapes are built:
-// 64x64 (4 waves, for large sector blocks) and 32x32 (1 wave, for the many
-// small ones); the host picks per GEMM the shape with the least padding.
-// Lane layout of v_mfma_f64_16x16x4f64 (checked on gfx950): A element
-// (lane%16, lane/16), B element (lane/16, lane%16), result register i holds
-// (lane/16 + 4*i, lane%16).
-template <int WR, int WC>
-struct Shape {
-  static constexpr int TM = 32 * WR, TN = 32 * WC, NT = 64 * WR * WC;
-  static constexpr int LDA = TK + 1;  // LDS row pitch of the A slice (doubles)
-  static constexpr int LDB = TN + 2;  // LDS row pitch of the B slice
-};
 
-template <int WR, int WC>
-__device__ inline void wave_origin(int& wr, int& wc) {
-  const int wave = threadIdx.x >> 6;
-  wr = (wave / WC) * 32;
-  wc = (wave % WC) * 32;
-}
 
 // A work unit: pairs [first, first+count) of one V block.  p_off < 0: the
 // unit owns the whole pair list and accumulates into V; otherwise it writes
@@ -125,6 +105,72 @@ __device__ inline double bload2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
+// ---- split precision: three BF16 slices on the fly ----------------------
+// An FP64 operand x is rounded to FP32 (error <= 2^-24 |x|) and split into
+// BF16 slices h + m + l that represent that FP32 value exactly (8 + 8 + 8
+// significant bits); BF16 keeps FP32's exponent range, so no block scale is
+// needed.  A product keeps the six slice products of order <= 2^-16 (i+j <=
+// 2), summed smallest first in FP32 by v_mfma_f32_16x16x32_bf16 (exact BF16
+// products) for one 32-deep K group, and each group's FP32 result is added
+// into FP64 accumulators.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ inline void split_f32(double x, __bf16& h, __bf16& m, __bf16& l) {
+  const float f = (float)x;
+  h = (__bf16)f;
+  const float r = f - (float)h;
+  m = (__bf16)r;
+  l = (__bf16)(r - (float)m);
+}
+// acc[r][t] += A(r) B(t) over one 32-deep K group for t < nv, r < nr.  Lane
+// (li, lk) supplies 8 values per operand, its K slots 8 lk .. 8 lk + 7; the
+// caller maps slots to K indices the same way for A and B (a bijection onto
+// the group's 32 K), so the sum is over all 32 K.  Result register i of a
+// block holds (4 lk + i, li) (v_mfma_f32_16x16x32_bf16 layout, checked on
+// gfx950).
+template <int NR, int NB>
+__device__ inline void mma_split32(int nv, int nr, dbl4 (&acc)[NR][NB], const double (&a)[NR][8],
+                                   const double (&b)[NB][8]) {
+  // B slices once, A slices one row block at a time (register budget)
+  bf16x8 bh[NB], bm[NB], bl[NB];
+#pragma unroll
+  for (int t = 0; t < NB; ++t)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      __bf16 h, m, l;
+      split_f32(b[t][e], h, m, l);
+      bh[t][e] = h;
+      bm[t][e] = m;
+      bl[t][e] = l;
+    }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r >= nr) continue;
+    bf16x8 ah, am, al;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      __bf16 h, m, l;
+      split_f32(a[r][e], h, m, l);
+      ah[e] = h;
+      am[e] = m;
+      al[e] = l;
+    }
+#pragma unroll
+    for (int t = 0; t < NB; ++t)
+      if (t < nv) {
+        f32x4 x = {0.f, 0.f, 0.f, 0.f};
+        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], x, 0, 0, 0);
+        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[t], x, 0, 0, 0);
+        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], x, 0, 0, 0);
+        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[t], x, 0, 0, 0);
+        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[t], x, 0, 0, 0);
+        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], x, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[r][t][i] += (double)x[i];
+      }
+  }
+}
+
 // acc[r][t] += A(row blocks of this wave, 4-deep K step at k) * b[t] for the
 // first nv column blocks and the first nr row blocks; A k-major in LDS.
 template <int NV>
@@ -156,6 +202,7 @@ __device__ inline void mma_run(int nv, int nr, dbl4 (&acc)[2][PB], const double*
 }
 
 // Tile geometry code in TileRef::pad: bits 0-1 log2(WR), bit 2 (NR - 1).
+template <bool SPLIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sigma(const bsg_vblock_t* __restrict__ vbs,
                                                const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
                                                const TileRef* __restrict__ tiles, int ntiles,
@@ -184,6 +231,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int wr = w / WC, wc = w % WC;
   const int arow = wr * nr * 16;  // first panel row of this wave
   const int li = lane & 15, lk = lane >> 4;
+  // row of accumulator register i inside its 16 x 16 block: lk + 4 i for
+  // v_mfma_f64_16x16x4f64, 4 lk + i for the split-precision BF16 MFMA
+  const int rq = SPLIT ? 4 * lk : lk, rs = SPLIT ? 1 : 4;
   // row blocks of this wave inside the block (0, 1 or 2)
   int nrw = (vb.m - row0 - arow + 15) / 16;
   nrw = nrw < 0 ? 0 : (nrw > nr ? nr : nrw);
@@ -217,7 +267,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
               const int col = colg + (wc + WC * t) * 16 + li;
 #pragma unroll
               for (int i = 0; i < 4; ++i) {
-                const int row = row0 + arow + r * 16 + lk + 4 * i;
+                const int row = row0 + arow + r * 16 + rq + rs * i;
                 const bool ok = r < nr && row < vb.m && col < vb.n;
                 accV[r][t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
               }
@@ -287,25 +337,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
               Ab[p * P + i] = pr.alpha_a * an[u];
             }
           }
-          double bc[4][PB];
-#pragma unroll
-          for (int s4 = 0; s4 < 4; ++s4)
-#pragma unroll
-            for (int t = 0; t < PB; ++t) bc[s4][t] = bn[s4][t];
-          __syncthreads();
-          if (kc > 16) load_c(k0, 16);
-          mma_run(nwb, nrw, accW, Ab, P, arow, 0, kc, li, lk, bc);
-          if (kc > 16) {
+          if constexpr (!SPLIT) {
+            double bc[4][PB];
 #pragma unroll
             for (int s4 = 0; s4 < 4; ++s4)
 #pragma unroll
               for (int t = 0; t < PB; ++t) bc[s4][t] = bn[s4][t];
+            __syncthreads();
+            if (kc > 16) load_c(k0, 16);
+            mma_run(nwb, nrw, accW, Ab, P, arow, 0, kc, li, lk, bc);
+            if (kc > 16) {
+#pragma unroll
+              for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+                for (int t = 0; t < PB; ++t) bc[s4][t] = bn[s4][t];
+            }
+            if (k0 + PKC < pr.k1) {
+              load_a(k0 + PKC);
+              load_c(k0 + PKC, 0);
+            }
+            if (kc > 16) mma_run(nwb, nrw, accW, Ab, P, arow, 16, kc, li, lk, bc);
+          } else {
+            // one 32-deep split step over both 16-deep halves (slot e: K row 4e + lk)
+            double bs[PB][8], as[2][8];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+              for (int t = 0; t < PB; ++t) bs[t][s4] = bn[s4][t];
+            __syncthreads();
+            if (kc > 16) load_c(k0, 16);
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+              for (int t = 0; t < PB; ++t) bs[t][4 + s4] = kc > 16 ? bn[s4][t] : 0.0;
+            if (k0 + PKC < pr.k1) {
+              load_a(k0 + PKC);
+              load_c(k0 + PKC, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+              for (int e = 0; e < 8; ++e) as[r][e] = r < nrw ? Ab[(4 * e + lk) * P + arow + r * 16 + li] : 0.0;
+            if (nwb > 0) mma_split32<2, PB>(nwb, nrw, accW, as, bs);
           }
-          if (k0 + PKC < pr.k1) {
-            load_a(k0 + PKC);
-            load_c(k0 + PKC, 0);
-          }
-          if (kc > 16) mma_run(nwb, nrw, accW, Ab, P, arow, 16, kc, li, lk, bc);
         }
         __syncthreads();  // op(A) buffers done before W (same LDS) is written
         if (pr.b_off < 0) {
@@ -326,7 +400,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             if (r < nr)
 #pragma unroll
               for (int i = 0; i < 4; ++i)
-                Ws[(cb * 16 + li) * P + arow + r * 16 + lk + 4 * i] = pr.alpha * accW[r][t][i];
+                Ws[(cb * 16 + li) * P + arow + r * 16 + rq + rs * i] = pr.alpha * accW[r][t][i];
           }
       }
       __syncthreads();
@@ -354,14 +428,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             for (int t = 0; t < PB; ++t) bn[s4][t] = bload2(rB, boff[t], (uint32_t)(j0 + kg + 4 * s4) * bstep);
         };
         load_b(0);
-        for (int kg = 0; kg < jw; kg += 16) {
-          double bb[4][PB];
+        if constexpr (!SPLIT) {
+          for (int kg = 0; kg < jw; kg += 16) {
+            double bb[4][PB];
 #pragma unroll
-          for (int s4 = 0; s4 < 4; ++s4)
+            for (int s4 = 0; s4 < 4; ++s4)
 #pragma unroll
-            for (int t = 0; t < PB; ++t) bb[s4][t] = bn[s4][t];
-          if (kg + 16 < jw) load_b(kg + 16);
-          mma_run(nvb, nrw, accV, Ws, P, arow, kg, jw, li, lk, bb);
+              for (int t = 0; t < PB; ++t) bb[s4][t] = bn[s4][t];
+            if (kg + 16 < jw) load_b(kg + 16);
+            mma_run(nvb, nrw, accV, Ws, P, arow, kg, jw, li, lk, bb);
+          }
+        } else {
+          // 32-deep split steps (the W chunk width is a multiple of 32)
+          for (int kg = 0; kg < jw; kg += 32) {
+            double bs[PB][8], as[2][8];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+              for (int t = 0; t < PB; ++t) bs[t][s4] = bn[s4][t];
+            if (kg + 16 < jw) load_b(kg + 16);
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+              for (int t = 0; t < PB; ++t) bs[t][4 + s4] = kg + 16 < jw ? bn[s4][t] : 0.0;
+            if (kg + 32 < jw) load_b(kg + 32);
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+              for (int e = 0; e < 8; ++e) as[r][e] = r < nrw ? Ws[(kg + 4 * e + lk) * P + arow + r * 16 + li] : 0.0;
+            if (nvb > 0) mma_split32<2, PB>(nvb, nrw, accV, as, bs);
+          }
         }
       }
       __syncthreads();
@@ -376,7 +472,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       const int col = colg + (wc + WC * t) * 16 + li;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int row = row0 + arow + r * 16 + lk + 4 * i;
+        const int row = row0 + arow + r * 16 + rq + rs * i;
         if (r < nr && row < vb.m && col < vb.n) {
           if (direct)
             out[(size_t)row * vb.n + col] += accV[r][t][i];
@@ -400,13 +496,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // orientation, whose result layout matches the V accumulators.
 // SB: 16-blocks per dimension of the V blocks an instantiation takes (the
 // plan uses SB = 2, blocks up to 32 x 32, two waves per SIMD).
-template <int SB>
+template <int SB, bool SPLIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_small(
-    const bsg_vblock_t* __restrict__ vbs,
-                                               const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
-                                               const int4* __restrict__ units, int nunits,
-                                               const double* __restrict__ arena, const double* __restrict__ c,
-                                               double* __restrict__ v, double* __restrict__ part) {
+    const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
+    const int4* __restrict__ units, int nunits, const double* __restrict__ arena, const double* __restrict__ c,
+    double* __restrict__ v, double* __restrict__ part) {
+  constexpr int KF = 4 * SB;  // 4-deep K steps of a dimension <= 16 SB
+  static_assert(KF == 8, "one 32-deep K group per product");
   const int u = blockIdx.x;
   if (u >= nunits) return;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -416,131 +512,102 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const Seg sg = segs[si];
   const bsg_vblock_t vb = vbs[sg.vb];
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  // row of accumulator register i inside its block (FP64 / split MFMA layout)
+  const int rq = SPLIT ? 4 * lk : lk, rs = SPLIT ? 1 : 4;
   const int nrb = (vb.m + 15) / 16, nnb = (vb.n + 15) / 16;
   dbl4 accV[SB][SB];
 #pragma unroll
   for (int r = 0; r < SB; ++r)
 #pragma unroll
     for (int t = 0; t < SB; ++t) accV[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  // The plan sends here only pairs with k1, k2 <= 16 SB and a non-identity A,
+  // or identity A and B' (the rest of a small block's pairs run in k_sigma).
   for (int q = 0; q < sg.count; ++q) {
     const bsg_pair_t pr = prs[sg.first + q];
     const __amdgpu_buffer_rsrc_t rC = block_rsrc(c + pr.c_off, (int64_t)pr.k1 * pr.k2);
     const uint32_t crow = (uint32_t)pr.k2 * 8u;
-    if (pr.a_off < 0) {
-      if (pr.b_off < 0) {  // identity A and B': V += alpha C
+    if (pr.a_off < 0) {  // identity A and B': V += alpha C
 #pragma unroll
-        for (int r = 0; r < SB; ++r)
+      for (int r = 0; r < SB; ++r)
 #pragma unroll
-          for (int t = 0; t < SB; ++t)
+        for (int t = 0; t < SB; ++t)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const int row = r * 16 + lk + 4 * i, col = t * 16 + li;
-              const bool ok = row < vb.m && col < vb.n;
-              accV[r][t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
-            }
-        continue;
-      }
-      // identity A: the A operand of GEMM 2 is alpha C(row, k) directly
-      const __amdgpu_buffer_rsrc_t rB = block_rsrc(arena + pr.b_off, (int64_t)vb.n * pr.k2);
-      const int ks = pr.b_trans ? 1 : vb.n, cs = pr.b_trans ? pr.k2 : 1;
-      for (int k0 = 0; k0 < pr.k2; k0 += 4) {
-        const int k = k0 + lk;
-        double a[SB], b[SB];
-#pragma unroll
-        for (int r = 0; r < SB; ++r) {
-          const bool ok = r * 16 + li < vb.m && k < pr.k2;
-          a[r] = pr.alpha * bload(rC, ok ? (uint32_t)((r * 16 + li) * pr.k2 + k) * 8u : OOB);
-        }
-#pragma unroll
-        for (int t = 0; t < SB; ++t) {
-          const bool ok = t * 16 + li < vb.n && k < pr.k2;
-          b[t] = bload(rB, ok ? (uint32_t)(k * ks + (t * 16 + li) * cs) * 8u : OOB);
-        }
-#pragma unroll
-        for (int r = 0; r < SB; ++r)
-#pragma unroll
-          for (int t = 0; t < SB; ++t)
-            if (r < nrb && t < nnb) accV[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[t], accV[r][t], 0, 0, 0);
-      }
+          for (int i = 0; i < 4; ++i) {
+            const int row = r * 16 + rq + rs * i, col = t * 16 + li;
+            const bool ok = row < vb.m && col < vb.n;
+            accV[r][t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
+          }
       continue;
     }
     const __amdgpu_buffer_rsrc_t rA = block_rsrc(arena + pr.a_off, (int64_t)vb.m * pr.k1);
     // op(A)(row, k) at row * ar + k * ak inside the block; rows past m masked,
-    // k past k1 meets zero C rows (out of the C block's range).  The C block
-    // of a pair may be larger than the V block: K runs in chunks of KS 4-deep
-    // steps, all loads of a chunk requested before its first MFMA.
-    constexpr int KS = 8;
+    // k past k1 meets zero C rows (out of the C block's range).  Every operand
+    // of the pair is requested before the first MFMA.
     const int nk1 = (pr.k1 + 3) / 4;
     const int ar = pr.a_trans ? 1 : pr.k1, ak = pr.a_trans ? vb.m : 1;
-    uint32_t aoff[SB];
-#pragma unroll
-    for (int r = 0; r < SB; ++r)
-      aoff[r] = (r < nrb && r * 16 + li < vb.m) ? (uint32_t)((r * 16 + li) * ar + lk * ak) * 8u : OOB;
     const double aw = pr.alpha_a * pr.alpha;
+    double a[SB][KF];  // a[r][s]: op(A)(r*16 + li, 4 s + lk)
+#pragma unroll
+    for (int r = 0; r < SB; ++r) {
+      const uint32_t off = (r < nrb && r * 16 + li < vb.m) ? (uint32_t)((r * 16 + li) * ar + lk * ak) * 8u : OOB;
+#pragma unroll
+      for (int s4 = 0; s4 < KF; ++s4) a[r][s4] = s4 < nk1 ? bload2(rA, off, (uint32_t)(s4 * ak) * 32u) : 0.0;
+    }
+    // C operands: ct[jb][s] = C(4 s + lk, jb*16 + li), zero past k2
+    const int njb = (pr.k2 + 15) / 16;
+    double ct[SB][KF];
+#pragma unroll
+    for (int jb = 0; jb < SB; ++jb) {
+      const int col = jb * 16 + li;
+      const uint32_t coff = (jb < njb && col < pr.k2) ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
+#pragma unroll
+      for (int s4 = 0; s4 < KF; ++s4) ct[jb][s4] = (jb < njb && s4 < nk1) ? bload2(rC, coff, (uint32_t)(4 * s4) * crow) : 0.0;
+    }
     if (pr.b_off < 0) {
-      // identity B': V(:, block t) += alpha alpha_a op(A) C(:, block t)
+      // identity B': V(:, block t) += alpha alpha_a op(A) C(:, block t) (k2 = n)
 #pragma unroll
       for (int t = 0; t < SB; ++t) {
         if (t >= nnb) continue;
-        const int col = t * 16 + li;
-        const uint32_t coff = col < pr.k2 ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
-        dbl4 accW[SB];
+        dbl4 accW[SB][1];
 #pragma unroll
-        for (int r = 0; r < SB; ++r) accW[r] = dbl4{0.0, 0.0, 0.0, 0.0};
-        for (int s0 = 0; s0 < nk1; s0 += KS) {
-          double a[SB][KS], cb[KS];
+        for (int r = 0; r < SB; ++r) accW[r][0] = dbl4{0.0, 0.0, 0.0, 0.0};
+        if constexpr (!SPLIT) {
 #pragma unroll
-          for (int s4 = 0; s4 < KS; ++s4) {
-            const bool ok = s0 + s4 < nk1;
-            cb[s4] = ok ? bload2(rC, coff, (uint32_t)(4 * (s0 + s4)) * crow) : 0.0;
-#pragma unroll
-            for (int r = 0; r < SB; ++r) a[r][s4] = ok ? bload2(rA, aoff[r], (uint32_t)((s0 + s4) * ak) * 32u) : 0.0;
-          }
-#pragma unroll
-          for (int s4 = 0; s4 < KS; ++s4)
-            if (s0 + s4 < nk1)
+          for (int s4 = 0; s4 < KF; ++s4)
+            if (s4 < nk1)
 #pragma unroll
               for (int r = 0; r < SB; ++r)
-                if (r < nrb) accW[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r][s4], cb[s4], accW[r], 0, 0, 0);
+                if (r < nrb) accW[r][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r][s4], ct[t][s4], accW[r][0], 0, 0, 0);
+        } else {
+          double bt[1][8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) bt[0][e] = ct[t][e];
+          mma_split32<SB, 1>(1, nrb, accW, a, bt);
         }
 #pragma unroll
         for (int r = 0; r < SB; ++r)
 #pragma unroll
-          for (int i = 0; i < 4; ++i) accV[r][t][i] += aw * accW[r][i];
+          for (int i = 0; i < 4; ++i) accV[r][t][i] += aw * accW[r][0][i];
       }
       continue;
     }
     const __amdgpu_buffer_rsrc_t rB = block_rsrc(arena + pr.b_off, (int64_t)vb.n * pr.k2);
     const int ks = pr.b_trans ? 1 : vb.n, cs = pr.b_trans ? pr.k2 : 1;
-    uint32_t boff[SB];
+    if constexpr (!SPLIT) {
+      double b[SB][4][SB];  // b[jb][i][t]: B'(jb*16 + 4 i + lk, t*16 + li)
 #pragma unroll
-    for (int t = 0; t < SB; ++t)
-      boff[t] = (t < nnb && t * 16 + li < vb.n) ? (uint32_t)(lk * ks + (t * 16 + li) * cs) * 8u : OOB;
-    const int njb = (pr.k2 + 15) / 16;
-    if (pr.k1 <= 16 * SB && pr.k2 <= 16 * SB) {
-      // the common case, C block no larger than the V block bound: every
-      // operand of the pair is requested before the first MFMA
-      constexpr int KF = 4 * SB;  // 4-deep K steps
-      double a[SB][KF], ct[SB][KF], b[SB][4][SB];
+      for (int t = 0; t < SB; ++t) {
+        const uint32_t boff = (t < nnb && t * 16 + li < vb.n) ? (uint32_t)(lk * ks + (t * 16 + li) * cs) * 8u : OOB;
 #pragma unroll
-      for (int s4 = 0; s4 < KF; ++s4)
-#pragma unroll
-        for (int r = 0; r < SB; ++r) a[r][s4] = s4 < nk1 ? bload2(rA, aoff[r], (uint32_t)(s4 * ak) * 32u) : 0.0;
-#pragma unroll
-      for (int jb = 0; jb < SB; ++jb) {
-        const int col = jb * 16 + li;
-        const uint32_t coff = (jb < njb && col < pr.k2) ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
-#pragma unroll
-        for (int s4 = 0; s4 < KF; ++s4) ct[jb][s4] = (jb < njb && s4 < nk1) ? bload2(rC, coff, (uint32_t)(4 * s4) * crow) : 0.0;
-#pragma unroll
-        for (int t = 0; t < SB; ++t)
+        for (int jb = 0; jb < SB; ++jb)
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            b[jb][i][t] = jb < njb ? bload2(rB, boff[t], (uint32_t)((jb * 4 + i) * ks) * 32u) : 0.0;
+            b[jb][i][t] = jb < njb ? bload2(rB, boff, (uint32_t)((jb * 4 + i) * ks) * 32u) : 0.0;
       }
 #pragma unroll
       for (int jb = 0; jb < SB; ++jb) {
         if (jb >= njb) continue;
+        // W^T block row jb: register i of lane (li, lk) is W(li, jb*16 + lk + 4 i)
         dbl4 accT[SB];
 #pragma unroll
         for (int r = 0; r < SB; ++r) accT[r] = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -550,6 +617,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int r = 0; r < SB; ++r)
               if (r < nrb) accT[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ct[jb][s4], a[r][s4], accT[r], 0, 0, 0);
+        // GEMM 2 over K = jb*16 + 4 i + lk, A operand = register i of accT
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -559,46 +627,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
               if (r < nrb && t < nnb)
                 accV[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aw * accT[r][i], b[jb][i][t], accV[r][t], 0, 0, 0);
       }
-      continue;
-    }
-    for (int jb = 0; jb < njb; ++jb) {
-      // operands of W^T block row jb and of its GEMM 2 steps
-      const int col = jb * 16 + li;
-      const uint32_t coff = col < pr.k2 ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
-      double b[4][SB];
+    } else {
+      // W^T blocks (rows: W columns of block jb, columns: W rows of block r);
+      // with the split MFMA layout register i of lane (li, lk) is
+      // W(r*16 + li, jb*16 + 4 lk + i)
+      dbl4 accT[SB][SB];
 #pragma unroll
-      for (int t = 0; t < SB; ++t)
+      for (int jb = 0; jb < SB; ++jb) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) b[i][t] = bload2(rB, boff[t], (uint32_t)((jb * 4 + i) * ks) * 32u);
-      // W^T block row jb (zero for W columns past k2)
-      dbl4 accT[SB];
+        for (int r = 0; r < SB; ++r) accT[jb][r] = dbl4{0.0, 0.0, 0.0, 0.0};
+        if (jb < njb) {
+          double cj[1][8];
+          dbl4 at[1][SB];
 #pragma unroll
-      for (int r = 0; r < SB; ++r) accT[r] = dbl4{0.0, 0.0, 0.0, 0.0};
-      for (int s0 = 0; s0 < nk1; s0 += KS) {
-        double a[SB][KS], ct[KS];
+          for (int e = 0; e < 8; ++e) cj[0][e] = ct[jb][e];
 #pragma unroll
-        for (int s4 = 0; s4 < KS; ++s4) {
-          const bool ok = s0 + s4 < nk1;
-          ct[s4] = ok ? bload2(rC, coff, (uint32_t)(4 * (s0 + s4)) * crow) : 0.0;
+          for (int r = 0; r < SB; ++r) at[0][r] = dbl4{0.0, 0.0, 0.0, 0.0};
+          mma_split32<1, SB>(nrb, 1, at, cj, a);
 #pragma unroll
-          for (int r = 0; r < SB; ++r) a[r][s4] = ok ? bload2(rA, aoff[r], (uint32_t)((s0 + s4) * ak) * 32u) : 0.0;
+          for (int r = 0; r < SB; ++r) accT[jb][r] = at[0][r];
         }
-#pragma unroll
-        for (int s4 = 0; s4 < KS; ++s4)
-          if (s0 + s4 < nk1)
-#pragma unroll
-            for (int r = 0; r < SB; ++r)
-              if (r < nrb) accT[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ct[s4], a[r][s4], accT[r], 0, 0, 0);
       }
-      // GEMM 2 over K = jb*16 + 4 i + lane/16, A operand = register i of accT
+      // GEMM 2 over the 32 W columns: slot e is column (e < 4 ? 4 lk + e : 16 + 4 lk + e - 4)
+      double aa[SB][8], bb[SB][8];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int r = 0; r < SB; ++r)
 #pragma unroll
-        for (int r = 0; r < SB; ++r)
+        for (int e = 0; e < 8; ++e) aa[r][e] = aw * accT[e >> 2][r][e & 3];
 #pragma unroll
-          for (int t = 0; t < SB; ++t)
-            if (r < nrb && t < nnb)
-              accV[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aw * accT[r][i], b[i][t], accV[r][t], 0, 0, 0);
+      for (int t = 0; t < SB; ++t) {
+        const uint32_t boff =
+            (t < nnb && t * 16 + li < vb.n) ? (uint32_t)(4 * lk * ks + (t * 16 + li) * cs) * 8u : OOB;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bb[t][e] = bload2(rB, boff, (uint32_t)(((e >> 2) * 16 + (e & 3)) * ks) * 8u);
+      }
+      mma_split32<SB, SB>(nnb, nrb, accV, aa, bb);
     }
   }
   const bool direct = sg.p_off < 0;
@@ -610,7 +673,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       const int col = t * 16 + li;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int row = r * 16 + lk + 4 * i;
+        const int row = r * 16 + rq + rs * i;
         if (row < vb.m && col < vb.n) {
           if (direct)
             out[(size_t)row * vb.n + col] += accV[r][t][i];
@@ -621,264 +684,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
-// ---------------------------------------------------------------- split precision
-// Every FP64 operand x is held as three BF16 slices x = x0 + x1 + x2 (x0 the
-// round-to-nearest BF16 of x, x1 of the remainder, x2 of the next remainder),
-// so |x - x0 - x1 - x2| <~ 2^-24 |x|.  BF16 carries FP32's 8-bit exponent, so
-// each slice is scaled by its own exponent and no block scale is needed.  A
-// product keeps the six slice products of order <= 2^-16 (a_i b_j, i+j <= 2),
-// summed smallest first into an FP32 accumulator per 32-deep K step
-// (v_mfma_f32_16x16x32_bf16, exact BF16 products), and every step's FP32
-// result is added into FP64 accumulators.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-constexpr int SK = 32;       // K depth of one BF16 MFMA step
-constexpr int SKP = SK + 8;  // LDS pitch (BF16 elements) of the k-contiguous operand rows
 
-__device__ inline void split3(double x, __bf16& h0, __bf16& h1, __bf16& h2) {
-  h0 = (__bf16)(float)x;
-  double r = x - (double)(float)h0;
-  h1 = (__bf16)(float)r;
-  r -= (double)(float)h1;
-  h2 = (__bf16)(float)r;
-}
-
-__global__ __launch_bounds__(256) void k_split3(const double* __restrict__ x, int64_t n, __bf16* __restrict__ out) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    __bf16 h0, h1, h2;
-    split3(x[i], h0, h1, h2);
-    out[i] = h0;
-    out[n + i] = h1;
-    out[2 * n + i] = h2;
-  }
-}
-
-struct Acc64 {
-  double r[2][2][4];
-};
-__device__ inline void acc64_zero(Acc64& a) {
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j)
-      for (int k = 0; k < 4; ++k) a.r[i][j][k] = 0.0;
-}
-
-// (row, col) inside the workgroup tile of element (bi, bj, i) of a
-// v_mfma_f32_16x16x32_bf16 accumulator: register i of lane l holds
-// (4*(l/16) + i, l%16) of its 16x16 block.
-template <int WR, int WC>
-__device__ inline void acc_pos_bf16(int bi, int bj, int i, int& row, int& col) {
-  const int lane = threadIdx.x & 63;
-  int wr, wc;
-  wave_origin<WR, WC>(wr, wc);
-  row = wr + bi * 16 + 4 * (lane >> 4) + i;
-  col = wc + bj * 16 + (lane & 15);
-}
-
-__device__ inline bf16x8 ld8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
-
-// acc += A * B over one 32-deep K step.  A: three slices (stride as) of
-// k-contiguous rows (pitch ap) starting at column ka; B: three slices (stride
-// bs) of k-contiguous columns (pitch bp).  Operand layout of
-// v_mfma_f32_16x16x32_bf16: lane l supplies A(l%16, 8(l/16)..8(l/16)+7) and
-// B(8(l/16)..8(l/16)+7, l%16).
-template <int WR, int WC>
-__device__ inline void mma_split(Acc64& acc, const __bf16* A, int ap, int as, int ka, const __bf16* B, int bp, int bs) {
-  const int lane = threadIdx.x & 63;
-  int wr, wc;
-  wave_origin<WR, WC>(wr, wc);
-  const int li = lane & 15, lk = (lane >> 4) * 8;
-  bf16x8 a[3][2], b[3][2];
-#pragma unroll
-  for (int s = 0; s < 3; ++s)
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      a[s][h] = ld8(A + s * as + (wr + h * 16 + li) * ap + ka + lk);
-      b[s][h] = ld8(B + s * bs + (wc + h * 16 + li) * bp + lk);
-    }
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj) {
-      f32x4 x = {0.f, 0.f, 0.f, 0.f};
-      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][bi], b[0][bj], x, 0, 0, 0);
-      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][bi], b[1][bj], x, 0, 0, 0);
-      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][bi], b[2][bj], x, 0, 0, 0);
-      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][bi], b[0][bj], x, 0, 0, 0);
-      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][bi], b[1][bj], x, 0, 0, 0);
-      x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][bi], b[0][bj], x, 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc.r[bi][bj][i] += (double)x[i];
-    }
-}
-
-template <int WR, int WC>
-struct SplitLds {
-  static constexpr int TM = 32 * WR, TN = 32 * WC, NT = 64 * WR * WC, TJ = TN;
-  static constexpr int WP = TJ + 8;                           // pitch of the W slices
-  static constexpr int RA = (TM > TN ? TM : TN) * SKP;        // one slice of the A (or B) stage
-  static constexpr int RC = TJ * SKP;                         // one slice of the C stage
-  static constexpr int RW = TM * WP;                          // one slice of W
-};
-
-// Split-precision twin of k_sigma: same tiles, segments and pair order; the
-// operator arena comes as three BF16 slices (asl, slice stride an) and the
-// wavefunction as three slices (csl, stride cn) next to its FP64 copy (used
-// for identity-A pairs).  alpha_a and alpha are applied in FP64 to W.
-template <int WR, int WC>
-__global__ __launch_bounds__(64 * WR * WC) void k_sigma_split(const bsg_vblock_t* __restrict__ vbs,
-                                                              const bsg_pair_t* __restrict__ prs,
-                                                              const Seg* __restrict__ segs,
-                                                              const TileRef* __restrict__ tiles, int ntiles,
-                                                              const __bf16* __restrict__ asl, int64_t an,
-                                                              const double* __restrict__ c,
-                                                              const __bf16* __restrict__ csl, int64_t cn,
-                                                              double* __restrict__ v, double* __restrict__ part) {
-  using L = SplitLds<WR, WC>;
-  __shared__ __attribute__((aligned(16))) __bf16 As[3 * L::RA];
-  __shared__ __attribute__((aligned(16))) __bf16 Ct[3 * L::RC];
-  __shared__ __attribute__((aligned(16))) __bf16 Ws[3 * L::RW];
-  __bf16* Bt = As;
-  const int tid = blockIdx.x;
-  if (tid >= ntiles) return;
-  const TileRef tr = tiles[tid];
-  const Seg sg = segs[tr.item];
-  const bsg_vblock_t vb = vbs[sg.vb];
-  const int row0 = tr.tm * L::TM, col0 = tr.tn * L::TN;
-  Acc64 accV;
-  acc64_zero(accV);
-  for (int q = 0; q < sg.count; ++q) {
-    const bsg_pair_t pr = prs[sg.first + q];
-    for (int j0 = 0; j0 < pr.k2; j0 += L::TJ) {
-      if (pr.b_off < 0 && j0 != col0) continue;  // identity B': only the diagonal chunk
-      if (pr.a_off < 0) {
-        // W = alpha C (rows of the tile, columns j0..j0+TJ)
-        const double* Cb = c + pr.c_off;
-        if (pr.b_off < 0) {
-          for (int bi = 0; bi < 2; ++bi)
-            for (int bj = 0; bj < 2; ++bj)
-              for (int i = 0; i < 4; ++i) {
-                int r, cc;
-                acc_pos_bf16<WR, WC>(bi, bj, i, r, cc);
-                int gi = row0 + r, gj = j0 + cc;
-                if (gi < vb.m && gj < pr.k2) accV.r[bi][bj][i] += pr.alpha * Cb[(size_t)gi * pr.k2 + gj];
-              }
-          continue;
-        }
-        for (int e = threadIdx.x; e < L::TM * L::TJ; e += L::NT) {
-          int i = e / L::TJ, j = e % L::TJ;
-          int gi = row0 + i, gj = j0 + j;
-          double w = (gi < vb.m && gj < pr.k2) ? pr.alpha * Cb[(size_t)gi * pr.k2 + gj] : 0.0;
-          __bf16 h0, h1, h2;
-          split3(w, h0, h1, h2);
-          Ws[i * L::WP + j] = h0;
-          Ws[L::RW + i * L::WP + j] = h1;
-          Ws[2 * L::RW + i * L::WP + j] = h2;
-        }
-      } else {
-        Acc64 accW;
-        acc64_zero(accW);
-        for (int k0 = 0; k0 < pr.k1; k0 += SK) {
-          // op(A) rows [row0, row0+TM) x cols [k0, k0+SK), k-contiguous
-          if (!pr.a_trans) {
-            for (int e = threadIdx.x; e < L::TM * SK; e += L::NT) {
-              int i = e / SK, p = e % SK;
-              int gi = row0 + i, gp = k0 + p;
-              bool ok = gi < vb.m && gp < pr.k1;
-              int64_t x = pr.a_off + (int64_t)gi * pr.k1 + gp;
-              for (int s = 0; s < 3; ++s) As[s * L::RA + i * SKP + p] = ok ? asl[s * an + x] : (__bf16)0.0f;
-            }
-          } else {
-            for (int e = threadIdx.x; e < L::TM * SK; e += L::NT) {
-              int p = e / L::TM, i = e % L::TM;
-              int gi = row0 + i, gp = k0 + p;
-              bool ok = gi < vb.m && gp < pr.k1;
-              int64_t x = pr.a_off + (int64_t)gp * vb.m + gi;
-              for (int s = 0; s < 3; ++s) As[s * L::RA + i * SKP + p] = ok ? asl[s * an + x] : (__bf16)0.0f;
-            }
-          }
-          // C rows [k0, k0+SK) x cols [j0, j0+TJ), stored column-wise (k-contiguous)
-          for (int e = threadIdx.x; e < SK * L::TJ; e += L::NT) {
-            int p = e / L::TJ, j = e % L::TJ;
-            int gp = k0 + p, gj = j0 + j;
-            bool ok = gp < pr.k1 && gj < pr.k2;
-            int64_t x = pr.c_off + (int64_t)gp * pr.k2 + gj;
-            for (int s = 0; s < 3; ++s) Ct[s * L::RC + j * SKP + p] = ok ? csl[s * cn + x] : (__bf16)0.0f;
-          }
-          __syncthreads();
-          mma_split<WR, WC>(accW, As, SKP, L::RA, 0, Ct, SKP, L::RC);
-          __syncthreads();
-        }
-        const double sc = pr.alpha_a * pr.alpha;
-        if (pr.b_off < 0) {
-          for (int bi = 0; bi < 2; ++bi)
-            for (int bj = 0; bj < 2; ++bj)
-              for (int i = 0; i < 4; ++i) accV.r[bi][bj][i] += sc * accW.r[bi][bj][i];
-          continue;
-        }
-        for (int bi = 0; bi < 2; ++bi)
-          for (int bj = 0; bj < 2; ++bj)
-            for (int i = 0; i < 4; ++i) {
-              int r, cc;
-              acc_pos_bf16<WR, WC>(bi, bj, i, r, cc);
-              __bf16 h0, h1, h2;
-              split3(sc * accW.r[bi][bj][i], h0, h1, h2);
-              Ws[r * L::WP + cc] = h0;
-              Ws[L::RW + r * L::WP + cc] = h1;
-              Ws[2 * L::RW + r * L::WP + cc] = h2;
-            }
-      }
-      __syncthreads();
-      // V tile += W chunk * B'[j0:j0+TJ, col0:col0+TN]
-      const __bf16* B = asl + pr.b_off;
-      for (int kk0 = 0; kk0 < L::TJ && j0 + kk0 < pr.k2; kk0 += SK) {
-        if (!pr.b_trans) {  // B' stored k2 x n
-          for (int e = threadIdx.x; e < SK * L::TN; e += L::NT) {
-            int p = e / L::TN, n = e % L::TN;
-            int gp = j0 + kk0 + p, gj = col0 + n;
-            bool ok = gp < pr.k2 && gj < vb.n;
-            int64_t x = (int64_t)gp * vb.n + gj;
-            for (int s = 0; s < 3; ++s) Bt[s * L::RA + n * SKP + p] = ok ? B[s * an + x] : (__bf16)0.0f;
-          }
-        } else {  // B' stored n x k2
-          for (int e = threadIdx.x; e < SK * L::TN; e += L::NT) {
-            int n = e / SK, p = e % SK;
-            int gp = j0 + kk0 + p, gj = col0 + n;
-            bool ok = gp < pr.k2 && gj < vb.n;
-            int64_t x = (int64_t)gj * pr.k2 + gp;
-            for (int s = 0; s < 3; ++s) Bt[s * L::RA + n * SKP + p] = ok ? B[s * an + x] : (__bf16)0.0f;
-          }
-        }
-        __syncthreads();
-        mma_split<WR, WC>(accV, Ws, L::WP, L::RW, kk0, Bt, SKP, L::RA);
-        __syncthreads();
-      }
-    }
-  }
-  const bool direct = sg.p_off < 0;
-  double* out = direct ? v + vb.v_off : part + sg.p_off;
-  for (int bi = 0; bi < 2; ++bi)
-    for (int bj = 0; bj < 2; ++bj)
-      for (int i = 0; i < 4; ++i) {
-        int r, cc;
-        acc_pos_bf16<WR, WC>(bi, bj, i, r, cc);
-        r += row0;
-        cc += col0;
-        if (r < vb.m && cc < vb.n) {
-          if (direct)
-            out[(size_t)r * vb.n + cc] += accV.r[bi][bj][i];
-          else
-            out[(size_t)r * vb.n + cc] = accV.r[bi][bj][i];
-        }
-      }
-}
-
-// tile-shape choice: the shape with the smaller padded area (ties: 64x64)
-inline int pick_shape(int m, int n) {
-  long a64 = (long)((m + 63) / 64) * ((n + 63) / 64) * 4096;
-  long a32 = (long)((m + 31) / 32) * ((n + 31) / 32) * 1024;
-  return a32 < a64 ? 1 : 0;
-}
-inline int shape_tm(int s) { return s ? 32 : 64; }
 
 // Tile geometry of the FP64 grouped kernel for an m x n V block: waves WR x
 // WC (WR WC = 4) and 16-row blocks per wave NR, TM = 16 NR WR <= 64 rows and
@@ -1126,8 +932,6 @@ struct PlanChunk {
   bsg_vblock_t* d_vb = nullptr;
   bsg_pair_t* d_pr = nullptr;
   Seg* d_seg = nullptr;
-  TileRef* d_t[2] = {nullptr, nullptr};  // split-precision tiles per shape (0: 64x64, 1: 32x32)
-  int n_t[2] = {0, 0};
   TileRef* d_g = nullptr;  // FP64 grouped-kernel tiles (costliest first)
   int n_g = 0;
   int4* d_s[2] = {nullptr, nullptr};  // FP64 small-block units, 4 per workgroup (SB = 2, 3)
@@ -1143,11 +947,8 @@ struct bsg_plan {
   int64_t wave_size = 0;
   int64_t arena_extent = 0;  // doubles of operator arena the pairs reach
   double flops = 0.0, flops1 = 0.0, flops2 = 0.0;
-  // split precision: BF16 slices of the arena (3 x an) and of one wavefunction
+  // split precision: the arena bsg_plan_split_arena accepted
   const double* split_src = nullptr;
-  __bf16* aslices = nullptr;
-  int64_t an = 0;
-  __bf16* cslices = nullptr;
 };
 
 extern "C" {
@@ -1266,7 +1067,6 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
   for (int ci = 0; ci < n_chunks; ++ci) {
     const bsg_chunk_t& ch = chunks[ci];
     PlanChunk pc;
-    std::vector<TileRef> tl[2];
     std::vector<GUnit> gunits;  // the tiles of each work unit
     std::vector<bsg_pair_t> perm;  // pairs in device order (grouped per V block as given)
     std::vector<std::pair<long, int>> sunits[2];  // (cost, segment) of small-block units
@@ -1278,7 +1078,6 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       if (vb.m <= 0 || vb.n <= 0 || vb.v_off < 0 || vb.v_off + (int64_t)vb.m * vb.n > wave_size || vb.first < 0 ||
           vb.count < 0 || vb.first + vb.count > ch.n_pairs)
         return cleanup(BSG_ERR_ARG, "bsg_plan_create: vblock out of range");
-      const int sh = pick_shape(vb.m, vb.n), T = shape_tm(sh);
       const Geom gm = pick_geom(vb.m, vb.n);
       // small blocks (up to 32 x 32) go to the per-wave kernel, except pairs
       // whose C block is larger than that (or whose A is the identity and B
@@ -1341,8 +1140,6 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         sg.p_off = nseg == 1 ? -1 : poff + (int64_t)u * mn;
         int si = (int)segs.size();
         segs.push_back(sg);
-        for (int tm = 0; tm < (vb.m + T - 1) / T; ++tm)
-          for (int tn = 0; tn < (vb.n + T - 1) / T; ++tn) tl[sh].push_back({si, tm, tn, 0});
         if (usmall[u]) {
           long sc = 0;
           for (int q = 0; q < sg.count; ++q) sc += small_cost(vb.m, vb.n, perm[sg.first + q]);
@@ -1382,9 +1179,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     if ((rc = up((void**)&pc.d_vb, ch.vblocks, sizeof(bsg_vblock_t) * ch.n_vblocks)) ||
         (rc = up((void**)&pc.d_pr, perm.data(), sizeof(bsg_pair_t) * perm.size())) ||
         (rc = up((void**)&pc.d_seg, segs.data(), sizeof(Seg) * segs.size())) ||
-        (rc = up((void**)&pc.d_red, reds.data(), sizeof(RedItem) * reds.size())) ||
-        (rc = up((void**)&pc.d_t[0], tl[0].data(), sizeof(TileRef) * tl[0].size())) ||
-        (rc = up((void**)&pc.d_t[1], tl[1].data(), sizeof(TileRef) * tl[1].size()))) {
+        (rc = up((void**)&pc.d_red, reds.data(), sizeof(RedItem) * reds.size()))) {
       p->chunks.push_back(pc);
       std::string m = g_err;
       return cleanup(rc, m);
@@ -1415,8 +1210,6 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       std::string m = g_err;
       return cleanup(rc, m);
     }
-    pc.n_t[0] = (int)tl[0].size();
-    pc.n_t[1] = (int)tl[1].size();
     pc.n_red = (int)reds.size();
     p->chunks.push_back(pc);
   }
@@ -1436,15 +1229,11 @@ int bsg_plan_destroy(bsg_plan* p) {
     (void)hipFree(c.d_pr);
     (void)hipFree(c.d_seg);
     (void)hipFree(c.d_red);
-    (void)hipFree(c.d_t[0]);
-    (void)hipFree(c.d_t[1]);
     (void)hipFree(c.d_g);
     (void)hipFree(c.d_s[0]);
     (void)hipFree(c.d_s[1]);
   }
   (void)hipFree(p->part);
-  (void)hipFree(p->aslices);
-  (void)hipFree(p->cslices);
   delete p;
   return BSG_OK;
 }
@@ -1455,22 +1244,7 @@ int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* p, const double* arena, int
   if (!ctx || !p || !arena || n < 0) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: bad args");
   if (p->ctx != ctx) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: plan belongs to another context");
   if (n < p->arena_extent) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: arena smaller than the plan's extent");
-  HIPCHK(hipSetDevice(ctx->device));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  if (p->an != n || !p->aslices) {
-    (void)hipFree(p->aslices);
-    p->aslices = nullptr;
-    p->an = 0;
-    hipError_t e = hipMalloc(&p->aslices, sizeof(__bf16) * 3 * std::max<int64_t>(1, n));
-    if (e != hipSuccess) return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: slice arena");
-    p->an = n;
-  }
-  if (!p->cslices) {
-    hipError_t e = hipMalloc(&p->cslices, sizeof(__bf16) * 3 * std::max<int64_t>(1, p->wave_size));
-    if (e != hipSuccess) return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: wavefunction slices");
-  }
-  if (n > 0) hipLaunchKernelGGL(k_split3, dim3(grid_for(n)), dim3(256), 0, ctx->stream, arena, n, p->aslices);
-  HIPCHK(hipGetLastError());
+  // the kernels split operands into BF16 slices in registers: nothing to build
   p->split_src = arena;
   return BSG_OK;
 }
@@ -1493,29 +1267,29 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
                   double* v, hipEvent_t* ev, bool split) {
   hipStream_t st = ctx->stream;
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
-  if (!split) {
-    // the small-block kernel runs on the side stream beside the grouped kernel
-    // (they write disjoint V blocks and partial tiles); the reduction waits
-    // for both
-    if (ch.n_s[0] > 0) {
-      HIPCHK(hipEventRecord(ctx->fork, st));
-      HIPCHK(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
-      hipLaunchKernelGGL(k_small<2>, dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_s[0],
-                         ch.n_s[0], arena, c, v, p->part);
-      HIPCHK(hipEventRecord(ctx->join, ctx->side));
-    }
-    if (ch.n_g > 0)
-      hipLaunchKernelGGL(k_sigma, dim3(ch.n_g), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g, ch.n_g, arena, c,
-                         v, p->part);
-    if (ch.n_s[0] > 0) HIPCHK(hipStreamWaitEvent(st, ctx->join, 0));
-  } else {
-    if (ch.n_t[0] > 0)
-      hipLaunchKernelGGL((k_sigma_split<2, 2>), dim3(ch.n_t[0]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg,
-                         ch.d_t[0], ch.n_t[0], p->aslices, p->an, c, p->cslices, p->wave_size, v, p->part);
-    if (ch.n_t[1] > 0)
-      hipLaunchKernelGGL((k_sigma_split<1, 1>), dim3(ch.n_t[1]), dim3(64), 0, st, ch.d_vb, ch.d_pr, ch.d_seg,
-                         ch.d_t[1], ch.n_t[1], p->aslices, p->an, c, p->cslices, p->wave_size, v, p->part);
+  // the small-block kernel runs on the side stream beside the grouped kernel
+  // (they write disjoint V blocks and partial tiles); the reduction waits
+  // for both
+  if (ch.n_s[0] > 0) {
+    HIPCHK(hipEventRecord(ctx->fork, st));
+    HIPCHK(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+    if (split)
+      hipLaunchKernelGGL((k_small<2, true>), dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
+                         ch.d_s[0], ch.n_s[0], arena, c, v, p->part);
+    else
+      hipLaunchKernelGGL((k_small<2, false>), dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
+                         ch.d_s[0], ch.n_s[0], arena, c, v, p->part);
+    HIPCHK(hipEventRecord(ctx->join, ctx->side));
   }
+  if (ch.n_g > 0) {
+    if (split)
+      hipLaunchKernelGGL(k_sigma<true>, dim3(ch.n_g), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g, ch.n_g,
+                         arena, c, v, p->part);
+    else
+      hipLaunchKernelGGL(k_sigma<false>, dim3(ch.n_g), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g, ch.n_g,
+                         arena, c, v, p->part);
+  }
+  if (ch.n_s[0] > 0) HIPCHK(hipStreamWaitEvent(st, ctx->join, 0));
   if (ev) HIPCHK(hipEventRecord(ev[1], st));
   if (ch.n_red > 0) hipLaunchKernelGGL(k_reduce, dim3(ch.n_red), dim3(256), 0, st, ch.d_red, p->part, v);
   if (ev) HIPCHK(hipEventRecord(ev[2], st));
@@ -1523,18 +1297,10 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
   return BSG_OK;
 }
 
-// split precision: the wavefunction's BF16 slices for this application
-int enqueue_csplit(hipStream_t st, const bsg_plan* p, const double* c) {
-  if (p->wave_size > 0)
-    hipLaunchKernelGGL(k_split3, dim3(grid_for(p->wave_size)), dim3(256), 0, st, c, p->wave_size, p->cslices);
-  HIPCHK(hipGetLastError());
-  return BSG_OK;
-}
-
 int check_precision(const bsg_plan* p, const double* arena, int32_t precision) {
   if (precision == BSG_FP64) return BSG_OK;
   if (precision != BSG_SPLIT_BF16) return fail(BSG_ERR_ARG, "unknown precision mode");
-  if (!p->aslices || p->split_src != arena)
+  if (p->split_src != arena)
     return fail(BSG_ERR_ARG, "split precision needs bsg_plan_split_arena on this arena first");
   return BSG_OK;
 }
@@ -1552,7 +1318,6 @@ int bsg_sigma_apply(bsg_context* ctx, bsg_plan* p, const double* arena, const do
   for (int iv = 0; iv < nvec; ++iv) {
     const double* cv = c + (size_t)iv * p->wave_size;
     double* vv = v + (size_t)iv * p->wave_size;
-    if (split && (rc = enqueue_csplit(ctx->stream, p, cv))) return rc;
     for (auto& ch : p->chunks)
       if ((rc = enqueue_chunk(ctx, p, ch, arena, cv, vv, nullptr, split))) return rc;
   }
@@ -1575,7 +1340,6 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
   size_t q = 1;
   for (int r = 0; r < reps; ++r) {
     HIPCHK(hipEventRecord(ev[q], ctx->stream));
-    if (split && (rc = enqueue_csplit(ctx->stream, p, c))) return rc;
     HIPCHK(hipEventRecord(ev[q + 1], ctx->stream));
     q += 2;
     for (auto& ch : p->chunks) {
@@ -1602,10 +1366,7 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
     k += 2;
     for (auto& ch : p->chunks) {
       int nf = 0;
-      if (split)
-        nf = (ch.n_t[0] > 0) + (ch.n_t[1] > 0);
-      else
-        nf = (ch.n_g > 0) + (ch.n_s[0] > 0) + (ch.n_s[1] > 0);
+      nf = (ch.n_g > 0) + (ch.n_s[0] > 0);
       const int present[2] = {nf, ch.n_red > 0};
       for (int i = 0; i < 2; ++i) {
         HIPCHK(hipEventElapsedTime(&ms, ev[k + i], ev[k + i + 1]));

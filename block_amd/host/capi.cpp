@@ -90,7 +90,7 @@ void* blkh_snapshot_dmrg(void* ints, int nelec, int twoS, int irrep, int nroots,
     P.twoS = twoS;
     P.irrep = irrep;
     P.nroots = nroots;
-    P.schedule = {{0, M, 1e-7, 1e-6}};
+    P.schedule = {{0, M, 1e-7, 0.0}};  // noise-free, as in converged sweeps
     P.verbose = verbose;
     GpuContext ctx(device);
     SuperBlock sb = dmrg_snapshot(I, P, [&](const SuperBlock& s) {
