@@ -15,6 +15,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import tempfile
 import time
@@ -59,6 +60,22 @@ def cpu_baseline(sb, seconds_budget):
             "sample": f"{n} full H.C of the same superblock ({sb.flops:.3e} flop each), {total:.1f} s"}
 
 
+def traffic_from_profile(config, M, wave_size, pairs):
+    """HBM bytes per H.C of the fused kernels, from the committed counter
+    summary of this workload (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+    passes, each scaled by the same counter on a plain copy of known size).
+    None when no summary for this exact superblock is committed."""
+    path = os.path.join(ROOT, "profiles", f"k_sigma_fp64_{config}_M{M}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic:"):
+            kv = dict(t.split("=") for t in line.split()[1:])
+            if int(kv["wave_size"]) == wave_size and int(kv["gemm_pairs"]) == pairs:
+                return int(kv["bytes_per_step"])
+    return None
+
+
 def copy_calibration(n, reps):
     """A plain device copy of n doubles (bsg_copy), `reps` times: the known
     traffic (8n bytes read, 8n written per launch) that scales the HBM
@@ -86,6 +103,88 @@ def copy_calibration(n, reps):
     return 0
 
 
+def _short(name):
+    """k_sigma<false> from the demangled signature."""
+    name = name.replace("void ", "").replace("(anonymous namespace)::", "")
+    return name.split("(")[0]
+
+
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+    files = glob.glob(os.path.join(d, f"*{suffix}"))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {d}")
+    return list(csv.DictReader(open(files[0])))
+
+
+def _last_by_kernel(rows, last, pattern):
+    by = {}
+    for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"])):
+        k = _short(r["Kernel_Name"])
+        if re.search(pattern, k):
+            by.setdefault(k, []).append(r)
+    return {k: v[-last:] for k, v in by.items()}
+
+
+def summarize_trace(d, last, pattern):
+    """Markdown table of the last `last` dispatches of every kernel matching
+    `pattern` in a rocprofv3 --kernel-trace CSV (the bench's own calls are the
+    last ones of a run: the superblock build before them also runs these
+    kernels on smaller superblocks)."""
+    print("| kernel | dispatches | average ms | min ms | max ms | grid | workgroup | VGPR | AGPR | SGPR | LDS B | scratch B |")
+    print("|---|---|---|---|---|---|---|---|---|---|---|---|")
+    for k, rs in _last_by_kernel(_csv_rows(d, "kernel_trace.csv"), last, pattern).items():
+        t = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in rs]
+        r = rs[-1]
+        print(f"| {k} | {len(t)} | {sum(t) / len(t):.4f} | {min(t):.4f} | {max(t):.4f} | {r['Grid_Size_X']} | "
+              f"{r['Workgroup_Size_X']} | {r['VGPR_Count']} | {r['Accum_VGPR_Count']} | {r['SGPR_Count']} | "
+              f"{r['LDS_Block_Size']} | {r['Scratch_Size']} |")
+
+
+def summarize_counters(d, last, pattern):
+    """Mean of every counter over the last `last` dispatches per kernel
+    (rocprofv3 --pmc CSV)."""
+    rows = _csv_rows(d, "counter_collection.csv")
+    out = {}
+    by = {}
+    for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"])):
+        k = _short(r["Kernel_Name"])
+        if re.search(pattern, k):
+            by.setdefault((k, r["Counter_Name"]), []).append(float(r["Counter_Value"]))
+    for (k, cn), v in by.items():
+        out.setdefault(k, {})[cn] = sum(v[-last:]) / len(v[-last:])
+    return out
+
+
+def summarize_traffic(dirs, last, pattern, copy_doubles):
+    """HBM bytes per dispatch from a FETCH_SIZE and a WRITE_SIZE pass (KiB per
+    dispatch), each scaled by the same counter on the copy calibration (a
+    copy of copy_doubles doubles: 8 copy_doubles bytes read and written)."""
+    fetch, write, cfetch, cwrite = (summarize_counters(d, last if i < 2 else 5, pattern if i < 2 else "k_copy")
+                                    for i, d in enumerate(dirs))
+    kib = 8 * copy_doubles / 1024
+    ff = kib / fetch_val(cfetch, "FETCH_SIZE")
+    fw = kib / fetch_val(cwrite, "WRITE_SIZE")
+    print(f"copy calibration: {copy_doubles} doubles = {kib:.0f} KiB each way; FETCH_SIZE reads "
+          f"{fetch_val(cfetch, 'FETCH_SIZE'):.0f} KiB (factor {ff:.3f}), WRITE_SIZE reads "
+          f"{fetch_val(cwrite, 'WRITE_SIZE'):.0f} KiB (factor {fw:.3f})")
+    print()
+    print("| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | read bytes (scaled) | written bytes (scaled) |")
+    print("|---|---|---|---|---|")
+    tot = 0.0
+    for k in sorted(fetch):
+        fr, wr = fetch[k]["FETCH_SIZE"], write[k]["WRITE_SIZE"]
+        tot += (fr * ff + wr * fw) * 1024
+        print(f"| {k} | {fr:.0f} | {wr:.0f} | {fr * ff * 1024:.4g} | {wr * fw * 1024:.4g} |")
+    print()
+    print(f"total per step: {tot:.4g} bytes")
+
+
+def fetch_val(d, name):
+    return next(iter(d.values()))[name]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,7 +198,16 @@ def main():
     ap.add_argument("--wilson", action="store_true", help="Wilson-warmed blocks instead of the DMRG warm-up")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run a device copy of N doubles (HBM counter calibration)")
+    ap.add_argument("--summarize-trace", metavar="DIR", help="table of a rocprofv3 kernel trace (no GPU run)")
+    ap.add_argument("--summarize-traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
+                    help="HBM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (no GPU run)")
+    ap.add_argument("--last", type=int, default=6, help="dispatches per kernel to summarise (the last ones)")
+    ap.add_argument("--kernels", default="k_sigma|k_small|k_reduce", help="regex of the kernel names to summarise")
     args = ap.parse_args()
+    if args.summarize_trace:
+        return summarize_trace(args.summarize_trace, args.last, args.kernels) or 0
+    if args.summarize_traffic:
+        return summarize_traffic(args.summarize_traffic, args.last, args.kernels, args.copy_calibration) or 0
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration, 5)
     if args.gpus != 1:
@@ -132,16 +240,20 @@ def main():
     value = flops * args.steps / (wall_ms * 1e-3) / 1e12
     # dominant kernel: the fused GEMM-pair kernel (k_reduce is the ordered
     # partial-tile sum); its algorithmic flops are both GEMMs of every pair
-    kname = "k_sigma"
+    # (k_sigma, the grouped kernel, with k_small, the small-block kernel,
+    # running beside it on a second stream: timed fork to join)
+    kname = "k_sigma+k_small"
     kflops = f1 + f2
     k = 0
     achieved = kflops * args.steps / (kern_ms[0] * 1e-3) / 1e12
+    traffic = traffic_from_profile(args.config, M, sb.wave_size, h.pairs)
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_MFMA, 5), "traffic": None, "kernel": kname,
+                "frac": round(achieved / PEAK_FP64_MFMA, 5), "traffic": traffic, "kernel": kname,
                 "kernel_ms_per_step": round(kern_ms[k] / args.steps, 4),
                 "launches_per_step": launches[k] / args.steps,
-                "kernel_ms_per_step_all": {"k_sigma": round(kern_ms[0] / args.steps, 4),
-                                           "k_reduce": round(kern_ms[1] / args.steps, 4)}}
+                "kernel_ms_per_step_all": {"k_sigma+k_small": round(kern_ms[0] / args.steps, 4),
+                                           "k_reduce": round(kern_ms[1] / args.steps, 4),
+                                           "k_small": round(kern_ms[2] / args.steps, 4)}}
     # split precision (BF16 slices, FP32 steps, FP64 sums) on the same superblock:
     # throughput in FP64-equivalent algorithmic flops and its error against the
     # FP64 GPU sigma (the FP64 path is parity-tested against the oracle)
@@ -154,9 +266,9 @@ def main():
         h.precision = block_amd.FP64
         mixed = {"tflops": round(flops * args.steps / (s_wall * 1e-3) / 1e12, 4),
                  "ms_per_step": round(s_wall / args.steps, 4),
-                 "kernel_ms_per_step": {"k_sigma_split": round(s_kern[0] / args.steps, 4),
+                 "kernel_ms_per_step": {"k_sigma+k_small": round(s_kern[0] / args.steps, 4),
                                         "k_reduce": round(s_kern[1] / args.steps, 4),
-                                        "k_split3": round(s_kern[2] / args.steps, 4)},
+                                        "k_small": round(s_kern[2] / args.steps, 4)},
                  "speedup_vs_fp64": round(wall_ms / s_wall, 3),
                  "sigma_rel_err_vs_fp64": float(np.linalg.norm(vs - v64) / np.linalg.norm(v64)),
                  "rayleigh_err_vs_fp64_Eh": float(abs(c @ vs - c @ v64))}

@@ -1260,9 +1260,11 @@ int bsg_plan_phase_flops(const bsg_plan* p, double* f1, double* f2) {
 }  // extern "C"
 
 namespace {
-// Launch one chunk: the fused kernel in both tile shapes, then the ordered
-// reduction.  ev (optional) receives events before the fused kernels, before
-// the reduction and after it.
+// Launch one chunk: the grouped kernel on the context stream with the
+// small-block kernel beside it on the side stream, then the ordered
+// reduction.  ev (optional) receives 5 events: before the fused kernels,
+// around the small-block kernel (side stream), before and after the
+// reduction.
 int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, const double* arena, const double* c,
                   double* v, hipEvent_t* ev, bool split) {
   hipStream_t st = ctx->stream;
@@ -1273,12 +1275,14 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
   if (ch.n_s[0] > 0) {
     HIPCHK(hipEventRecord(ctx->fork, st));
     HIPCHK(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+    if (ev) HIPCHK(hipEventRecord(ev[1], ctx->side));
     if (split)
       hipLaunchKernelGGL((k_small<2, true>), dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
                          ch.d_s[0], ch.n_s[0], arena, c, v, p->part);
     else
       hipLaunchKernelGGL((k_small<2, false>), dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
                          ch.d_s[0], ch.n_s[0], arena, c, v, p->part);
+    if (ev) HIPCHK(hipEventRecord(ev[2], ctx->side));
     HIPCHK(hipEventRecord(ctx->join, ctx->side));
   }
   if (ch.n_g > 0) {
@@ -1290,9 +1294,9 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
                          arena, c, v, p->part);
   }
   if (ch.n_s[0] > 0) HIPCHK(hipStreamWaitEvent(st, ctx->join, 0));
-  if (ev) HIPCHK(hipEventRecord(ev[1], st));
+  if (ev) HIPCHK(hipEventRecord(ev[3], st));
   if (ch.n_red > 0) hipLaunchKernelGGL(k_reduce, dim3(ch.n_red), dim3(256), 0, st, ch.d_red, p->part, v);
-  if (ev) HIPCHK(hipEventRecord(ev[2], st));
+  if (ev) HIPCHK(hipEventRecord(ev[4], st));
   HIPCHK(hipGetLastError());
   return BSG_OK;
 }
@@ -1332,21 +1336,16 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
   if (rc) return rc;
   const bool split = precision == BSG_SPLIT_BF16;
   const size_t nch = p->chunks.size();
-  // per rep: [split start, split end] then 3 events per chunk
-  const size_t per = 2 + nch * 3;
-  std::vector<hipEvent_t> ev((size_t)reps * per + 2);
+  constexpr size_t per = 5;  // events per chunk launch (enqueue_chunk)
+  std::vector<hipEvent_t> ev((size_t)reps * nch * per + 2);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipEventRecord(ev[0], ctx->stream));
   size_t q = 1;
-  for (int r = 0; r < reps; ++r) {
-    HIPCHK(hipEventRecord(ev[q], ctx->stream));
-    HIPCHK(hipEventRecord(ev[q + 1], ctx->stream));
-    q += 2;
+  for (int r = 0; r < reps; ++r)
     for (auto& ch : p->chunks) {
       if ((rc = enqueue_chunk(ctx, p, ch, arena, c, v, &ev[q], split))) return rc;
-      q += 3;
+      q += per;
     }
-  }
   HIPCHK(hipEventRecord(ev[q], ctx->stream));
   HIPCHK(hipEventSynchronize(ev[q]));
   float ms = 0.f;
@@ -1357,27 +1356,21 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
     launches[i] = 0;
   }
   size_t k = 1;
-  for (int r = 0; r < reps; ++r) {
-    if (split) {
-      HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
-      kernel_ms[2] += ms;
-      launches[2] += 1;
-    }
-    k += 2;
+  for (int r = 0; r < reps; ++r)
     for (auto& ch : p->chunks) {
-      int nf = 0;
-      nf = (ch.n_g > 0) + (ch.n_s[0] > 0);
-      const int present[2] = {nf, ch.n_red > 0};
-      for (int i = 0; i < 2; ++i) {
-        HIPCHK(hipEventElapsedTime(&ms, ev[k + i], ev[k + i + 1]));
-        if (present[i]) {
-          kernel_ms[i] += ms;
-          launches[i] += present[i];
-        }
+      const bool fused = ch.n_g > 0 || ch.n_s[0] > 0;
+      // [0] fork to join of both fused kernels, [1] reduction, [2] the
+      // small-block kernel alone
+      const int from[3] = {0, 3, 1}, to[3] = {3, 4, 2};
+      const bool present[3] = {fused, ch.n_red > 0, ch.n_s[0] > 0};
+      for (int i = 0; i < 3; ++i) {
+        if (!present[i]) continue;
+        HIPCHK(hipEventElapsedTime(&ms, ev[k + from[i]], ev[k + to[i]]));
+        kernel_ms[i] += ms;
+        launches[i] += 1;
       }
-      k += 3;
+      k += per;
     }
-  }
   for (auto& e : ev) HIPCHK(hipEventDestroy(e));
   return BSG_OK;
 }

@@ -54,6 +54,16 @@ void write_fcidump(const std::string& path, const Integrals& I);
 struct SpinBlock;
 using BlockPtr = std::shared_ptr<SpinBlock>;
 
+// Rotation that made a renormalised block from its composite (block (x) dot)
+// basis (the rotation matrices Block saves per block, rotationmat.C:21-60):
+// kept sector s of `kept` is composite sector oldidx[s]; rot[Q] is the
+// d_Q x m_Q row-major matrix of composite sector Q (m_Q = 0: dropped).
+struct RotRecord {
+  SIPtr composite, kept;
+  std::vector<int> oldidx;
+  std::vector<std::vector<double>> rot;
+};
+
 struct SpinBlock {
   int norb = 0;
   std::vector<int> sites;      // sorted orbital indices
@@ -66,6 +76,8 @@ struct SpinBlock {
   // single-site helpers kept on dot blocks: "des" (D_d), "n" (n_d),
   // "cn" (C_d n_d), "dd0" ([D_d D_d]^0)
   std::map<std::string, OpPtr> extra;
+  // set on renormalised blocks: the rotation from their composite basis
+  std::shared_ptr<const RotRecord> rotrec;
 
   bool in(int i) const { return inblock[i] != 0; }
   size_t pidx(int i, int j, int K) const { return ((size_t)i * norb + j) * 2 + (K >> 1); }  // K doubled (0 or 2)

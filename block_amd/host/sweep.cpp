@@ -1,4 +1,5 @@
 #include "sweep.h"
+#include "guess.h"
 
 #include <algorithm>
 #include <chrono>
@@ -191,11 +192,12 @@ struct Solved {
   std::vector<std::vector<double>> b;  // wave solutions
   std::vector<double> e;               // energies
   DavidsonStats st;
+  bool guessed = false;  // started from the previous step's wavefunction
 };
 
 // Solver::solve_wavefunction (solver.C:19-143): diagonal, guess, block Davidson
 Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, const SigmaFactory& make_sigma,
-                        std::mt19937_64& rng) {
+                        std::mt19937_64& rng, const std::vector<std::vector<double>>* guess = nullptr) {
   std::normal_distribution<double> gauss(0.0, 1.0);
   const long N = sb.wave->size;
   if (N < P.nroots) throw std::runtime_error("run_dmrg: superblock smaller than the number of roots");
@@ -214,6 +216,16 @@ Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, c
   for (int r = 0; r < P.nroots; ++r) {
     for (long x = 0; x < N; ++x) out.b[r][x] = 1e-3 * gauss(rng);
     out.b[r][ord[r]] += 1.0;
+  }
+  // the previous step's wavefunction carried into this basis, when every
+  // root kept most of its norm (guess_wavefunctions, guess_wavefunction.C:275-400)
+  if (guess && (int)guess->size() == P.nroots) {
+    bool ok = true;
+    for (const auto& g : *guess) ok = ok && (long)g.size() == N && ddot(g, g) > 1e-2;
+    if (ok) {
+      out.b = *guess;
+      out.guessed = true;
+    }
   }
   out.e = diag;
   const int dmax = std::max(P.deflation_max, P.nroots + 2);
@@ -261,6 +273,12 @@ BlockPtr truncate_block(const Integrals& I, const SpinBlock& B, const SpinBlock&
   Rotation Rot = make_rotation_info(*B.si, rot, mkeep);
   BlockPtr nb = renormalise(B, Rot);
   if (!B.has_comp) add_rotated_comp_ops(I, B, B1, dot, scr, Rot, *nb);
+  auto rec = std::make_shared<RotRecord>();
+  rec->composite = B.si;
+  rec->kept = Rot.ns;
+  rec->oldidx = Rot.oldidx;
+  rec->rot = std::move(rot);
+  nb->rotrec = rec;
   return nb;
 }
 
@@ -280,7 +298,8 @@ Quantum partial_target(const Quantum& t, int kp, int k) {
 // their density matrices.  Fills fwd[0..kh-1] and bwd[k-kh..k-1]; returns the
 // last (full-chain, mid-point) superblock.
 SuperBlock infinite_warmup(const Integrals& I, const DmrgParams& P, const SigmaFactory& make_sigma,
-                           std::vector<BlockPtr>& fwd, std::vector<BlockPtr>& bwd, std::mt19937_64& rng) {
+                           std::vector<BlockPtr>& fwd, std::vector<BlockPtr>& bwd, std::mt19937_64& rng,
+                           std::vector<std::vector<double>>* last_b = nullptr) {
   const int k = I.norb;
   const Quantum target{P.nelec, P.twoS, P.irrep};
   Screen scr;
@@ -318,7 +337,10 @@ SuperBlock infinite_warmup(const Integrals& I, const DmrgParams& P, const SigmaF
     fwd[s + 1] = truncate_block(I, *L, *fwd[s], *dot1, rhoL, M, scr, &d1);
     BlockPtr Rn = truncate_block(I, *R, *bwd[k - 1 - s], *dot2, rhoR, M, scr, &d2);
     bwd[k - 2 - s] = Rn;
-    if (kp == k || 2 * (s + 3) > k) last = sb;
+    if (kp == k || 2 * (s + 3) > k) {
+      last = sb;
+      if (last_b) *last_b = sol.b;
+    }
   }
   return last;
 }
@@ -336,7 +358,18 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
 
   std::vector<BlockPtr> fwd(k), bwd(k);  // fwd[l]: sites 0..l ; bwd[l]: sites l..k-1
   double tw = now_s();
-  infinite_warmup(I, P, make_sigma, fwd, bwd, rng);
+  // the previous step: its superblock layout and wavefunctions, its
+  // environment block and the block it made (for the next guess)
+  struct Prev {
+    std::shared_ptr<const WaveLayout> wave;
+    std::vector<std::vector<double>> b;
+    BlockPtr env, nsys;
+  } prev;
+  {
+    SuperBlock last = infinite_warmup(I, P, make_sigma, fwd, bwd, rng, &prev.b);
+    prev.wave = last.wave;
+  }
+  if (!P.transform_guess) prev.b.clear();
   int fwd_max = 0;
   while (fwd_max + 1 < k && fwd[fwd_max + 1]) ++fwd_max;
   const int l0 = fwd_max - 1;  // first forward step (env = bwd[l0 + 3] exists)
@@ -381,7 +414,27 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       const long N = sb.wave->size;
       rec.build_seconds += now_s() - ts;
       ts = now_s();
-      Solved sol = solve_superblock(sb, P, S->davidson_tol, make_sigma, rng);
+      // guess: the previous wavefunction in this basis (same superblock after
+      // the warm-up, the halves swapped at a turn, else the two-site transform)
+      std::vector<std::vector<double>> guess;
+      bool have_guess = false;
+      if (!prev.b.empty() && prev.wave) {
+        const WaveLayout& w0 = *prev.wave;
+        const WaveLayout& w1 = *sb.wave;
+        if (sweep == 0 && step == first) {
+          if (same_sectors(*w0.L, *w1.L) && same_sectors(*w0.R, *w1.R) && w0.target == w1.target) {
+            guess = prev.b;
+            have_guess = true;
+          }
+        } else if (step == first) {
+          have_guess = guess_transpose(w0, prev.b, w1, guess);
+        } else if (prev.nsys && prev.nsys->rotrec && prev.env && prev.env->rotrec) {
+          have_guess = guess_transform(w0, prev.b, *prev.nsys->rotrec, *prev.env->rotrec, w1, guess);
+        }
+      }
+      rec.guess_seconds += now_s() - ts;
+      Solved sol = solve_superblock(sb, P, S->davidson_tol, make_sigma, rng, have_guess ? &guess : nullptr);
+      rec.guessed_steps += sol.guessed;
       rec.davidson_seconds += now_s() - ts - sol.st.sigma_seconds;
       rec.setup_seconds += sol.st.setup_seconds;
       rec.diag_seconds += sol.st.diag_seconds;
@@ -400,9 +453,16 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
         fwd[l + 1] = nsys;
       else
         bwd[l - 1] = nsys;
+      if (P.transform_guess) {
+        prev.wave = sb.wave;
+        prev.b = std::move(sol.b);
+        prev.env = env;
+        prev.nsys = nsys;
+      }
       if (P.verbose > 1)
-        std::fprintf(stderr, "  sweep %d step %d  N=%ld secL=%d secR=%d E0=%.12f  iters=%d  disc=%.2e\n", sweep, step,
-                     N, L->si->nsec(), R->si->nsec(), sol.e[0], sol.st.iterations, disc);
+        std::fprintf(stderr, "  sweep %d step %d  N=%ld secL=%d secR=%d E0=%.12f  iters=%d%s  disc=%.2e\n", sweep,
+                     step, N, L->si->nsec(), R->si->nsec(), sol.e[0], sol.st.iterations,
+                     sol.guessed ? " (guess)" : "", disc);
     }
     // the sweep ends at the far edge: the last block is the single-site dot
     if (forward)
@@ -414,10 +474,10 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
     if (P.verbose)
       std::fprintf(stderr,
                    "sweep %d %s M=%d E=%.12f disc=%.2e t=%.2fs (build %.2f, sigma %.2f in %ld calls, davidson %.2f "
-                   "[setup %.2f, diag %.2f], renorm %.2f)\n",
+                   "[setup %.2f, diag %.2f, guess %.2f for %ld steps], renorm %.2f)\n",
                    sweep, forward ? "fwd" : "bwd", S->M, rec.energies[0], rec.max_discarded, rec.seconds,
                    rec.build_seconds, rec.sigma_seconds, rec.sigma_calls, rec.davidson_seconds, rec.setup_seconds,
-                   rec.diag_seconds, rec.renorm_seconds);
+                   rec.diag_seconds, rec.guess_seconds, rec.guessed_steps, rec.renorm_seconds);
     res.energies = rec.energies;
     // convergence between consecutive full sweeps once the schedule is exhausted
     if (!forward) {
@@ -457,6 +517,7 @@ void write_sweep_records(const DmrgResult& res, double* out) {
     o[10] = s.build_seconds;
     o[11] = s.davidson_seconds;
     o[12] = s.renorm_seconds;
+    o[13] = (double)s.guessed_steps;
   }
 }
 

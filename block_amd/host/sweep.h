@@ -30,6 +30,7 @@ struct DmrgParams {
   int deflation_min = 2, deflation_max = 20;
   int davidson_max_iter = 200;
   double ladder = 0.0;  // > 0: split-precision sigma until the residual norm falls below this (FP64 after)
+  bool transform_guess = true;  // start each step from the previous wavefunction in the new basis
   uint64_t seed = 12345;
   int verbose = 0;
 };
@@ -50,6 +51,8 @@ struct SweepRecord {
   double renorm_seconds = 0.0;    // density matrix, rotation and renormalised operators
   double setup_seconds = 0.0;     // of davidson_seconds: sigma functor construction
   double diag_seconds = 0.0;      // of davidson_seconds: diagonal of H
+  double guess_seconds = 0.0;     // of davidson_seconds: guess wavefunction transforms
+  long guessed_steps = 0;         // steps that started from the transformed wavefunction
 };
 
 
@@ -60,10 +63,10 @@ struct DmrgResult {
 };
 
 // Columns of the per-sweep records the C entry points write.
-constexpr int kSweepCols = 13;
+constexpr int kSweepCols = 14;
 // sweep, forward, M, E0, max discarded weight, seconds, sigma seconds, sigma
 // flops, sigma calls, split-precision sigma calls, build seconds, other
-// Davidson seconds, renormalisation seconds
+// Davidson seconds, renormalisation seconds, steps started from a guess
 void write_sweep_records(const DmrgResult& res, double* out);
 
 using SigmaFactory = std::function<std::unique_ptr<DavidsonFunctor>(const SuperBlock&)>;

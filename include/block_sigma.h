@@ -107,10 +107,10 @@ int bsg_plan_phase_flops(const bsg_plan* plan, double* phase1, double* phase2);
  * bsg_sigma_apply must hold at least this many). */
 int64_t bsg_plan_arena_extent(const bsg_plan* plan);
 
-/* Split precision set-up: the plan builds (and owns) three BF16 slices of
- * every arena element, x = x0 + x1 + x2 with |x - x0 - x1 - x2| <~ 2^-24 |x|,
- * plus a slice buffer for one wavefunction.  Call once after the arena is
- * uploaded and again whenever its contents change; n >= the plan's extent.
+/* Split precision set-up: checks that the arena holds the plan's extent and
+ * enables BSG_SPLIT_BF16 for this arena.  The kernels form the BF16 slices in
+ * registers (x = x0 + x1 + x2 of the FP32 rounding of x, exact), so nothing
+ * is allocated; call again after the arena moves.
  * No counterpart in the reference (its sigma is FP64 only). */
 int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* plan, const double* arena, int64_t n);
 
@@ -124,9 +124,10 @@ int bsg_sigma_apply(bsg_context* ctx, bsg_plan* plan, const double* arena, const
 
 /* Timing helper: runs `reps` sigma applications on the context stream with
  * HIP events around every kernel launch.  wall_ms: whole loop;
- * kernel_ms[0..2]: summed time of the fused GEMM-pair kernels, of the
- * partial-tile reduction and of the wavefunction split (split precision
- * only); launches[0..2]: number of launches of each. */
+ * kernel_ms[0..2]: summed time of the fused GEMM-pair kernels (grouped
+ * kernel on the context stream and small-block kernel beside it, fork to
+ * join), of the partial-tile reduction and of the small-block kernel alone;
+ * launches[0..2]: number of timed intervals of each. */
 int bsg_sigma_profile(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t reps,
                       int32_t precision, double* wall_ms, double* kernel_ms, long* launches);
 

@@ -133,9 +133,9 @@ void* orc_exact_superblock(void* ints, int nelec, int twoS, int irrep, int nL) {
 }
 
 // the sweep driver with the oracle sigma (CPU reference run); same outputs as
-// blkh_dmrg_gpu (sweep_out rows of 10)
+// blkh_dmrg_gpu (sweep_out rows of kSweepCols)
 int orc_dmrg_cpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
-                 const double* sched, int max_sweeps, double sweep_tol, int nthreads, int verbose,
+                 const double* sched, int max_sweeps, double sweep_tol, int guess, int nthreads, int verbose,
                  double* energies_out, double* sweep_out) {
   return guarded([&]() {
     const Integrals& I = *static_cast<Integrals*>(ints);
@@ -150,6 +150,7 @@ int orc_dmrg_cpu(void* ints, int nelec, int twoS, int irrep, int nroots, const d
     P.max_sweeps = max_sweeps;
     P.sweep_tol = sweep_tol;
     P.verbose = verbose;
+    P.transform_guess = guess != 0;
     auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {
       return std::unique_ptr<DavidsonFunctor>(new OracleFunctor(sb, nthreads));
     });
