@@ -60,6 +60,35 @@ def cpu_baseline(sb, seconds_budget):
             "sample": f"{n} full H.C of the same superblock ({sb.flops:.3e} flop each), {total:.1f} s"}
 
 
+def sweep_leg(ints, cfg, M):
+    """One full two-site sweep (forward and backward half-sweeps) at M after a
+    ramp from the warm-up (M/2 for one full sweep), in FP64 and with the
+    precision ladder (split-precision sigmas until the Davidson residual norm
+    is below 1e-5), both from the same seed; every step starts from the
+    previous step's transformed wavefunction."""
+    sched = [(0, M // 2, 1e-7, 1e-6), (2, M, 1e-9, 0.0)]
+    cols = block_amd.SWEEP_COLS
+    out = {"schedule": [list(r) for r in sched], "timed_sweeps": [2, 3]}
+    for name, ladder in (("fp64", 0.0), ("mixed", 1e-5)):
+        t0 = time.time()
+        e, sw = block_amd.dmrg(ints, nelec=cfg["nelec"], schedule=sched, max_sweeps=4, sweep_tol=0.0,
+                               ladder=ladder)
+        wall = time.time() - t0
+        rows = sw[2:4]
+        col = lambda c: float(rows[:, cols.index(c)].sum())
+        out[name] = {"sweep_seconds": round(col("seconds"), 3), "sigma_seconds": round(col("sigma_seconds"), 3),
+                     "sigma_calls": int(col("sigma_calls")), "split_calls": int(col("split_calls")),
+                     "build_seconds": round(col("build_seconds"), 3),
+                     "davidson_other_seconds": round(col("davidson_seconds"), 3),
+                     "renorm_seconds": round(col("renorm_seconds"), 3),
+                     "sigma_tflops": round(col("sigma_flops") / max(col("sigma_seconds"), 1e-9) / 1e12, 3),
+                     "energy": float(e[0]), "run_seconds": round(wall, 2)}
+    out["sweep_speedup_mixed_vs_fp64"] = round(out["fp64"]["sweep_seconds"] / out["mixed"]["sweep_seconds"], 3)
+    out["sigma_speedup_mixed_vs_fp64"] = round(out["fp64"]["sigma_seconds"] / out["mixed"]["sigma_seconds"], 3)
+    out["abs_dE_Eh"] = abs(out["fp64"]["energy"] - out["mixed"]["energy"])
+    return out
+
+
 def traffic_from_profile(config, M, wave_size, pairs):
     """HBM bytes per H.C of the fused kernels, from the committed counter
     summary of this workload (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
@@ -195,6 +224,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision timing")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the sweep-time leg")
     ap.add_argument("--wilson", action="store_true", help="Wilson-warmed blocks instead of the DMRG warm-up")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run a device copy of N doubles (HBM counter calibration)")
@@ -296,6 +326,8 @@ def main():
     }
     if mixed:
         out["mixed_split_bf16"] = mixed
+    if not args.no_sweep:
+        out["sweep"] = sweep_leg(ints, cfg, M)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sb, args.cpu_seconds)
     print(json.dumps(out))

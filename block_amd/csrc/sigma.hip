@@ -893,23 +893,32 @@ __global__ __launch_bounds__(256) void k_lincomb(const double* __restrict__ X, i
   }
 }
 
+// diag(H) by V block: item i covers elements [e0, e0 + 256) of the block of
+// its term group; every thread sums the group's terms in list order for its
+// element (alpha_t A_ii B_jj, identity partners read as 1).
 struct DiagGroup {
   int32_t first, count;
 };
+struct DiagItem {
+  int32_t group, e0;
+};
 __global__ __launch_bounds__(256) void k_diag(const bsg_diag_t* __restrict__ terms, const DiagGroup* __restrict__ groups,
-                                              const double* __restrict__ arena, double* __restrict__ diag) {
-  const DiagGroup g = groups[blockIdx.x];
+                                              const DiagItem* __restrict__ items, const double* __restrict__ arena,
+                                              double* __restrict__ diag) {
+  const DiagItem it = items[blockIdx.x];
+  const DiagGroup g = groups[it.group];
+  const bsg_diag_t t0 = terms[g.first];
+  const int64_t e = (int64_t)it.e0 + threadIdx.x;
+  if (e >= (int64_t)t0.m * t0.n) return;
+  const int i = (int)(e / t0.n), j = (int)(e % t0.n);
+  double s = 0.0;
   for (int q = 0; q < g.count; ++q) {
     const bsg_diag_t t = terms[g.first + q];
-    const int64_t tot = (int64_t)t.m * t.n;
-    for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
-      int i = (int)(e / t.n), j = (int)(e % t.n);
-      double a = t.a_off < 0 ? 1.0 : arena[t.a_off + (int64_t)i * (t.m + 1)];
-      double b = t.b_off < 0 ? 1.0 : arena[t.b_off + (int64_t)j * (t.n + 1)];
-      diag[t.v_off + e] += t.alpha * a * b;
-    }
-    __syncthreads();
+    const double a = t.a_off < 0 ? 1.0 : arena[t.a_off + (int64_t)i * (t.m + 1)];
+    const double b = t.b_off < 0 ? 1.0 : arena[t.b_off + (int64_t)j * (t.n + 1)];
+    s += t.alpha * a * b;
   }
+  diag[t0.v_off + e] += s;
 }
 
 int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256)); }
@@ -1034,6 +1043,24 @@ int bsg_memcpy_h2d(bsg_context* ctx, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return BSG_OK;
   HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BSG_OK;
+}
+int bsg_host_alloc(bsg_context* ctx, size_t bytes, void** hptr) {
+  if (!ctx || !hptr) return fail(BSG_ERR_ARG, "bsg_host_alloc: null argument");
+  hipError_t e = hipHostMalloc(hptr, std::max<size_t>(bytes, 8), hipHostMallocDefault);
+  if (e != hipSuccess) return fail(BSG_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  return BSG_OK;
+}
+int bsg_host_free(bsg_context* ctx, void* hptr) {
+  if (!ctx) return fail(BSG_ERR_ARG, "null ctx");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipHostFree(hptr));
+  return BSG_OK;
+}
+int bsg_memcpy_h2d_async(bsg_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx) return fail(BSG_ERR_ARG, "null ctx");
+  if (bytes == 0) return BSG_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
   return BSG_OK;
 }
 int bsg_memcpy_d2h(bsg_context* ctx, void* dst, const void* src, size_t bytes) {
@@ -1387,19 +1414,31 @@ int bsg_diag_apply(bsg_context* ctx, const bsg_diag_t* terms, int32_t n, const d
   for (int i = 0; i < n; ++i) {
     sorted[i] = terms[idx[i]];
     if (i == 0 || sorted[i].v_off != sorted[i - 1].v_off) groups.push_back({i, 0});
+    else if (sorted[i].m != sorted[i - 1].m || sorted[i].n != sorted[i - 1].n)
+      return fail(BSG_ERR_ARG, "bsg_diag_apply: terms of one block disagree on its shape");
     groups.back().count++;
   }
+  std::vector<DiagItem> items;
+  for (size_t g = 0; g < groups.size(); ++g) {
+    const int64_t mn = (int64_t)sorted[groups[g].first].m * sorted[groups[g].first].n;
+    for (int64_t e0 = 0; e0 < mn; e0 += 256) items.push_back({(int32_t)g, (int32_t)e0});
+  }
+  if (items.empty()) return BSG_OK;
   bsg_diag_t* dt = nullptr;
   DiagGroup* dg = nullptr;
+  DiagItem* di = nullptr;
   HIPCHK(hipMalloc(&dt, sizeof(bsg_diag_t) * n));
   HIPCHK(hipMalloc(&dg, sizeof(DiagGroup) * groups.size()));
+  HIPCHK(hipMalloc(&di, sizeof(DiagItem) * items.size()));
   HIPCHK(hipMemcpyAsync(dt, sorted.data(), sizeof(bsg_diag_t) * n, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(dg, groups.data(), sizeof(DiagGroup) * groups.size(), hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_diag, dim3(groups.size()), dim3(256), 0, ctx->stream, dt, dg, arena, diag);
+  HIPCHK(hipMemcpyAsync(di, items.data(), sizeof(DiagItem) * items.size(), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_diag, dim3(items.size()), dim3(256), 0, ctx->stream, dt, dg, di, arena, diag);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipFree(dt));
   HIPCHK(hipFree(dg));
+  HIPCHK(hipFree(di));
   return BSG_OK;
 }
 

@@ -1,6 +1,7 @@
 #include "gpu_sigma.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 namespace blk {
@@ -11,8 +12,17 @@ void bsg_check(int rc, const char* what) {
 
 GpuContext::GpuContext(int device) { bsg_check(bsg_init(&ctx, device), "bsg_init"); }
 GpuContext::~GpuContext() {
-  if (ctx) bsg_free(ctx);
+  if (!ctx) return;
+  if (arena) bsg_mfree(ctx, arena);
+  if (staging) bsg_host_free(ctx, staging);
+  bsg_free(ctx);
 }
+
+namespace {
+SetupTimes g_setup;
+double secs() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+}  // namespace
+const SetupTimes& gpu_setup_times() { return g_setup; }
 
 WorkList compile_worklist(const SuperBlock& sb) {
   WorkList wl;
@@ -113,16 +123,63 @@ WorkList compile_worklist(const SuperBlock& sb) {
 }
 
 GpuSigma::GpuSigma(GpuContext& g, const SuperBlock& sb, int max_vec) : g_(g) {
+  double t0 = secs();
   wave_size_ = sb.wave->size;
   WorkList wl = compile_worklist(sb);
   n_pairs_ = wl.pairs.size();
   arena_size_ = wl.arena_size;
   diag_terms_ = std::move(wl.diag);
-  bsg_check(bsg_malloc(g_.ctx, sizeof(double) * std::max<int64_t>(1, arena_size_), (void**)&d_arena_), "arena");
-  // upload operators
+  double t1 = secs();
+  const size_t need = (size_t)std::max<int64_t>(1, arena_size_);
+  if (!g_.arena_leased) {
+    if (g_.arena_cap < need) {
+      if (g_.arena) bsg_check(bsg_mfree(g_.ctx, g_.arena), "arena free");
+      g_.arena = nullptr;
+      g_.arena_cap = 0;
+      const size_t cap = need + need / 4;  // headroom for the next, larger superblocks
+      bsg_check(bsg_malloc(g_.ctx, sizeof(double) * cap, (void**)&g_.arena), "arena");
+      g_.arena_cap = cap;
+    }
+    d_arena_ = g_.arena;
+    g_.arena_leased = true;
+  } else {
+    bsg_check(bsg_malloc(g_.ctx, sizeof(double) * need, (void**)&d_arena_), "arena");
+    owns_arena_ = true;
+  }
+  // upload: operators copied into page-locked staging by all threads in
+  // batches of about 64 MB, each batch sent while the next one is copied
+  if (g_.staging_cap < need) {
+    if (g_.staging) bsg_check(bsg_host_free(g_.ctx, g_.staging), "staging free");
+    g_.staging = nullptr;
+    g_.staging_cap = 0;
+    const size_t cap = need + need / 4;
+    bsg_check(bsg_host_alloc(g_.ctx, sizeof(double) * cap, (void**)&g_.staging), "staging");
+    g_.staging_cap = cap;
+  }
+  struct Piece {
+    const double* src;
+    int64_t off, n;
+  };
+  std::vector<Piece> pieces;
+  constexpr int64_t PIECE = 1 << 19;  // doubles (4 MB)
   for (const auto& kv : wl.arena_off)
-    bsg_check(bsg_memcpy_h2d(g_.ctx, d_arena_ + kv.second, kv.first->data.data(), sizeof(double) * kv.first->data.size()),
-              "arena upload");
+    for (int64_t x = 0; x < (int64_t)kv.first->data.size(); x += PIECE)
+      pieces.push_back({kv.first->data.data() + x, kv.second + x, std::min<int64_t>(PIECE, kv.first->data.size() - x)});
+  std::sort(pieces.begin(), pieces.end(), [](const Piece& a, const Piece& b) { return a.off < b.off; });
+  constexpr int64_t BATCH = 1 << 23;  // doubles (64 MB)
+  for (size_t p0 = 0; p0 < pieces.size();) {
+    size_t p1 = p0;
+    int64_t len = 0;
+    while (p1 < pieces.size() && (len == 0 || len + pieces[p1].n <= BATCH)) len += pieces[p1++].n;
+#pragma omp parallel for schedule(dynamic)
+    for (long i = (long)p0; i < (long)p1; ++i)
+      std::memcpy(g_.staging + pieces[i].off, pieces[i].src, sizeof(double) * pieces[i].n);
+    const int64_t b0 = pieces[p0].off, b1 = pieces[p1 - 1].off + pieces[p1 - 1].n;
+    bsg_check(bsg_memcpy_h2d_async(g_.ctx, d_arena_ + b0, g_.staging + b0, sizeof(double) * (b1 - b0)), "arena upload");
+    p0 = p1;
+  }
+  bsg_check(bsg_synchronize(g_.ctx), "arena upload");
+  double t2 = secs();
   size_t wbytes = sizeof(double) * std::max<long>(1, wave_size_) * std::max(1, max_vec);
   bsg_check(bsg_malloc(g_.ctx, wbytes, (void**)&d_c_), "c buffer");
   bsg_check(bsg_malloc(g_.ctx, wbytes, (void**)&d_v_), "v buffer");
@@ -133,11 +190,20 @@ GpuSigma::GpuSigma(GpuContext& g, const SuperBlock& sb, int max_vec) : g_(g) {
   chunk.n_pairs = (int32_t)wl.pairs.size();
   bsg_check(bsg_plan_create(g_.ctx, &chunk, 1, wave_size_, &plan_), "plan");
   flops_ = bsg_plan_flops(plan_);
+  double t3 = secs();
+  g_setup.compile += t1 - t0;
+  g_setup.upload += t2 - t1;
+  g_setup.plan += t3 - t2;
 }
 
 GpuSigma::~GpuSigma() {
   if (plan_) bsg_plan_destroy(plan_);
-  if (d_arena_) bsg_mfree(g_.ctx, d_arena_);
+  if (d_arena_) {
+    if (owns_arena_)
+      bsg_mfree(g_.ctx, d_arena_);
+    else
+      g_.arena_leased = false;
+  }
   if (d_c_) bsg_mfree(g_.ctx, d_c_);
   if (d_v_) bsg_mfree(g_.ctx, d_v_);
 }

@@ -16,8 +16,16 @@ namespace blk {
 // fail loudly: every C-ABI status is checked
 void bsg_check(int rc, const char* what);
 
+// Device context plus buffers reused by successive superblocks of a sweep:
+// the operator arena (leased by one GpuSigma at a time) and the page-locked
+// staging it is uploaded through, both grown on demand.
 struct GpuContext {
   bsg_context* ctx = nullptr;
+  double* arena = nullptr;
+  size_t arena_cap = 0;  // doubles
+  bool arena_leased = false;
+  double* staging = nullptr;
+  size_t staging_cap = 0;  // doubles
   explicit GpuContext(int device = 0);
   ~GpuContext();
   GpuContext(const GpuContext&) = delete;
@@ -33,6 +41,12 @@ struct WorkList {
 };
 // Flatten the term list into GEMM pairs grouped by the V block they update.
 WorkList compile_worklist(const SuperBlock& sb);
+
+// Seconds spent constructing GpuSigma objects, by phase (accumulated).
+struct SetupTimes {
+  double compile = 0, upload = 0, plan = 0;
+};
+const SetupTimes& gpu_setup_times();
 
 class GpuSigma {
  public:
@@ -67,6 +81,7 @@ class GpuSigma {
   size_t n_pairs_ = 0;
   int64_t arena_size_ = 0;
   double* d_arena_ = nullptr;
+  bool owns_arena_ = false;
   double* d_c_ = nullptr;
   double* d_v_ = nullptr;
   bsg_plan* plan_ = nullptr;

@@ -1,4 +1,5 @@
 #include "sweep.h"
+#include "gpu_sigma.h"
 #include "guess.h"
 
 #include <algorithm>
@@ -112,11 +113,19 @@ double make_rotation(const StateInfo& si, const std::vector<std::vector<double>>
   // weights sorted descending; ties in (sector, index) order (sort_weights)
   std::multimap<double, std::pair<int, int>> wmap;
   double total = 0.0;
-  for (int q = 0; q < ns; ++q) {
-    int d = si.dim[q];
+  // sectors are independent: largest first over the threads
+  std::vector<int> order(ns);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return si.dim[a] > si.dim[b]; });
+#pragma omp parallel for schedule(dynamic)
+  for (int i = 0; i < ns; ++i) {
+    const int q = order[i], d = si.dim[q];
     evec[q] = rho[q];
     eval[q].assign(d, 0.0);
     syev(d, evec[q].data(), eval[q].data());
+  }
+  for (int q = 0; q < ns; ++q) {
+    int d = si.dim[q];
     for (int k = 0; k < d; ++k) {
       if (eval[q][k] < 1e-14) eval[q][k] = 0.0;
       wmap.insert({eval[q][k], {q, k}});
@@ -265,14 +274,27 @@ std::vector<std::vector<double>> density_matrix(const SuperBlock& sb, const Solv
   return rho;
 }
 
+// seconds spent in the phases of truncate_block (eigen-decomposition,
+// rotation of the stored operators, complementary operators built rotated)
+struct RenormTimes {
+  double eig = 0, rotate = 0, comp = 0;
+};
+RenormTimes g_rt;
+
 BlockPtr truncate_block(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
                         const std::vector<std::vector<double>>& rho, int M, const Screen& scr, double* disc) {
   std::vector<std::vector<double>> rot;
   std::vector<int> mkeep;
+  double t0 = now_s();
   *disc = make_rotation(*B.si, rho, M, rot, mkeep);
   Rotation Rot = make_rotation_info(*B.si, rot, mkeep);
+  double t1 = now_s();
   BlockPtr nb = renormalise(B, Rot);
+  double t2 = now_s();
   if (!B.has_comp) add_rotated_comp_ops(I, B, B1, dot, scr, Rot, *nb);
+  g_rt.eig += t1 - t0;
+  g_rt.rotate += t2 - t1;
+  g_rt.comp += now_s() - t2;
   auto rec = std::make_shared<RotRecord>();
   rec->composite = B.si;
   rec->kept = Rot.ns;
@@ -444,7 +466,10 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       rec.sigma_calls_split += sol.st.sigma_calls_split;
       rec.sigma_flops += sol.st.sigma_calls * sigma_flops(sb);
       for (int r = 0; r < P.nroots; ++r) rec.energies[r] = std::min(rec.energies[r], sol.e[r]);
+      const RenormTimes rt0 = g_rt;
+      double trho = now_s();
       auto rho = density_matrix(sb, sol, weights, true, S->noise, rng);
+      trho = now_s() - trho;
       double disc = 0.0;
       BlockPtr nsys = truncate_block(I, *L, *sys, *dot1, rho, S->M, scr, &disc);
       rec.max_discarded = std::max(rec.max_discarded, disc);
@@ -460,9 +485,12 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
         prev.nsys = nsys;
       }
       if (P.verbose > 1)
-        std::fprintf(stderr, "  sweep %d step %d  N=%ld secL=%d secR=%d E0=%.12f  iters=%d%s  disc=%.2e\n", sweep,
-                     step, N, L->si->nsec(), R->si->nsec(), sol.e[0], sol.st.iterations,
-                     sol.guessed ? " (guess)" : "", disc);
+        std::fprintf(stderr,
+                     "  sweep %d step %d  N=%ld secL=%d secR=%d E0=%.12f  iters=%d%s  disc=%.2e  renorm: rho %.3f "
+                     "eig %.3f rotate %.3f comp %.3f\n",
+                     sweep, step, N, L->si->nsec(), R->si->nsec(), sol.e[0], sol.st.iterations,
+                     sol.guessed ? " (guess)" : "", disc, trho, g_rt.eig - rt0.eig, g_rt.rotate - rt0.rotate,
+                     g_rt.comp - rt0.comp);
     }
     // the sweep ends at the far edge: the last block is the single-site dot
     if (forward)
@@ -471,6 +499,11 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       fwd[0] = make_dot(I, 0, scr);
     rec.seconds = now_s() - t0;
     res.sweeps.push_back(rec);
+    if (P.verbose > 1) {
+      const SetupTimes& st = gpu_setup_times();
+      std::fprintf(stderr, "  sigma set-up so far: compile %.2f upload %.2f plan %.2f; renorm so far: eig %.2f rotate %.2f comp %.2f\n",
+                   st.compile, st.upload, st.plan, g_rt.eig, g_rt.rotate, g_rt.comp);
+    }
     if (P.verbose)
       std::fprintf(stderr,
                    "sweep %d %s M=%d E=%.12f disc=%.2e t=%.2fs (build %.2f, sigma %.2f in %ld calls, davidson %.2f "

@@ -91,6 +91,13 @@ int bsg_mfree(bsg_context* ctx, void* dptr);
 int bsg_memcpy_h2d(bsg_context* ctx, void* dst, const void* src, size_t bytes);
 int bsg_memcpy_d2h(bsg_context* ctx, void* dst, const void* src, size_t bytes);
 int bsg_memset_zero(bsg_context* ctx, void* dst, size_t bytes);
+/* Page-locked host staging for operator uploads (no counterpart in the
+ * reference, whose operators never leave host memory), and an upload that
+ * returns before the copy ends: src must be page-locked and unchanged until
+ * bsg_synchronize. */
+int bsg_host_alloc(bsg_context* ctx, size_t bytes, void** hptr);
+int bsg_host_free(bsg_context* ctx, void* hptr);
+int bsg_memcpy_h2d_async(bsg_context* ctx, void* dst, const void* src, size_t bytes);
 
 /* Build a device plan from host work-list chunks (copied; the host arrays may
  * be freed afterwards).  The plan owns the partial-tile scratch of the
