@@ -67,7 +67,7 @@ _sig(_h, "blkh_gpu_sigma_timed", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c
 _sig(_h, "blkh_gpu_sigma_set_precision", ctypes.c_int, [_vp, ctypes.c_int])
 _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
      [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
-      ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp])
+      ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp])
 
 FP64, SPLIT_BF16 = 0, 1
 SWEEP_COLS = ("sweep", "forward", "M", "E0", "discarded", "seconds", "sigma_seconds", "sigma_flops",
@@ -223,11 +223,12 @@ def _schedule_array(schedule):
 
 
 def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=None, max_sweeps=20,
-         sweep_tol=1e-8, device=0, verbose=0, ladder=0.0, guess=True):
+         sweep_tol=1e-8, device=0, verbose=0, ladder=0.0, guess=True, ladder_from=0):
     """Two-site spin-adapted DMRG (Sweep::do_one, sweep.C:341) with the Davidson
     sigma on the GPU.  schedule: rows (start_sweep, M, davidson_tol, noise).
     ladder > 0: split-precision sigma until the Davidson residual norm falls
-    below it, FP64 after (convergence is always decided on FP64 sigmas).
+    below it, FP64 after (convergence is always decided on FP64 sigmas), from
+    sweep ladder_from on (0: also in the warm-up).
     guess: start every step from the previous step's wavefunction carried into
     the new basis (guess_wavefunction.C:524-636), else from the diagonal.
     Returns (energies, sweeps) with sweeps rows named by SWEEP_COLS."""
@@ -240,7 +241,8 @@ def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=
     nc = len(SWEEP_COLS)
     sw = np.zeros(nc * max_sweeps)
     n = host_lib.blkh_dmrg_gpu(ints._h, nelec, twoS, irrep, nroots, _ptr(w), len(schedule), _ptr(sched), max_sweeps,
-                               sweep_tol, float(ladder), int(guess), device, verbose, _ptr(e), _ptr(sw))
+                               sweep_tol, float(ladder), int(ladder_from), int(guess), device, verbose, _ptr(e),
+                               _ptr(sw))
     if n < 0:
         raise _err()
     return e, sw[: nc * n].reshape(n, nc)

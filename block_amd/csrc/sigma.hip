@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -483,6 +485,333 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
+// ---------------------------------------------------------------- split-precision grouped kernel
+// The grouped kernel's tiles and work units with every operand already in
+// three BF16 slices (x = h + m + l, the exact FP32 rounding of x):
+//   op(A) blocks, row-major with K contiguous   (built once per arena)
+//   B' blocks, [V column][K] with K contiguous  (built once per arena)
+//   C blocks transposed, [k2][k1]               (built per application)
+// each slice padded to a multiple of 8 along K and zero there.  A lane's 8
+// K slots are then one 16-byte load per slice (K = 8 lk .. 8 lk + 7 for lane
+// (li, lk), the same bijection for both MFMA operands).  GEMM 1 runs in the
+// transposed orientation W^T = C^T op(A)^T so that a lane's result registers
+// hold 4 consecutive W columns of one W row: they are scaled, split and
+// written to LDS as one 8-byte store per slice, the layout GEMM 2 reads as
+// its A operand.  Identity-B' pairs keep the direct orientation, whose result
+// layout is the V accumulators'.  Each 32-deep K group is 6 slice products
+// (i + j <= 2, smallest first) in FP32, added into FP64 accumulators.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+struct PairS {
+  int64_t a, b, c;  // BF16 element offsets of the op(A), B' and C^T slice 0 (-1: identity)
+};
+struct SliceItem {
+  // element (r, k) of the sliced matrix is src[trans ? k rows + r : r cols + k]
+  // for k < cols, zero for cols <= k < pitch; slice s at dst + s rows pitch
+  int64_t src, dst;
+  int32_t rows, cols, pitch, trans;
+};
+struct Frag {
+  bf16x8 h, m, l;
+};
+__device__ inline bf16x8 bload8(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ inline __amdgpu_buffer_rsrc_t slice_rsrc(const __bf16* p, int64_t elems) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(elems * 2), 0x00020000);
+}
+__device__ inline void mma6(dbl4& acc, const Frag& a, const Frag& b) {
+  f32x4 x = {0.f, 0.f, 0.f, 0.f};
+  x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.l, b.h, x, 0, 0, 0);
+  x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.m, b.m, x, 0, 0, 0);
+  x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.l, x, 0, 0, 0);
+  x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.m, b.h, x, 0, 0, 0);
+  x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.m, x, 0, 0, 0);
+  x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.h, x, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) acc[i] += (double)x[i];
+}
+
+__global__ __launch_bounds__(256) void k_slices(const SliceItem* __restrict__ items, const double* __restrict__ src,
+                                                __bf16* __restrict__ dst) {
+  const SliceItem it = items[blockIdx.x];
+  const int64_t tot = (int64_t)it.rows * it.pitch;
+  for (int64_t e = threadIdx.x; e < tot; e += 256) {
+    const int r = (int)(e / it.pitch), k = (int)(e % it.pitch);
+    double x = 0.0;
+    if (k < it.cols) x = src[it.src + (it.trans ? (int64_t)k * it.rows + r : (int64_t)r * it.cols + k)];
+    __bf16 h, m, l;
+    split_f32(x, h, m, l);
+    dst[it.dst + e] = h;
+    dst[it.dst + tot + e] = m;
+    dst[it.dst + 2 * tot + e] = l;
+  }
+}
+
+constexpr int SAP = 40;                 // LDS pitch (BF16) of a staged op(A) row: 32 K + 8
+constexpr int SAB = 3 * GTM * SAP;      // one staged op(A) chunk, 3 slices
+constexpr int SWMAX = 3 * 64 * 72;      // W chunk slices: max over geometries of 3 TM (JC + 8)
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sigma_bf(
+    const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const PairS* __restrict__ pss,
+    const Seg* __restrict__ segs, const TileRef* __restrict__ tiles, int ntiles, const __bf16* __restrict__ slices,
+    const __bf16* __restrict__ cslices, const double* __restrict__ c, double* __restrict__ v,
+    double* __restrict__ part) {
+  // two op(A) chunk buffers for GEMM 1, the W chunk for GEMM 2 (aliased,
+  // barriers separate the phases)
+  constexpr int LDSE = 2 * SAB > SWMAX ? 2 * SAB : SWMAX;
+  __shared__ __attribute__((aligned(16))) __bf16 lds[LDSE];
+  const int tid = blockIdx.x;
+  if (tid >= ntiles) return;
+  const TileRef tr = tiles[tid];
+  if (tr.item < 0) return;
+  const Seg sg = segs[tr.item];
+  const bsg_vblock_t vb = vbs[sg.vb];
+  const int lwr = tr.pad & 3, nr = ((tr.pad >> 2) & 1) + 1;
+  const int WR = 1 << lwr, WC = 4 >> lwr;
+  const int TM = 16 * nr * WR, JC = 16 * PB * WC, WP = JC + 8;
+  const int row0 = tr.tm * TM, colg = tr.tn * JC;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wr = w / WC, wc = w % WC;
+  const int arow = wr * nr * 16;
+  const int li = lane & 15, lk = lane >> 4;
+  int nrw = (vb.m - row0 - arow + 15) / 16;
+  nrw = nrw < 0 ? 0 : (nrw > nr ? nr : nrw);
+  int nvb = 0;
+  for (int t = 0; t < PB; ++t)
+    if (colg + (wc + WC * t) * 16 < vb.n) nvb = t + 1;
+  if (nrw == 0) nvb = 0;
+  dbl4 accV[2][PB];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int t = 0; t < PB; ++t) accV[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  // op(A) staging: piece u (8 BF16) of a chunk is slice u / (4 TM), row
+  // (u / 4) % TM, K quarter u % 4; three pieces per thread at most
+  constexpr int SPT = 3 * GTM * 4 / 256;
+  int abuf = 0;
+  for (int q = 0; q < sg.count; ++q) {
+    const bsg_pair_t pr = prs[sg.first + q];
+    const PairS ps = pss[sg.first + q];
+    const int k1p = (pr.k1 + 7) & ~7, k2p = (pr.k2 + 7) & ~7;
+    for (int j0 = 0; j0 < pr.k2; j0 += JC) {
+      if (pr.b_off < 0 && j0 != colg) continue;
+      const int jw = min(JC, pr.k2 - j0);
+      int nwb = 0;
+      for (int t = 0; t < PB; ++t)
+        if ((wc + WC * t) * 16 < jw) nwb = t + 1;
+      if (nrw == 0) nwb = 0;
+      if (pr.a_off < 0) {
+        const __amdgpu_buffer_rsrc_t rC = block_rsrc(c + pr.c_off, (int64_t)pr.k1 * pr.k2);
+        if (pr.b_off < 0) {  // V += alpha C
+#pragma unroll
+          for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int t = 0; t < PB; ++t) {
+              const int col = colg + (wc + WC * t) * 16 + li;
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                const int row = row0 + arow + r * 16 + 4 * lk + i;
+                const bool ok = r < nr && row < vb.m && col < vb.n;
+                accV[r][t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
+              }
+            }
+          continue;
+        }
+        // W = alpha C (panel rows), split into the W slices
+        for (int e = threadIdx.x; e < TM * JC; e += 256) {
+          const int j = e % JC, i = e / JC;
+          const int gi = row0 + i;
+          const bool ok = gi < vb.m && j < jw;
+          __bf16 h, m, l;
+          split_f32(pr.alpha * bload(rC, ok ? (uint32_t)(gi * pr.k2 + j0 + j) * 8u : OOB), h, m, l);
+          lds[i * WP + j] = h;
+          lds[TM * WP + i * WP + j] = m;
+          lds[2 * TM * WP + i * WP + j] = l;
+        }
+      } else {
+        const __amdgpu_buffer_rsrc_t rA = slice_rsrc(slices + ps.a, 3 * (int64_t)vb.m * k1p);
+        const __amdgpu_buffer_rsrc_t rC = slice_rsrc(cslices + ps.c, 3 * (int64_t)pr.k2 * k1p);
+        const uint32_t aslice = (uint32_t)vb.m * k1p * 2u, cslice = (uint32_t)pr.k2 * k1p * 2u;  // bytes per slice
+        dbl4 accW[2][PB];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int t = 0; t < PB; ++t) accW[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+        uint32_t aoff[SPT];
+        int alds[SPT], aq[SPT];
+#pragma unroll
+        for (int x = 0; x < SPT; ++x) {
+          const int u = threadIdx.x + 256 * x;
+          const int s = u / (4 * TM), i = (u >> 2) % TM, kq = u & 3;
+          const bool ok = u < 12 * TM && row0 + i < vb.m;
+          aoff[x] = ok ? (uint32_t)s * aslice + (uint32_t)((row0 + i) * k1p + 8 * kq) * 2u : OOB;
+          alds[x] = u < 12 * TM ? (s * GTM + i) * SAP + 8 * kq : -1;
+          aq[x] = 8 * kq;
+        }
+        uint32_t coff[PB];
+#pragma unroll
+        for (int t = 0; t < PB; ++t) {
+          const int j = j0 + (wc + WC * t) * 16 + li;
+          coff[t] = (t < nwb && j < pr.k2) ? (uint32_t)(j * k1p + 8 * lk) * 2u : OOB;
+        }
+        i32x4 an[SPT];
+        Frag cn[PB];
+        auto load_a = [&](int k0) {
+#pragma unroll
+          for (int x = 0; x < SPT; ++x)
+            an[x] = __builtin_amdgcn_raw_buffer_load_b128(rA, k0 + aq[x] < k1p ? aoff[x] : OOB, (uint32_t)k0 * 2u, 0);
+        };
+        auto load_c = [&](int k0) {
+          const bool in = k0 + 8 * lk < k1p;
+#pragma unroll
+          for (int t = 0; t < PB; ++t) {
+            const uint32_t o = in ? coff[t] : OOB;
+            cn[t].h = bload8(rC, o, (uint32_t)k0 * 2u);
+            cn[t].m = bload8(rC, o, cslice + (uint32_t)k0 * 2u);
+            cn[t].l = bload8(rC, o, 2u * cslice + (uint32_t)k0 * 2u);
+          }
+        };
+        load_a(0);
+        load_c(0);
+        for (int k0 = 0; k0 < pr.k1; k0 += PKC) {
+          __bf16* Ab = lds + abuf * SAB;
+          abuf ^= 1;
+#pragma unroll
+          for (int x = 0; x < SPT; ++x)
+            if (alds[x] >= 0) *(i32x4*)(Ab + alds[x]) = an[x];
+          Frag cc[PB];
+#pragma unroll
+          for (int t = 0; t < PB; ++t) cc[t] = cn[t];
+          __syncthreads();
+          if (k0 + PKC < pr.k1) {
+            load_a(k0 + PKC);
+            load_c(k0 + PKC);
+          }
+          Frag af[2];
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            if (r >= nr) {
+              af[r] = af[0];
+              continue;
+            }
+            const int o = (arow + r * 16 + li) * SAP + 8 * lk;
+            af[r].h = *(const bf16x8*)(Ab + o);
+            af[r].m = *(const bf16x8*)(Ab + GTM * SAP + o);
+            af[r].l = *(const bf16x8*)(Ab + 2 * GTM * SAP + o);
+          }
+#pragma unroll
+          for (int t = 0; t < PB; ++t)
+            if (t < nwb)
+#pragma unroll
+              for (int r = 0; r < 2; ++r)
+                if (r < nrw) {
+                  if (pr.b_off < 0)
+                    mma6(accW[r][t], af[r], cc[t]);  // W block: (row 4 lk + i, col li)
+                  else
+                    mma6(accW[r][t], cc[t], af[r]);  // W^T block: (W col 4 lk + i, W row li)
+                }
+        }
+        __syncthreads();  // op(A) buffers done before the W slices (same LDS) are written
+        const double aw = pr.alpha * pr.alpha_a;
+        if (pr.b_off < 0) {
+#pragma unroll
+          for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int t = 0; t < PB; ++t)
+#pragma unroll
+              for (int i = 0; i < 4; ++i) accV[r][t][i] += aw * accW[r][t][i];
+          continue;
+        }
+        // lane holds W(arow + 16 r + li, 16 cb + 4 lk + i), i < 4
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+          if (r < nr)
+#pragma unroll
+            for (int t = 0; t < PB; ++t) {
+              const int cb = wc + WC * t;
+              __bf16 hs[4], ms[4], ls[4];
+#pragma unroll
+              for (int i = 0; i < 4; ++i) split_f32(aw * accW[r][t][i], hs[i], ms[i], ls[i]);
+              const int o = (arow + r * 16 + li) * WP + cb * 16 + 4 * lk;
+              typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+              *(bf16x4*)(lds + o) = bf16x4{hs[0], hs[1], hs[2], hs[3]};
+              *(bf16x4*)(lds + TM * WP + o) = bf16x4{ms[0], ms[1], ms[2], ms[3]};
+              *(bf16x4*)(lds + 2 * TM * WP + o) = bf16x4{ls[0], ls[1], ls[2], ls[3]};
+            }
+      }
+      __syncthreads();
+      // GEMM 2: V[tile] += W chunk B'[j0 + .., tile columns], 32 deep per group
+      {
+        const __amdgpu_buffer_rsrc_t rB = slice_rsrc(slices + ps.b, 3 * (int64_t)vb.n * k2p);
+        const uint32_t bslice = (uint32_t)vb.n * k2p * 2u;
+        uint32_t boff[PB];
+#pragma unroll
+        for (int t = 0; t < PB; ++t) {
+          const int col = colg + (wc + WC * t) * 16 + li;
+          boff[t] = (t < nvb && col < vb.n) ? (uint32_t)(col * k2p + 8 * lk) * 2u : OOB;
+        }
+        Frag bn[PB];
+        auto load_b = [&](int kg) {
+          const bool in = j0 + kg + 8 * lk < k2p;
+          const uint32_t so = (uint32_t)(j0 + kg) * 2u;
+#pragma unroll
+          for (int t = 0; t < PB; ++t) {
+            const uint32_t o = in ? boff[t] : OOB;
+            bn[t].h = bload8(rB, o, so);
+            bn[t].m = bload8(rB, o, bslice + so);
+            bn[t].l = bload8(rB, o, 2u * bslice + so);
+          }
+        };
+        load_b(0);
+        for (int kg = 0; kg < jw; kg += 32) {
+          Frag bc[PB];
+#pragma unroll
+          for (int t = 0; t < PB; ++t) bc[t] = bn[t];
+          if (kg + 32 < jw) load_b(kg + 32);
+          Frag af[2];
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            if (r >= nr) {
+              af[r] = af[0];
+              continue;
+            }
+            const int o = (arow + r * 16 + li) * WP + kg + 8 * lk;
+            af[r].h = *(const bf16x8*)(lds + o);
+            af[r].m = *(const bf16x8*)(lds + TM * WP + o);
+            af[r].l = *(const bf16x8*)(lds + 2 * TM * WP + o);
+          }
+#pragma unroll
+          for (int r = 0; r < 2; ++r)
+            if (r < nrw)
+#pragma unroll
+              for (int t = 0; t < PB; ++t)
+                if (t < nvb) mma6(accV[r][t], af[r], bc[t]);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const bool direct = sg.p_off < 0;
+  double* out = direct ? v + vb.v_off : part + sg.p_off;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int t = 0; t < PB; ++t) {
+      const int col = colg + (wc + WC * t) * 16 + li;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = row0 + arow + r * 16 + 4 * lk + i;
+        if (r < nr && row < vb.m && col < vb.n) {
+          if (direct)
+            out[(size_t)row * vb.n + col] += accV[r][t][i];
+          else
+            out[(size_t)row * vb.n + col] = accV[r][t][i];
+        }
+      }
+    }
+}
+
 // ---------------------------------------------------------------- FP64 small-block kernel
 // V blocks of at most 32 x 32 (2 x 2 blocks of 16) are latency-bound in the
 // grouped kernel: their pairs are tiny and every pair pays the barriers and
@@ -940,6 +1269,7 @@ struct bsg_context {
 struct PlanChunk {
   bsg_vblock_t* d_vb = nullptr;
   bsg_pair_t* d_pr = nullptr;
+  PairS* d_ps = nullptr;  // slice offsets of the grouped kernel's pairs (split precision)
   Seg* d_seg = nullptr;
   TileRef* d_g = nullptr;  // FP64 grouped-kernel tiles (costliest first)
   int n_g = 0;
@@ -956,8 +1286,14 @@ struct bsg_plan {
   int64_t wave_size = 0;
   int64_t arena_extent = 0;  // doubles of operator arena the pairs reach
   double flops = 0.0, flops1 = 0.0, flops2 = 0.0;
-  // split precision: the arena bsg_plan_split_arena accepted
+  // split precision: the arena bsg_plan_split_arena sliced, the slice
+  // recipes of the operator blocks (once per arena) and of the C blocks
+  // (every application) and their BF16 buffers
   const double* split_src = nullptr;
+  std::vector<SliceItem> ab_items, c_items;
+  int64_t ab_elems = 0, c_elems = 0;
+  SliceItem *d_ab_items = nullptr, *d_c_items = nullptr;
+  __bf16 *d_abs = nullptr, *d_cs = nullptr;
 };
 
 extern "C" {
@@ -1091,11 +1427,38 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     bsg_plan_destroy(p);
     return fail(code, m);
   };
+  // BF16 slice recipes, one per distinct (block, orientation)
+  std::map<std::tuple<int64_t, int, int, int>, int64_t> ab_map;
+  std::map<int64_t, int64_t> c_map;
+  auto slice_of = [&](std::vector<SliceItem>& items, int64_t& elems, int64_t src, int rows, int cols, int trans) {
+    SliceItem it;
+    it.src = src;
+    it.dst = elems;
+    it.rows = rows;
+    it.cols = cols;
+    it.pitch = (cols + 7) & ~7;
+    it.trans = trans;
+    items.push_back(it);
+    elems += 3 * (int64_t)rows * it.pitch;
+    return it.dst;
+  };
+  auto ab_slice = [&](int64_t off, int trans, int rows, int cols) {
+    auto key = std::make_tuple(off, trans, rows, cols);
+    auto f = ab_map.find(key);
+    if (f != ab_map.end()) return f->second;
+    return ab_map[key] = slice_of(p->ab_items, p->ab_elems, off, rows, cols, trans);
+  };
+  auto c_slice = [&](int64_t off, int k1, int k2) {
+    auto f = c_map.find(off);
+    if (f != c_map.end()) return f->second;
+    return c_map[off] = slice_of(p->c_items, p->c_elems, off, k2, k1, 1);  // C^T: (j, p) = C[p k2 + j]
+  };
   for (int ci = 0; ci < n_chunks; ++ci) {
     const bsg_chunk_t& ch = chunks[ci];
     PlanChunk pc;
     std::vector<GUnit> gunits;  // the tiles of each work unit
     std::vector<bsg_pair_t> perm;  // pairs in device order (grouped per V block as given)
+    std::vector<PairS> pslices;    // their slice offsets
     std::vector<std::pair<long, int>> sunits[2];  // (cost, segment) of small-block units
     std::vector<Seg> segs;
     std::vector<RedItem> reds;
@@ -1131,8 +1494,21 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       }
       // the block's pairs in device order: the small-kernel ones, then the rest
       const int base = (int)perm.size();
-      for (int x : fast) perm.push_back(ch.pairs[x]);
-      for (int x : slow) perm.push_back(ch.pairs[x]);
+      for (int x : fast) {
+        perm.push_back(ch.pairs[x]);
+        pslices.push_back({-1, -1, -1});
+      }
+      for (int x : slow) {
+        const auto& pr = ch.pairs[x];
+        perm.push_back(pr);
+        PairS q{-1, -1, -1};
+        if (pr.a_off >= 0) {
+          q.a = ab_slice(pr.a_off, pr.a_trans, vb.m, pr.k1);  // op(A)(i, p), K = p contiguous
+          q.c = c_slice(pr.c_off, pr.k1, pr.k2);
+        }
+        if (pr.b_off >= 0) q.b = ab_slice(pr.b_off, pr.b_trans ? 0 : 1, vb.n, pr.k2);  // B'(k, col) as [col][k]
+        pslices.push_back(q);
+      }
       // cut each list into work units of about SEG_COST(_SMALL) MFMAs per wave
       std::vector<std::pair<int, int>> units;  // (first, count) in device order
       std::vector<char> usmall;
@@ -1205,6 +1581,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     int rc = BSG_OK;
     if ((rc = up((void**)&pc.d_vb, ch.vblocks, sizeof(bsg_vblock_t) * ch.n_vblocks)) ||
         (rc = up((void**)&pc.d_pr, perm.data(), sizeof(bsg_pair_t) * perm.size())) ||
+        (rc = up((void**)&pc.d_ps, pslices.data(), sizeof(PairS) * pslices.size())) ||
         (rc = up((void**)&pc.d_seg, segs.data(), sizeof(Seg) * segs.size())) ||
         (rc = up((void**)&pc.d_red, reds.data(), sizeof(RedItem) * reds.size()))) {
       p->chunks.push_back(pc);
@@ -1244,6 +1621,13 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     hipError_t e = hipMalloc(&p->part, sizeof(double) * pmax);
     if (e != hipSuccess) return cleanup(BSG_ERR_NOMEM, "bsg_plan_create: partial-tile scratch");
   }
+  for (auto* v : {&p->ab_items, &p->c_items})
+    if (!v->empty()) {
+      SliceItem** d = v == &p->ab_items ? &p->d_ab_items : &p->d_c_items;
+      if (hipMalloc((void**)d, sizeof(SliceItem) * v->size()) != hipSuccess ||
+          hipMemcpy(*d, v->data(), sizeof(SliceItem) * v->size(), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(BSG_ERR_NOMEM, "bsg_plan_create: slice recipes");
+    }
   *plan = p;
   return BSG_OK;
 }
@@ -1254,6 +1638,7 @@ int bsg_plan_destroy(bsg_plan* p) {
   for (auto& c : p->chunks) {
     (void)hipFree(c.d_vb);
     (void)hipFree(c.d_pr);
+    (void)hipFree(c.d_ps);
     (void)hipFree(c.d_seg);
     (void)hipFree(c.d_red);
     (void)hipFree(c.d_g);
@@ -1261,6 +1646,10 @@ int bsg_plan_destroy(bsg_plan* p) {
     (void)hipFree(c.d_s[1]);
   }
   (void)hipFree(p->part);
+  (void)hipFree(p->d_ab_items);
+  (void)hipFree(p->d_c_items);
+  (void)hipFree(p->d_abs);
+  (void)hipFree(p->d_cs);
   delete p;
   return BSG_OK;
 }
@@ -1271,7 +1660,19 @@ int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* p, const double* arena, int
   if (!ctx || !p || !arena || n < 0) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: bad args");
   if (p->ctx != ctx) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: plan belongs to another context");
   if (n < p->arena_extent) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: arena smaller than the plan's extent");
-  // the kernels split operands into BF16 slices in registers: nothing to build
+  // operator slices for the grouped kernel (the small-block kernel splits
+  // in registers), and room for the C slices of one application
+  if (!p->d_abs && p->ab_elems > 0) {
+    hipError_t e = hipMalloc((void**)&p->d_abs, 2 * p->ab_elems);
+    if (e != hipSuccess) return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: operator slices");
+  }
+  if (!p->d_cs && p->c_elems > 0) {
+    hipError_t e = hipMalloc((void**)&p->d_cs, 2 * p->c_elems);
+    if (e != hipSuccess) return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: wavefunction slices");
+  }
+  if (!p->ab_items.empty())
+    hipLaunchKernelGGL(k_slices, dim3(p->ab_items.size()), dim3(256), 0, ctx->stream, p->d_ab_items, arena, p->d_abs);
+  HIPCHK(hipGetLastError());
   p->split_src = arena;
   return BSG_OK;
 }
@@ -1314,8 +1715,8 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
   }
   if (ch.n_g > 0) {
     if (split)
-      hipLaunchKernelGGL(k_sigma<true>, dim3(ch.n_g), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g, ch.n_g,
-                         arena, c, v, p->part);
+      hipLaunchKernelGGL(k_sigma_bf, dim3(ch.n_g), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_ps, ch.d_seg, ch.d_g,
+                         ch.n_g, p->d_abs, p->d_cs, c, v, p->part);
     else
       hipLaunchKernelGGL(k_sigma<false>, dim3(ch.n_g), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g, ch.n_g,
                          arena, c, v, p->part);
@@ -1331,7 +1732,7 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
 int check_precision(const bsg_plan* p, const double* arena, int32_t precision) {
   if (precision == BSG_FP64) return BSG_OK;
   if (precision != BSG_SPLIT_BF16) return fail(BSG_ERR_ARG, "unknown precision mode");
-  if (p->split_src != arena)
+  if (p->split_src != arena || (p->ab_elems > 0 && !p->d_abs) || (p->c_elems > 0 && !p->d_cs))
     return fail(BSG_ERR_ARG, "split precision needs bsg_plan_split_arena on this arena first");
   return BSG_OK;
 }
@@ -1349,6 +1750,8 @@ int bsg_sigma_apply(bsg_context* ctx, bsg_plan* p, const double* arena, const do
   for (int iv = 0; iv < nvec; ++iv) {
     const double* cv = c + (size_t)iv * p->wave_size;
     double* vv = v + (size_t)iv * p->wave_size;
+    if (split && !p->c_items.empty())
+      hipLaunchKernelGGL(k_slices, dim3(p->c_items.size()), dim3(256), 0, ctx->stream, p->d_c_items, cv, p->d_cs);
     for (auto& ch : p->chunks)
       if ((rc = enqueue_chunk(ctx, p, ch, arena, cv, vv, nullptr, split))) return rc;
   }
@@ -1368,11 +1771,14 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipEventRecord(ev[0], ctx->stream));
   size_t q = 1;
-  for (int r = 0; r < reps; ++r)
+  for (int r = 0; r < reps; ++r) {
+    if (split && !p->c_items.empty())
+      hipLaunchKernelGGL(k_slices, dim3(p->c_items.size()), dim3(256), 0, ctx->stream, p->d_c_items, c, p->d_cs);
     for (auto& ch : p->chunks) {
       if ((rc = enqueue_chunk(ctx, p, ch, arena, c, v, &ev[q], split))) return rc;
       q += per;
     }
+  }
   HIPCHK(hipEventRecord(ev[q], ctx->stream));
   HIPCHK(hipEventSynchronize(ev[q]));
   float ms = 0.f;

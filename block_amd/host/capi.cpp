@@ -189,7 +189,8 @@ int blkh_gpu_sigma_timed(void* h, const double* c, int warmup, int reps, double*
 // discarded, seconds, sigma_seconds, sigma_flops, sigma calls, split calls.
 // Returns the number of sweeps run.
 int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
-                  const double* sched, int max_sweeps, double sweep_tol, double ladder, int guess, int device,
+                  const double* sched, int max_sweeps, double sweep_tol, double ladder, int ladder_from, int guess,
+                  int device,
                   int verbose, double* energies_out, double* sweep_out) {
   return guarded([&]() {
     const Integrals& I = *static_cast<Integrals*>(ints);
@@ -205,6 +206,7 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
     P.sweep_tol = sweep_tol;
     P.verbose = verbose;
     P.ladder = ladder;
+    P.ladder_from_sweep = ladder_from;
     P.transform_guess = guess != 0;
     GpuContext ctx(device);
     auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {

@@ -206,7 +206,7 @@ struct Solved {
 
 // Solver::solve_wavefunction (solver.C:19-143): diagonal, guess, block Davidson
 Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, const SigmaFactory& make_sigma,
-                        std::mt19937_64& rng, const std::vector<std::vector<double>>* guess = nullptr) {
+                        std::mt19937_64& rng, const std::vector<std::vector<double>>* guess, double ladder) {
   std::normal_distribution<double> gauss(0.0, 1.0);
   const long N = sb.wave->size;
   if (N < P.nroots) throw std::runtime_error("run_dmrg: superblock smaller than the number of roots");
@@ -238,8 +238,8 @@ Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, c
   }
   out.e = diag;
   const int dmax = std::max(P.deflation_max, P.nroots + 2);
-  if (!hfun->davidson(out.b, out.e, tol, true, P.deflation_min, dmax, &out.st, P.davidson_max_iter, P.ladder))
-    block_davidson(out.b, out.e, tol, *hfun, true, P.deflation_min, dmax, &out.st, P.davidson_max_iter, P.ladder);
+  if (!hfun->davidson(out.b, out.e, tol, true, P.deflation_min, dmax, &out.st, P.davidson_max_iter, ladder))
+    block_davidson(out.b, out.e, tol, *hfun, true, P.deflation_min, dmax, &out.st, P.davidson_max_iter, ladder);
   out.e.resize(P.nroots);
   out.st.setup_seconds = t1 - t0;
   out.st.diag_seconds = t2 - t1;
@@ -349,7 +349,7 @@ SuperBlock infinite_warmup(const Integrals& I, const DmrgParams& P, const SigmaF
     BlockPtr R = build_sum_block(I, *bwd[k - 1 - s], *dot2, target, k - (s + 2), sr, scr);
     SuperBlock sb = make_superblock(I, L, R, tp);
     if (sb.wave->size < P.nroots) throw std::runtime_error("infinite_warmup: partial superblock too small");
-    Solved sol = solve_superblock(sb, P, tol, make_sigma, rng);
+    Solved sol = solve_superblock(sb, P, tol, make_sigma, rng, nullptr, P.ladder_from_sweep <= 0 ? P.ladder : 0.0);
     if (P.verbose > 1)
       std::fprintf(stderr, "  warmup step %d  sites %d  N=%d  wave=%ld  E0=%.12f iters=%d\n", s, kp, tp.n,
                    sb.wave->size, sol.e[0], sol.st.iterations);
@@ -455,7 +455,8 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
         }
       }
       rec.guess_seconds += now_s() - ts;
-      Solved sol = solve_superblock(sb, P, S->davidson_tol, make_sigma, rng, have_guess ? &guess : nullptr);
+      Solved sol = solve_superblock(sb, P, S->davidson_tol, make_sigma, rng, have_guess ? &guess : nullptr,
+                                    sweep >= P.ladder_from_sweep ? P.ladder : 0.0);
       rec.guessed_steps += sol.guessed;
       rec.davidson_seconds += now_s() - ts - sol.st.sigma_seconds;
       rec.setup_seconds += sol.st.setup_seconds;
