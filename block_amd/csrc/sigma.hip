@@ -175,41 +175,44 @@ __device__ inline void mma_split32(int nv, int nr, dbl4 (&acc)[NR][NB], const do
 
 // acc[r][t] += A(row blocks of this wave, 4-deep K step at k) * b[t] for the
 // first nv column blocks and the first nr row blocks; A k-major in LDS.
-template <int NV>
-__device__ inline void mma_step(int nv, int nr, dbl4 (&acc)[2][PB], const double* Al, int P, int arow, int k, int li, int lk,
-                                const double (&b)[PB]) {
+template <int NRM, int NB>
+__device__ inline void mma_step(int nv, int nr, dbl4 (&acc)[NRM][NB], const double* Al, int P, int arow, int k, int li,
+                                int lk, const double (&b)[NB]) {
   const double a0 = Al[(k + lk) * P + arow + li];
   double a1 = 0.0;
-  if (nr > 1) a1 = Al[(k + lk) * P + arow + 16 + li];
+  if (NRM > 1 && nr > 1) a1 = Al[(k + lk) * P + arow + 16 + li];
 #pragma unroll
-  for (int t = 0; t < NV; ++t) {
+  for (int t = 0; t < NB; ++t) {
     if (t < nv) {
       acc[0][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b[t], acc[0][t], 0, 0, 0);
-      if (nr > 1) acc[1][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b[t], acc[1][t], 0, 0, 0);
+      if constexpr (NRM > 1)
+        if (nr > 1) acc[1][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b[t], acc[1][t], 0, 0, 0);
     }
   }
 }
-// 4 K steps (16 deep) from k0; steps at or past kend skipped
-template <int NV>
-__device__ inline void mma_run_n(int nv, int nr, dbl4 (&acc)[2][PB], const double* Al, int P, int arow, int k0, int kend,
-                                 int li, int lk, const double (&b)[4][PB]) {
+// S K steps (4 S deep) from k0; steps at or past kend skipped, column slots
+// past nv skipped (uniform branches)
+template <int NRM, int NB, int S = 4>
+__device__ inline void mma_run(int nv, int nr, dbl4 (&acc)[NRM][NB], const double* Al, int P, int arow, int k0, int kend,
+                               int li, int lk, const double (&b)[S][NB]) {
+  if (nv <= 0) return;
 #pragma unroll
-  for (int s = 0; s < 4; ++s)
-    if (k0 + 4 * s < kend) mma_step<NV>(nv, nr, acc, Al, P, arow, k0 + 4 * s, li, lk, b[s]);
-}
-__device__ inline void mma_run(int nv, int nr, dbl4 (&acc)[2][PB], const double* Al, int P, int arow, int k0,
-                               int kend, int li, int lk, const double (&b)[4][PB]) {
-  // column slots past nv are skipped (uniform branches)
-  if (nv > 0) mma_run_n<PB>(nv, nr, acc, Al, P, arow, k0, kend, li, lk, b);
+  for (int s = 0; s < S; ++s)
+    if (k0 + 4 * s < kend) mma_step<NRM, NB>(nv, nr, acc, Al, P, arow, k0 + 4 * s, li, lk, b[s]);
 }
 
-// Tile geometry code in TileRef::pad: bits 0-1 log2(WR), bit 2 (NR - 1).
-template <bool SPLIT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sigma(const bsg_vblock_t* __restrict__ vbs,
-                                               const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
-                                               const TileRef* __restrict__ tiles, int ntiles,
-                                               const double* __restrict__ arena, const double* __restrict__ c,
-                                               double* __restrict__ v, double* __restrict__ part) {
+// Tile geometry code in TileRef::pad: bits 0-1 log2(WR), bit 2 (NR - 1);
+// NRM: row blocks per wave the accumulators hold (NR <= NRM).
+// NV: V column blocks per wave (the V tile is 16 NV WC wide); the W chunk of
+// GEMM 1 is PB blocks per wave (16 PB WC wide), so a V tile wider than the W
+// chunk forms every W column once and contracts it into NV / PB chunks' worth
+// of V columns.
+template <int NRM, int NV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sigma(
+    const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
+    const TileRef* __restrict__ tiles, int ntiles, const double* __restrict__ arena, const double* __restrict__ c,
+    double* __restrict__ v, double* __restrict__ part) {
+  static_assert(NV % PB == 0, "V tile width is a whole number of W chunks");
   constexpr int NT = 256;
   // GEMM 1 uses two op(A) chunk buffers, GEMM 2 the W chunk; they share the
   // LDS (barriers separate the phases).  Both k-major: X[k * P + row].
@@ -224,36 +227,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   if (tr.item < 0) return;  // padding of the XCD-interleaved order
   const Seg sg = segs[tr.item];
   const bsg_vblock_t vb = vbs[sg.vb];
-  const int lwr = tr.pad & 3, nr = ((tr.pad >> 2) & 1) + 1;
+  const int lwr = tr.pad & 3, nr = min(((tr.pad >> 2) & 1) + 1, NRM);
   const int WR = 1 << lwr, WC = 4 >> lwr;
-  const int TM = 16 * nr * WR, JC = 16 * PB * WC, P = TM + 1;
-  const int row0 = tr.tm * TM, colg = tr.tn * JC;
+  const int TM = 16 * nr * WR, JC = 16 * PB * WC, JV = 16 * NV * WC, P = TM + 1;
+  const int row0 = tr.tm * TM, colg = tr.tn * JV;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wr = w / WC, wc = w % WC;
   const int arow = wr * nr * 16;  // first panel row of this wave
   const int li = lane & 15, lk = lane >> 4;
-  // row of accumulator register i inside its 16 x 16 block: lk + 4 i for
-  // v_mfma_f64_16x16x4f64, 4 lk + i for the split-precision BF16 MFMA
-  const int rq = SPLIT ? 4 * lk : lk, rs = SPLIT ? 1 : 4;
   // row blocks of this wave inside the block (0, 1 or 2)
   int nrw = (vb.m - row0 - arow + 15) / 16;
   nrw = nrw < 0 ? 0 : (nrw > nr ? nr : nrw);
   int nvb = 0;  // V column blocks of this wave that exist
-  for (int t = 0; t < PB; ++t)
+  for (int t = 0; t < NV; ++t)
     if (colg + (wc + WC * t) * 16 < vb.n) nvb = t + 1;
   if (nrw == 0) nvb = 0;
-  dbl4 accV[2][PB];
+  dbl4 accV[NRM][NV];
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
+  for (int r = 0; r < NRM; ++r)
 #pragma unroll
-    for (int t = 0; t < PB; ++t) accV[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < NV; ++t) accV[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
   int abuf = 0;
   for (int q = 0; q < sg.count; ++q) {
     const bsg_pair_t pr = prs[sg.first + q];
     const __amdgpu_buffer_rsrc_t rC = block_rsrc(c + pr.c_off, (int64_t)pr.k1 * pr.k2);
+    if (pr.a_off < 0 && pr.b_off < 0) {
+      // identity A and B': V += alpha C
+#pragma unroll
+      for (int r = 0; r < NRM; ++r)
+#pragma unroll
+        for (int t = 0; t < NV; ++t) {
+          const int col = colg + (wc + WC * t) * 16 + li;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = row0 + arow + r * 16 + lk + 4 * i;
+            const bool ok = r < nr && row < vb.m && col < vb.n;
+            accV[r][t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
+          }
+        }
+      continue;
+    }
     for (int j0 = 0; j0 < pr.k2; j0 += JC) {
-      if (pr.b_off < 0 && j0 != colg) continue;  // identity B': only the matching chunk
+      // identity B': only the W chunks of this tile's V columns
+      if (pr.b_off < 0 && (j0 < colg || j0 >= colg + JV)) continue;
       const int jw = min(JC, pr.k2 - j0);        // W columns in this chunk
       int nwb = 0;                               // W column blocks of this wave in the chunk
       for (int t = 0; t < PB; ++t)
@@ -261,21 +278,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       if (nrw == 0) nwb = 0;
       if (pr.a_off < 0) {
         // identity A: W = alpha C (panel rows of C)
-        if (pr.b_off < 0) {
-#pragma unroll
-          for (int r = 0; r < 2; ++r)
-#pragma unroll
-            for (int t = 0; t < PB; ++t) {
-              const int col = colg + (wc + WC * t) * 16 + li;
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                const int row = row0 + arow + r * 16 + rq + rs * i;
-                const bool ok = r < nr && row < vb.m && col < vb.n;
-                accV[r][t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
-              }
-            }
-          continue;
-        }
         for (int e = threadIdx.x; e < TM * JC; e += NT) {
           const int j = e % JC, i = e / JC;
           const int gi = row0 + i;
@@ -284,9 +286,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
       } else {
         const __amdgpu_buffer_rsrc_t rA = block_rsrc(arena + pr.a_off, (int64_t)vb.m * pr.k1);
-        dbl4 accW[2][PB];
+        dbl4 accW[NRM][PB];
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < NRM; ++r)
 #pragma unroll
           for (int t = 0; t < PB; ++t) accW[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
         // Pipeline over 32-deep chunks, each consumed in two 16-deep halves:
@@ -296,7 +298,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         // loop-invariant; the K position goes in the uniform offset.  Loads
         // past the end of a block read zero (buffer range check); op(A)
         // elements loaded past k1 inside the block only meet zero C rows.
-        constexpr int AE = GTM * PKC / NT;
+        // wide tiles are always one 16-row block per wave in one wave row
+        constexpr int AE = (NV > PB ? 16 : GTM) * PKC / NT;
         uint32_t aoff[AE];
 #pragma unroll
         for (int u = 0; u < AE; ++u) {
@@ -339,127 +342,87 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
               Ab[p * P + i] = pr.alpha_a * an[u];
             }
           }
-          if constexpr (!SPLIT) {
-            double bc[4][PB];
+          double bc[4][PB];
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+            for (int t = 0; t < PB; ++t) bc[s4][t] = bn[s4][t];
+          __syncthreads();
+          if (kc > 16) load_c(k0, 16);
+          mma_run<NRM, PB>(nwb, nrw, accW, Ab, P, arow, 0, kc, li, lk, bc);
+          if (kc > 16) {
 #pragma unroll
             for (int s4 = 0; s4 < 4; ++s4)
 #pragma unroll
               for (int t = 0; t < PB; ++t) bc[s4][t] = bn[s4][t];
-            __syncthreads();
-            if (kc > 16) load_c(k0, 16);
-            mma_run(nwb, nrw, accW, Ab, P, arow, 0, kc, li, lk, bc);
-            if (kc > 16) {
-#pragma unroll
-              for (int s4 = 0; s4 < 4; ++s4)
-#pragma unroll
-                for (int t = 0; t < PB; ++t) bc[s4][t] = bn[s4][t];
-            }
-            if (k0 + PKC < pr.k1) {
-              load_a(k0 + PKC);
-              load_c(k0 + PKC, 0);
-            }
-            if (kc > 16) mma_run(nwb, nrw, accW, Ab, P, arow, 16, kc, li, lk, bc);
-          } else {
-            // one 32-deep split step over both 16-deep halves (slot e: K row 4e + lk)
-            double bs[PB][8], as[2][8];
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4)
-#pragma unroll
-              for (int t = 0; t < PB; ++t) bs[t][s4] = bn[s4][t];
-            __syncthreads();
-            if (kc > 16) load_c(k0, 16);
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4)
-#pragma unroll
-              for (int t = 0; t < PB; ++t) bs[t][4 + s4] = kc > 16 ? bn[s4][t] : 0.0;
-            if (k0 + PKC < pr.k1) {
-              load_a(k0 + PKC);
-              load_c(k0 + PKC, 0);
-            }
-#pragma unroll
-            for (int r = 0; r < 2; ++r)
-#pragma unroll
-              for (int e = 0; e < 8; ++e) as[r][e] = r < nrw ? Ab[(4 * e + lk) * P + arow + r * 16 + li] : 0.0;
-            if (nwb > 0) mma_split32<2, PB>(nwb, nrw, accW, as, bs);
           }
+          if (k0 + PKC < pr.k1) {
+            load_a(k0 + PKC);
+            load_c(k0 + PKC, 0);
+          }
+          if (kc > 16) mma_run<NRM, PB>(nwb, nrw, accW, Ab, P, arow, 16, kc, li, lk, bc);
         }
         __syncthreads();  // op(A) buffers done before W (same LDS) is written
         if (pr.b_off < 0) {
-          // identity B': the W blocks of this wave are its V blocks
+          // identity B': the W blocks of this wave are V blocks t + ch PB of
+          // the tile, ch = the chunk's position in the tile
+          const int ch = (j0 - colg) / JC;
 #pragma unroll
-          for (int r = 0; r < 2; ++r)
+          for (int cc = 0; cc < NV / PB; ++cc)
+            if (cc == ch)
 #pragma unroll
-            for (int t = 0; t < PB; ++t)
+              for (int r = 0; r < NRM; ++r)
 #pragma unroll
-              for (int i = 0; i < 4; ++i) accV[r][t][i] += pr.alpha * accW[r][t][i];
+                for (int t = 0; t < PB; ++t)
+#pragma unroll
+                  for (int i = 0; i < 4; ++i) accV[r][cc * PB + t][i] += pr.alpha * accW[r][t][i];
           continue;
         }
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < NRM; ++r)
 #pragma unroll
           for (int t = 0; t < PB; ++t) {
             const int cb = wc + WC * t;
             if (r < nr)
 #pragma unroll
-              for (int i = 0; i < 4; ++i)
-                Ws[(cb * 16 + li) * P + arow + r * 16 + rq + rs * i] = pr.alpha * accW[r][t][i];
+              for (int i = 0; i < 4; ++i) Ws[(cb * 16 + li) * P + arow + r * 16 + lk + 4 * i] = pr.alpha * accW[r][t][i];
           }
       }
       __syncthreads();
-      // GEMM 2: V[tile] += W chunk * B'[j0:j0+jw, group columns]; W columns
+      // GEMM 2: V[tile] += W chunk * B'[j0:j0+jw, tile columns]; W columns
       // past jw up to the next multiple of 4 are zero.  B' operands of the
-      // next 32-deep batch are loaded while the current one runs.
+      // next batch (16 deep, 8 for the widest tiles: register budget) are
+      // loaded while the current one runs.
       {
+        constexpr int S2 = NV >= 6 ? 2 : 4;
         const __amdgpu_buffer_rsrc_t rB = block_rsrc(arena + pr.b_off, (int64_t)vb.n * pr.k2);
         // B'(k, col) at k * ks + col * cs inside the block
         const int ks = pr.b_trans ? 1 : vb.n, cs = pr.b_trans ? pr.k2 : 1;
         // per-lane offsets loop-invariant, K position in the uniform offset;
         // rows past k2 read zero or meet zero W columns
-        uint32_t boff[PB];
+        uint32_t boff[NV];
 #pragma unroll
-        for (int t = 0; t < PB; ++t) {
+        for (int t = 0; t < NV; ++t) {
           const int col = colg + (wc + WC * t) * 16 + li;
           boff[t] = (t < nvb && col < vb.n) ? (uint32_t)(lk * ks + col * cs) * 8u : OOB;
         }
         const uint32_t bstep = (uint32_t)ks * 8u;
-        double bn[4][PB];
+        double bn[S2][NV];
         auto load_b = [&](int kg) {
 #pragma unroll
-          for (int s4 = 0; s4 < 4; ++s4)
+          for (int s4 = 0; s4 < S2; ++s4)
 #pragma unroll
-            for (int t = 0; t < PB; ++t) bn[s4][t] = bload2(rB, boff[t], (uint32_t)(j0 + kg + 4 * s4) * bstep);
+            for (int t = 0; t < NV; ++t) bn[s4][t] = bload2(rB, boff[t], (uint32_t)(j0 + kg + 4 * s4) * bstep);
         };
         load_b(0);
-        if constexpr (!SPLIT) {
-          for (int kg = 0; kg < jw; kg += 16) {
-            double bb[4][PB];
+        for (int kg = 0; kg < jw; kg += 4 * S2) {
+          double bb[S2][NV];
 #pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4)
+          for (int s4 = 0; s4 < S2; ++s4)
 #pragma unroll
-              for (int t = 0; t < PB; ++t) bb[s4][t] = bn[s4][t];
-            if (kg + 16 < jw) load_b(kg + 16);
-            mma_run(nvb, nrw, accV, Ws, P, arow, kg, jw, li, lk, bb);
-          }
-        } else {
-          // 32-deep split steps (the W chunk width is a multiple of 32)
-          for (int kg = 0; kg < jw; kg += 32) {
-            double bs[PB][8], as[2][8];
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4)
-#pragma unroll
-              for (int t = 0; t < PB; ++t) bs[t][s4] = bn[s4][t];
-            if (kg + 16 < jw) load_b(kg + 16);
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4)
-#pragma unroll
-              for (int t = 0; t < PB; ++t) bs[t][4 + s4] = kg + 16 < jw ? bn[s4][t] : 0.0;
-            if (kg + 32 < jw) load_b(kg + 32);
-#pragma unroll
-            for (int r = 0; r < 2; ++r)
-#pragma unroll
-              for (int e = 0; e < 8; ++e) as[r][e] = r < nrw ? Ws[(kg + 4 * e + lk) * P + arow + r * 16 + li] : 0.0;
-            if (nvb > 0) mma_split32<2, PB>(nvb, nrw, accV, as, bs);
-          }
+            for (int t = 0; t < NV; ++t) bb[s4][t] = bn[s4][t];
+          if (kg + 4 * S2 < jw) load_b(kg + 4 * S2);
+          mma_run<NRM, NV, S2>(nvb, nrw, accV, Ws, P, arow, kg, jw, li, lk, bb);
         }
       }
       __syncthreads();
@@ -468,13 +431,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const bool direct = sg.p_off < 0;
   double* out = direct ? v + vb.v_off : part + sg.p_off;
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
+  for (int r = 0; r < NRM; ++r)
 #pragma unroll
-    for (int t = 0; t < PB; ++t) {
+    for (int t = 0; t < NV; ++t) {
       const int col = colg + (wc + WC * t) * 16 + li;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int row = row0 + arow + r * 16 + rq + rs * i;
+        const int row = row0 + arow + r * 16 + lk + 4 * i;
         if (r < nr && row < vb.m && col < vb.n) {
           if (direct)
             out[(size_t)row * vb.n + col] += accV[r][t][i];
@@ -1015,46 +978,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 
 
-// Tile geometry of the FP64 grouped kernel for an m x n V block: waves WR x
-// WC (WR WC = 4) and 16-row blocks per wave NR, TM = 16 NR WR <= 64 rows and
-// JC = 16 PB WC columns per tile.  Chosen to minimise the MFMA issue time of
-// the block's tiles (per-wave MFMAs per pair, summed over tiles), which
-// counts row padding, idle column slots and GEMM 1 repeated per column group.
+// Tile geometry of the grouped kernels for an m x n V block: waves WR x WC
+// (WR WC = 4), NR 16-row blocks per wave (TM = 16 NR WR <= 64 rows), a W
+// chunk of JC = 16 PB WC columns and a V tile of JV = 16 NV WC columns.
+// NV = PB is the basic tile; the wide tiles (WR = 1, NR = 1, NV = 4 or 6)
+// cover up to 256 or 384 V columns, so every W column of a pair is formed
+// once per row panel instead of once per column group.  cls: the FP64
+// kernel instantiation (0: <2, 2>, 1: <1, 4>, 2: <1, 6>); the split-precision
+// kernel runs every tile class as basic tiles (JV = JC).  Chosen to minimise
+// the MFMA issue time of the block's tiles (per-wave MFMAs per pair, summed
+// over tiles), which counts row padding, idle column slots and GEMM 1
+// repeated per column group.
+constexpr int NCLS = 3;
 struct Geom {
-  int lwr, nr, tm, jc, code;
+  int lwr, nr, nv, tm, jc, jv, code, cls;
 };
 inline long geom_cost(const Geom& g, long m, long n, long k) {
   auto cdiv = [](long a, long b) { return (a + b - 1) / b; };
   const int wc = 4 >> g.lwr;
-  const long panels = cdiv(m, g.tm), groups = cdiv(n, g.jc);
-  const long wblk = cdiv(cdiv(k, 16), wc);          // W column blocks per wave (all chunks)
-  const long vblk = std::min<long>(PB, cdiv(std::min<long>(n, g.jc), 16 * wc) > 0 ? cdiv(std::min<long>(n, g.jc), 16 * wc) : 1);
+  const long panels = cdiv(m, g.tm), groups = cdiv(n, g.jv);
+  const long wblk = cdiv(cdiv(k, 16), wc);  // W column blocks per wave (all chunks)
+  const long vblk = std::max<long>(1, std::min<long>(g.nv, cdiv(std::min<long>(n, g.jv), 16 * wc)));
   return panels * groups * g.nr * (wblk + vblk) * cdiv(k, 4);
 }
-inline Geom make_geom(int lwr, int nr) {
+inline Geom make_geom(int lwr, int nr, int nv) {
   Geom g;
   g.lwr = lwr;
   g.nr = nr;
+  g.nv = nv;
   g.tm = 16 * nr * (1 << lwr);
   g.jc = 16 * PB * (4 >> lwr);
+  g.jv = 16 * nv * (4 >> lwr);
   g.code = lwr | ((nr - 1) << 2);
+  g.cls = nv / 2 - 1;
   return g;
 }
-inline Geom pick_geom(int m, int n) {
+inline Geom pick_geom(int m, int n, bool wide) {
   // typical contraction depth: the block's own dimensions
   const long k = std::max(m, n);
-  Geom best = make_geom(0, 1);
-  long bc = -1;
+  std::vector<Geom> cand;
   for (int lwr = 0; lwr <= 2; ++lwr)
-    for (int nr = 1; nr <= 2; ++nr) {
-      const Geom g = make_geom(lwr, nr);
-      if (g.tm > GTM) continue;
-      const long cst = geom_cost(g, m, n, k);
-      if (bc < 0 || cst < bc || (cst == bc && g.tm > best.tm)) {
-        bc = cst;
-        best = g;
-      }
+    for (int nr = 1; nr <= 2; ++nr) cand.push_back(make_geom(lwr, nr, PB));
+  if (wide) {
+    cand.push_back(make_geom(0, 1, 4));
+    cand.push_back(make_geom(0, 1, 6));
+  }
+  Geom best = cand[0];
+  long bc = -1;
+  for (const Geom& g : cand) {
+    if (g.tm > GTM) continue;
+    const long cst = geom_cost(g, m, n, k);
+    if (bc < 0 || cst < bc || (cst == bc && g.tm > best.tm)) {
+      bc = cst;
+      best = g;
     }
+  }
   return best;
 }
 // MFMA instructions of one pair in the small-block kernel plus a latency
@@ -1070,9 +1048,11 @@ inline long small_cost(int m, int n, const bsg_pair_t& pr) {
 inline long pair_cost(const Geom& g, int n, const bsg_pair_t& pr) {
   auto cdiv = [](long a, long b) { return (a + b - 1) / b; };
   const int wc = 4 >> g.lwr;
-  const long nch = pr.b_off < 0 ? 1 : cdiv(pr.k2, g.jc);
-  const long g1 = pr.a_off < 0 ? 0 : g.nr * cdiv(cdiv(pr.k2, 16), wc) * cdiv(pr.k1, 4);
-  const long g2 = pr.b_off < 0 ? 0 : g.nr * std::min<long>(PB, cdiv(cdiv(std::min<long>(n, g.jc), 16), wc)) * cdiv(std::min<long>(pr.k2, g.jc), 4) * nch;
+  const long kw = pr.b_off < 0 ? std::min<long>(pr.k2, g.jv) : pr.k2;  // W columns formed per tile
+  const long nch = cdiv(kw, g.jc);
+  const long g1 = pr.a_off < 0 ? 0 : g.nr * cdiv(cdiv(kw, 16), wc) * cdiv(pr.k1, 4);
+  const long g2 =
+      pr.b_off < 0 ? 0 : g.nr * std::min<long>(g.nv, cdiv(cdiv(std::min<long>(n, g.jv), 16), wc)) * cdiv(pr.k2, 4);
   return g1 + g2 + 16 * nch;
 }
 
@@ -1271,12 +1251,14 @@ struct PlanChunk {
   bsg_pair_t* d_pr = nullptr;
   PairS* d_ps = nullptr;  // slice offsets of the grouped kernel's pairs (split precision)
   Seg* d_seg = nullptr;
-  TileRef* d_g = nullptr;  // FP64 grouped-kernel tiles (costliest first)
-  int n_g = 0;
+  TileRef* d_g[NCLS] = {nullptr, nullptr, nullptr};  // FP64 grouped-kernel tiles per kernel class
+  int n_g[NCLS] = {0, 0, 0};
+  TileRef* d_gbf = nullptr;  // the same units as basic tiles, for the split-precision kernel
+  int n_gbf = 0;
   int4* d_s[2] = {nullptr, nullptr};  // FP64 small-block units, 4 per workgroup (SB = 2, 3)
   int n_s[2] = {0, 0};
-  RedItem* d_red = nullptr;
-  int n_red = 0;
+  RedItem* d_red[2] = {nullptr, nullptr};  // after the FP64 / the split-precision kernels
+  int n_red[2] = {0, 0};
 };
 
 struct bsg_plan {
@@ -1456,19 +1438,19 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
   for (int ci = 0; ci < n_chunks; ++ci) {
     const bsg_chunk_t& ch = chunks[ci];
     PlanChunk pc;
-    std::vector<GUnit> gunits;  // the tiles of each work unit
+    std::vector<GUnit> gunits[NCLS], gunits_bf;  // the tiles of each work unit
     std::vector<bsg_pair_t> perm;  // pairs in device order (grouped per V block as given)
     std::vector<PairS> pslices;    // their slice offsets
     std::vector<std::pair<long, int>> sunits[2];  // (cost, segment) of small-block units
     std::vector<Seg> segs;
-    std::vector<RedItem> reds;
+    std::vector<RedItem> reds[2];  // ordered reductions after the FP64 / the split-precision kernels
     int64_t poff = 0;
     for (int i = 0; i < ch.n_vblocks; ++i) {
       const auto& vb = ch.vblocks[i];
       if (vb.m <= 0 || vb.n <= 0 || vb.v_off < 0 || vb.v_off + (int64_t)vb.m * vb.n > wave_size || vb.first < 0 ||
           vb.count < 0 || vb.first + vb.count > ch.n_pairs)
         return cleanup(BSG_ERR_ARG, "bsg_plan_create: vblock out of range");
-      const Geom gm = pick_geom(vb.m, vb.n);
+      const Geom gm = pick_geom(vb.m, vb.n, true);
       // small blocks (up to 32 x 32) go to the per-wave kernel, except pairs
       // whose C block is larger than that (or whose A is the identity and B
       // is not): those run in the grouped kernel with its K pipeline
@@ -1509,66 +1491,82 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         if (pr.b_off >= 0) q.b = ab_slice(pr.b_off, pr.b_trans ? 0 : 1, vb.n, pr.k2);  // B'(k, col) as [col][k]
         pslices.push_back(q);
       }
-      // cut each list into work units of about SEG_COST(_SMALL) MFMAs per wave
-      std::vector<std::pair<int, int>> units;  // (first, count) in device order
-      std::vector<char> usmall;
-      auto cut = [&](int first, int n, bool sm) {
-        int start = 0;
-        long cost = 0;
-        for (int q = 0; q < n; ++q) {
-          const auto& pr = perm[first + q];
-          cost += sm ? small_cost(vb.m, vb.n, pr) : pair_cost(gm, vb.n, pr);
-          if (cost >= (sm ? SEG_COST_SMALL : SEG_COST) && q + 1 < n) {
-            units.push_back({first + start, q + 1 - start});
-            usmall.push_back(sm);
-            start = q + 1;
-            cost = 0;
+      // cut each list into work units of about SEG_COST(_SMALL) MFMAs per
+      // wave.  A block on wide tiles is cut twice: for the FP64 wide tiles
+      // and for the split-precision kernel's basic tiles (mode 1 and 2, each
+      // with its own partial tiles and reduction); other blocks share one
+      // cut (mode 0).  Wide blocks have no small-kernel pairs.
+      const int64_t mn = (int64_t)vb.m * vb.n;
+      auto emit = [&](const Geom& g, int mode) {
+        std::vector<std::pair<int, int>> units;  // (first, count) in device order
+        std::vector<char> usmall;
+        auto cut = [&](int first, int n, bool sm) {
+          int start = 0;
+          long cost = 0;
+          for (int q = 0; q < n; ++q) {
+            const auto& pr = perm[first + q];
+            cost += sm ? small_cost(vb.m, vb.n, pr) : pair_cost(g, vb.n, pr);
+            if (cost >= (sm ? SEG_COST_SMALL : SEG_COST) && q + 1 < n) {
+              units.push_back({first + start, q + 1 - start});
+              usmall.push_back(sm);
+              start = q + 1;
+              cost = 0;
+            }
           }
+          if (n > start) {
+            units.push_back({first + start, n - start});
+            usmall.push_back(sm);
+          }
+        };
+        cut(base, (int)fast.size(), true);
+        cut(base + (int)fast.size(), (int)slow.size(), false);
+        const int nseg = (int)units.size();
+        for (int u = 0; u < nseg; ++u) {
+          Seg sg;
+          sg.vb = i;
+          sg.first = units[u].first;
+          sg.count = units[u].second;
+          sg.pad = 0;
+          sg.p_off = nseg == 1 ? -1 : poff + (int64_t)u * mn;
+          int si = (int)segs.size();
+          segs.push_back(sg);
+          if (usmall[u]) {
+            long sc = 0;
+            for (int q = 0; q < sg.count; ++q) sc += small_cost(vb.m, vb.n, perm[sg.first + q]);
+            sunits[0].push_back({sc, si});
+            continue;
+          }
+          long ucost = 0;
+          for (int q = 0; q < sg.count; ++q) ucost += pair_cost(g, vb.n, perm[sg.first + q]);
+          GUnit gu, gb;
+          gu.cost = gb.cost = ucost;
+          for (int tm = 0; tm < (vb.m + g.tm - 1) / g.tm; ++tm) {
+            for (int tn = 0; tn < (vb.n + g.jv - 1) / g.jv; ++tn) gu.tiles.push_back({si, tm, tn, g.code});
+            for (int tn = 0; tn < (vb.n + g.jc - 1) / g.jc; ++tn) gb.tiles.push_back({si, tm, tn, g.code});
+          }
+          if (mode != 2) gunits[g.cls].push_back(std::move(gu));
+          if (mode != 1) gunits_bf.push_back(std::move(gb));
         }
-        if (n > start) {
-          units.push_back({first + start, n - start});
-          usmall.push_back(sm);
+        if (nseg > 1) {
+          for (int64_t e0 = 0; e0 < mn; e0 += 256) {
+            RedItem r;
+            r.v_off = vb.v_off;
+            r.p_off = poff;
+            r.nseg = nseg;
+            r.mn = (int32_t)mn;
+            r.e0 = (int32_t)e0;
+            r.len = (int32_t)std::min<int64_t>(256, mn - e0);
+            if (mode != 2) reds[0].push_back(r);
+            if (mode != 1) reds[1].push_back(r);
+          }
+          poff += nseg * mn;
         }
       };
-      cut(base, (int)fast.size(), true);
-      cut(base + (int)fast.size(), (int)slow.size(), false);
-      const int nseg = (int)units.size();
-      const int64_t mn = (int64_t)vb.m * vb.n;
-      for (int u = 0; u < nseg; ++u) {
-        Seg sg;
-        sg.vb = i;
-        sg.first = units[u].first;
-        sg.count = units[u].second;
-        sg.pad = 0;
-        sg.p_off = nseg == 1 ? -1 : poff + (int64_t)u * mn;
-        int si = (int)segs.size();
-        segs.push_back(sg);
-        if (usmall[u]) {
-          long sc = 0;
-          for (int q = 0; q < sg.count; ++q) sc += small_cost(vb.m, vb.n, perm[sg.first + q]);
-          sunits[0].push_back({sc, si});
-          continue;
-        }
-        long ucost = 0;
-        for (int q = 0; q < sg.count; ++q) ucost += pair_cost(gm, vb.n, perm[sg.first + q]);
-        GUnit gu;
-        gu.cost = ucost;
-        for (int tm = 0; tm < (vb.m + gm.tm - 1) / gm.tm; ++tm)
-          for (int tn = 0; tn < (vb.n + gm.jc - 1) / gm.jc; ++tn) gu.tiles.push_back({si, tm, tn, gm.code});
-        gunits.push_back(std::move(gu));
-      }
-      if (nseg > 1) {
-        for (int64_t e0 = 0; e0 < mn; e0 += 256) {
-          RedItem r;
-          r.v_off = vb.v_off;
-          r.p_off = poff;
-          r.nseg = nseg;
-          r.mn = (int32_t)mn;
-          r.e0 = (int32_t)e0;
-          r.len = (int32_t)std::min<int64_t>(256, mn - e0);
-          reds.push_back(r);
-        }
-        poff += nseg * mn;
+      if (gm.nv == PB) {
+        emit(gm, 0);
+      } else {
+        emit(gm, 1);
+        emit(pick_geom(vb.m, vb.n, false), 2);
       }
     }
     pmax = std::max(pmax, poff);
@@ -1583,12 +1581,15 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         (rc = up((void**)&pc.d_pr, perm.data(), sizeof(bsg_pair_t) * perm.size())) ||
         (rc = up((void**)&pc.d_ps, pslices.data(), sizeof(PairS) * pslices.size())) ||
         (rc = up((void**)&pc.d_seg, segs.data(), sizeof(Seg) * segs.size())) ||
-        (rc = up((void**)&pc.d_red, reds.data(), sizeof(RedItem) * reds.size()))) {
+        (rc = up((void**)&pc.d_red[0], reds[0].data(), sizeof(RedItem) * reds[0].size())) ||
+        (rc = up((void**)&pc.d_red[1], reds[1].data(), sizeof(RedItem) * reds[1].size()))) {
       p->chunks.push_back(pc);
       std::string m = g_err;
       return cleanup(rc, m);
     }
-    std::vector<TileRef> gtiles = xcd_order(gunits);
+    std::vector<TileRef> gtiles[NCLS];
+    for (int k = 0; k < NCLS; ++k) gtiles[k] = xcd_order(gunits[k]);
+    std::vector<TileRef> gtiles_bf = xcd_order(gunits_bf);
     // small-block units, costliest first, four per workgroup
     std::vector<int4> sq[2];
     for (int k = 0; k < 2; ++k) {
@@ -1607,14 +1608,21 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       std::string m = g_err;
       return cleanup(rc, m);
     }
-    rc = up((void**)&pc.d_g, gtiles.data(), sizeof(TileRef) * gtiles.size());
-    pc.n_g = (int)gtiles.size();
+    for (int k = 0; k < NCLS && !rc; ++k) {
+      rc = up((void**)&pc.d_g[k], gtiles[k].data(), sizeof(TileRef) * gtiles[k].size());
+      pc.n_g[k] = (int)gtiles[k].size();
+    }
+    if (!rc) {
+      rc = up((void**)&pc.d_gbf, gtiles_bf.data(), sizeof(TileRef) * gtiles_bf.size());
+      pc.n_gbf = (int)gtiles_bf.size();
+    }
     if (rc) {
       p->chunks.push_back(pc);
       std::string m = g_err;
       return cleanup(rc, m);
     }
-    pc.n_red = (int)reds.size();
+    pc.n_red[0] = (int)reds[0].size();
+    pc.n_red[1] = (int)reds[1].size();
     p->chunks.push_back(pc);
   }
   if (pmax > 0) {
@@ -1640,8 +1648,10 @@ int bsg_plan_destroy(bsg_plan* p) {
     (void)hipFree(c.d_pr);
     (void)hipFree(c.d_ps);
     (void)hipFree(c.d_seg);
-    (void)hipFree(c.d_red);
-    (void)hipFree(c.d_g);
+    (void)hipFree(c.d_red[0]);
+    (void)hipFree(c.d_red[1]);
+    for (int k = 0; k < NCLS; ++k) (void)hipFree(c.d_g[k]);
+    (void)hipFree(c.d_gbf);
     (void)hipFree(c.d_s[0]);
     (void)hipFree(c.d_s[1]);
   }
@@ -1688,42 +1698,52 @@ int bsg_plan_phase_flops(const bsg_plan* p, double* f1, double* f2) {
 }  // extern "C"
 
 namespace {
-// Launch one chunk: the grouped kernel on the context stream with the
-// small-block kernel beside it on the side stream, then the ordered
-// reduction.  ev (optional) receives 5 events: before the fused kernels,
-// around the small-block kernel (side stream), before and after the
-// reduction.
+// Launch one chunk: the grouped kernels on the context stream with the
+// side-stream kernels beside them, then the ordered reduction.  ev
+// (optional) receives 5 events: before the fused kernels, around the side
+// stream's kernels, before and after the reduction.
 int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, const double* arena, const double* c,
                   double* v, hipEvent_t* ev, bool split) {
   hipStream_t st = ctx->stream;
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
-  // the small-block kernel runs on the side stream beside the grouped kernel
-  // (they write disjoint V blocks and partial tiles); the reduction waits
-  // for both
-  if (ch.n_s[0] > 0) {
+  // the side stream runs the small-block kernel (and, in FP64, the basic
+  // tiles) beside the wide tiles on the context stream; they write disjoint
+  // V blocks and partial tiles, and the reduction waits for both
+  const bool side = ch.n_s[0] > 0 || (!split && ch.n_g[0] > 0);
+  if (side) {
     HIPCHK(hipEventRecord(ctx->fork, st));
     HIPCHK(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
     if (ev) HIPCHK(hipEventRecord(ev[1], ctx->side));
-    if (split)
-      hipLaunchKernelGGL((k_small<2, true>), dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
-                         ch.d_s[0], ch.n_s[0], arena, c, v, p->part);
-    else
-      hipLaunchKernelGGL((k_small<2, false>), dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
-                         ch.d_s[0], ch.n_s[0], arena, c, v, p->part);
+    if (ch.n_s[0] > 0) {
+      if (split)
+        hipLaunchKernelGGL((k_small<2, true>), dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
+                           ch.d_s[0], ch.n_s[0], arena, c, v, p->part);
+      else
+        hipLaunchKernelGGL((k_small<2, false>), dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
+                           ch.d_s[0], ch.n_s[0], arena, c, v, p->part);
+    }
+    if (!split && ch.n_g[0] > 0)
+      hipLaunchKernelGGL((k_sigma<2, 2>), dim3(ch.n_g[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
+                         ch.d_g[0], ch.n_g[0], arena, c, v, p->part);
     if (ev) HIPCHK(hipEventRecord(ev[2], ctx->side));
     HIPCHK(hipEventRecord(ctx->join, ctx->side));
   }
-  if (ch.n_g > 0) {
-    if (split)
-      hipLaunchKernelGGL(k_sigma_bf, dim3(ch.n_g), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_ps, ch.d_seg, ch.d_g,
-                         ch.n_g, p->d_abs, p->d_cs, c, v, p->part);
-    else
-      hipLaunchKernelGGL(k_sigma<false>, dim3(ch.n_g), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g, ch.n_g,
-                         arena, c, v, p->part);
+  if (split) {
+    if (ch.n_gbf > 0)
+      hipLaunchKernelGGL(k_sigma_bf, dim3(ch.n_gbf), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_ps, ch.d_seg, ch.d_gbf,
+                         ch.n_gbf, p->d_abs, p->d_cs, c, v, p->part);
+  } else {
+    if (ch.n_g[2] > 0)
+      hipLaunchKernelGGL((k_sigma<1, 6>), dim3(ch.n_g[2]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g[2],
+                         ch.n_g[2], arena, c, v, p->part);
+    if (ch.n_g[1] > 0)
+      hipLaunchKernelGGL((k_sigma<1, 4>), dim3(ch.n_g[1]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g[1],
+                         ch.n_g[1], arena, c, v, p->part);
   }
-  if (ch.n_s[0] > 0) HIPCHK(hipStreamWaitEvent(st, ctx->join, 0));
+  if (side) HIPCHK(hipStreamWaitEvent(st, ctx->join, 0));
   if (ev) HIPCHK(hipEventRecord(ev[3], st));
-  if (ch.n_red > 0) hipLaunchKernelGGL(k_reduce, dim3(ch.n_red), dim3(256), 0, st, ch.d_red, p->part, v);
+  if (ch.n_red[split] > 0)
+    hipLaunchKernelGGL(k_reduce, dim3(ch.n_red[split]), dim3(256), 0, st, ch.d_red[split], p->part, v);
   if (ev) HIPCHK(hipEventRecord(ev[4], st));
   HIPCHK(hipGetLastError());
   return BSG_OK;
@@ -1791,11 +1811,11 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
   size_t k = 1;
   for (int r = 0; r < reps; ++r)
     for (auto& ch : p->chunks) {
-      const bool fused = ch.n_g > 0 || ch.n_s[0] > 0;
-      // [0] fork to join of both fused kernels, [1] reduction, [2] the
-      // small-block kernel alone
+      const bool sidek = ch.n_s[0] > 0 || (!split && ch.n_g[0] > 0);
+      // [0] fork to join of all fused kernels, [1] reduction, [2] the side
+      // stream's kernels alone
       const int from[3] = {0, 3, 1}, to[3] = {3, 4, 2};
-      const bool present[3] = {fused, ch.n_red > 0, ch.n_s[0] > 0};
+      const bool present[3] = {true, ch.n_red[split] > 0, sidek};
       for (int i = 0; i < 3; ++i) {
         if (!present[i]) continue;
         HIPCHK(hipEventElapsedTime(&ms, ev[k + from[i]], ev[k + to[i]]));
