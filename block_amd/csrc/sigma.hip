@@ -514,6 +514,9 @@ constexpr int SAP = 40;                 // LDS pitch (BF16) of a staged op(A) ro
 constexpr int SAB = 3 * GTM * SAP;      // one staged op(A) chunk, 3 slices
 constexpr int SWMAX = 3 * 64 * 72;      // W chunk slices: max over geometries of 3 TM (JC + 8)
 
+// NRM, NV: as for the FP64 kernel (row blocks per wave the accumulators
+// hold, V column blocks per wave); the widest split tiles are 256 columns.
+template <int NRM, int NV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sigma_bf(
     const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const PairS* __restrict__ pss,
     const Seg* __restrict__ segs, const TileRef* __restrict__ tiles, int ntiles, const __bf16* __restrict__ slices,
@@ -529,10 +532,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   if (tr.item < 0) return;
   const Seg sg = segs[tr.item];
   const bsg_vblock_t vb = vbs[sg.vb];
-  const int lwr = tr.pad & 3, nr = ((tr.pad >> 2) & 1) + 1;
+  static_assert(NV % PB == 0, "V tile width is a whole number of W chunks");
+  const int lwr = tr.pad & 3, nr = min(((tr.pad >> 2) & 1) + 1, NRM);
   const int WR = 1 << lwr, WC = 4 >> lwr;
-  const int TM = 16 * nr * WR, JC = 16 * PB * WC, WP = JC + 8;
-  const int row0 = tr.tm * TM, colg = tr.tn * JC;
+  const int TM = 16 * nr * WR, JC = 16 * PB * WC, JV = 16 * NV * WC, WP = JC + 8;
+  const int row0 = tr.tm * TM, colg = tr.tn * JV;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wr = w / WC, wc = w % WC;
@@ -541,24 +545,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   int nrw = (vb.m - row0 - arow + 15) / 16;
   nrw = nrw < 0 ? 0 : (nrw > nr ? nr : nrw);
   int nvb = 0;
-  for (int t = 0; t < PB; ++t)
+  for (int t = 0; t < NV; ++t)
     if (colg + (wc + WC * t) * 16 < vb.n) nvb = t + 1;
   if (nrw == 0) nvb = 0;
-  dbl4 accV[2][PB];
+  dbl4 accV[NRM][NV];
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
+  for (int r = 0; r < NRM; ++r)
 #pragma unroll
-    for (int t = 0; t < PB; ++t) accV[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < NV; ++t) accV[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
   // op(A) staging: piece u (8 BF16) of a chunk is slice u / (4 TM), row
-  // (u / 4) % TM, K quarter u % 4; three pieces per thread at most
-  constexpr int SPT = 3 * GTM * 4 / 256;
+  // (u / 4) % TM, K quarter u % 4; three pieces per thread at most (one on
+  // the wide tiles, which are 16 rows)
+  constexpr int SPT = ((NV > PB ? 16 : GTM) * 12 + 255) / 256;
   int abuf = 0;
   for (int q = 0; q < sg.count; ++q) {
     const bsg_pair_t pr = prs[sg.first + q];
     const PairS ps = pss[sg.first + q];
     const int k1p = (pr.k1 + 7) & ~7, k2p = (pr.k2 + 7) & ~7;
+    if (pr.a_off < 0 && pr.b_off < 0) {  // V += alpha C
+      const __amdgpu_buffer_rsrc_t rC = block_rsrc(c + pr.c_off, (int64_t)pr.k1 * pr.k2);
+#pragma unroll
+      for (int r = 0; r < NRM; ++r)
+#pragma unroll
+        for (int t = 0; t < NV; ++t) {
+          const int col = colg + (wc + WC * t) * 16 + li;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = row0 + arow + r * 16 + 4 * lk + i;
+            const bool ok = r < nr && row < vb.m && col < vb.n;
+            accV[r][t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
+          }
+        }
+      continue;
+    }
     for (int j0 = 0; j0 < pr.k2; j0 += JC) {
-      if (pr.b_off < 0 && j0 != colg) continue;
+      // identity B': only the W chunks of this tile's V columns
+      if (pr.b_off < 0 && (j0 < colg || j0 >= colg + JV)) continue;
       const int jw = min(JC, pr.k2 - j0);
       int nwb = 0;
       for (int t = 0; t < PB; ++t)
@@ -566,21 +588,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       if (nrw == 0) nwb = 0;
       if (pr.a_off < 0) {
         const __amdgpu_buffer_rsrc_t rC = block_rsrc(c + pr.c_off, (int64_t)pr.k1 * pr.k2);
-        if (pr.b_off < 0) {  // V += alpha C
-#pragma unroll
-          for (int r = 0; r < 2; ++r)
-#pragma unroll
-            for (int t = 0; t < PB; ++t) {
-              const int col = colg + (wc + WC * t) * 16 + li;
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                const int row = row0 + arow + r * 16 + 4 * lk + i;
-                const bool ok = r < nr && row < vb.m && col < vb.n;
-                accV[r][t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
-              }
-            }
-          continue;
-        }
         // W = alpha C (panel rows), split into the W slices
         for (int e = threadIdx.x; e < TM * JC; e += 256) {
           const int j = e % JC, i = e / JC;
@@ -596,9 +603,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const __amdgpu_buffer_rsrc_t rA = slice_rsrc(slices + ps.a, 3 * (int64_t)vb.m * k1p);
         const __amdgpu_buffer_rsrc_t rC = slice_rsrc(cslices + ps.c, 3 * (int64_t)pr.k2 * k1p);
         const uint32_t aslice = (uint32_t)vb.m * k1p * 2u, cslice = (uint32_t)pr.k2 * k1p * 2u;  // bytes per slice
-        dbl4 accW[2][PB];
+        dbl4 accW[NRM][PB];
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < NRM; ++r)
 #pragma unroll
           for (int t = 0; t < PB; ++t) accW[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
         uint32_t aoff[SPT];
@@ -651,9 +658,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             load_a(k0 + PKC);
             load_c(k0 + PKC);
           }
-          Frag af[2];
+          Frag af[NRM];
 #pragma unroll
-          for (int r = 0; r < 2; ++r) {
+          for (int r = 0; r < NRM; ++r) {
             if (r >= nr) {
               af[r] = af[0];
               continue;
@@ -667,7 +674,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
           for (int t = 0; t < PB; ++t)
             if (t < nwb)
 #pragma unroll
-              for (int r = 0; r < 2; ++r)
+              for (int r = 0; r < NRM; ++r)
                 if (r < nrw) {
                   if (pr.b_off < 0)
                     mma6(accW[r][t], af[r], cc[t]);  // W block: (row 4 lk + i, col li)
@@ -678,17 +685,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         __syncthreads();  // op(A) buffers done before the W slices (same LDS) are written
         const double aw = pr.alpha * pr.alpha_a;
         if (pr.b_off < 0) {
+          // identity B': the W blocks of this wave are V blocks t + ch PB
+          const int ch = (j0 - colg) / JC;
 #pragma unroll
-          for (int r = 0; r < 2; ++r)
+          for (int cc = 0; cc < NV / PB; ++cc)
+            if (cc == ch)
 #pragma unroll
-            for (int t = 0; t < PB; ++t)
+              for (int r = 0; r < NRM; ++r)
 #pragma unroll
-              for (int i = 0; i < 4; ++i) accV[r][t][i] += aw * accW[r][t][i];
+                for (int t = 0; t < PB; ++t)
+#pragma unroll
+                  for (int i = 0; i < 4; ++i) accV[r][cc * PB + t][i] += aw * accW[r][t][i];
           continue;
         }
         // lane holds W(arow + 16 r + li, 16 cb + 4 lk + i), i < 4
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < NRM; ++r)
           if (r < nr)
 #pragma unroll
             for (int t = 0; t < PB; ++t) {
@@ -708,18 +720,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       {
         const __amdgpu_buffer_rsrc_t rB = slice_rsrc(slices + ps.b, 3 * (int64_t)vb.n * k2p);
         const uint32_t bslice = (uint32_t)vb.n * k2p * 2u;
-        uint32_t boff[PB];
+        uint32_t boff[NV];
 #pragma unroll
-        for (int t = 0; t < PB; ++t) {
+        for (int t = 0; t < NV; ++t) {
           const int col = colg + (wc + WC * t) * 16 + li;
           boff[t] = (t < nvb && col < vb.n) ? (uint32_t)(col * k2p + 8 * lk) * 2u : OOB;
         }
-        Frag bn[PB];
+        Frag bn[NV];
         auto load_b = [&](int kg) {
           const bool in = j0 + kg + 8 * lk < k2p;
           const uint32_t so = (uint32_t)(j0 + kg) * 2u;
 #pragma unroll
-          for (int t = 0; t < PB; ++t) {
+          for (int t = 0; t < NV; ++t) {
             const uint32_t o = in ? boff[t] : OOB;
             bn[t].h = bload8(rB, o, so);
             bn[t].m = bload8(rB, o, bslice + so);
@@ -728,13 +740,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         };
         load_b(0);
         for (int kg = 0; kg < jw; kg += 32) {
-          Frag bc[PB];
+          Frag bc[NV];
 #pragma unroll
-          for (int t = 0; t < PB; ++t) bc[t] = bn[t];
+          for (int t = 0; t < NV; ++t) bc[t] = bn[t];
           if (kg + 32 < jw) load_b(kg + 32);
-          Frag af[2];
+          Frag af[NRM];
 #pragma unroll
-          for (int r = 0; r < 2; ++r) {
+          for (int r = 0; r < NRM; ++r) {
             if (r >= nr) {
               af[r] = af[0];
               continue;
@@ -745,10 +757,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             af[r].l = *(const bf16x8*)(lds + 2 * TM * WP + o);
           }
 #pragma unroll
-          for (int r = 0; r < 2; ++r)
+          for (int r = 0; r < NRM; ++r)
             if (r < nrw)
 #pragma unroll
-              for (int t = 0; t < PB; ++t)
+              for (int t = 0; t < NV; ++t)
                 if (t < nvb) mma6(accV[r][t], af[r], bc[t]);
         }
       }
@@ -758,9 +770,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const bool direct = sg.p_off < 0;
   double* out = direct ? v + vb.v_off : part + sg.p_off;
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
+  for (int r = 0; r < NRM; ++r)
 #pragma unroll
-    for (int t = 0; t < PB; ++t) {
+    for (int t = 0; t < NV; ++t) {
       const int col = colg + (wc + WC * t) * 16 + li;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -1013,16 +1025,13 @@ inline Geom make_geom(int lwr, int nr, int nv) {
   g.cls = nv / 2 - 1;
   return g;
 }
-inline Geom pick_geom(int m, int n, bool wide) {
+inline Geom pick_geom(int m, int n, int max_nv) {
   // typical contraction depth: the block's own dimensions
   const long k = std::max(m, n);
   std::vector<Geom> cand;
   for (int lwr = 0; lwr <= 2; ++lwr)
     for (int nr = 1; nr <= 2; ++nr) cand.push_back(make_geom(lwr, nr, PB));
-  if (wide) {
-    cand.push_back(make_geom(0, 1, 4));
-    cand.push_back(make_geom(0, 1, 6));
-  }
+  for (int nv = 4; nv <= max_nv; nv += 2) cand.push_back(make_geom(0, 1, nv));
   Geom best = cand[0];
   long bc = -1;
   for (const Geom& g : cand) {
@@ -1253,8 +1262,8 @@ struct PlanChunk {
   Seg* d_seg = nullptr;
   TileRef* d_g[NCLS] = {nullptr, nullptr, nullptr};  // FP64 grouped-kernel tiles per kernel class
   int n_g[NCLS] = {0, 0, 0};
-  TileRef* d_gbf = nullptr;  // the same units as basic tiles, for the split-precision kernel
-  int n_gbf = 0;
+  TileRef* d_gbf[2] = {nullptr, nullptr};  // split-precision tiles (basic, 256 wide)
+  int n_gbf[2] = {0, 0};
   int4* d_s[2] = {nullptr, nullptr};  // FP64 small-block units, 4 per workgroup (SB = 2, 3)
   int n_s[2] = {0, 0};
   RedItem* d_red[2] = {nullptr, nullptr};  // after the FP64 / the split-precision kernels
@@ -1438,7 +1447,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
   for (int ci = 0; ci < n_chunks; ++ci) {
     const bsg_chunk_t& ch = chunks[ci];
     PlanChunk pc;
-    std::vector<GUnit> gunits[NCLS], gunits_bf;  // the tiles of each work unit
+    std::vector<GUnit> gunits[NCLS], gunits_bf[2];  // the tiles of each work unit (FP64, split)
     std::vector<bsg_pair_t> perm;  // pairs in device order (grouped per V block as given)
     std::vector<PairS> pslices;    // their slice offsets
     std::vector<std::pair<long, int>> sunits[2];  // (cost, segment) of small-block units
@@ -1450,7 +1459,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       if (vb.m <= 0 || vb.n <= 0 || vb.v_off < 0 || vb.v_off + (int64_t)vb.m * vb.n > wave_size || vb.first < 0 ||
           vb.count < 0 || vb.first + vb.count > ch.n_pairs)
         return cleanup(BSG_ERR_ARG, "bsg_plan_create: vblock out of range");
-      const Geom gm = pick_geom(vb.m, vb.n, true);
+      const Geom gm = pick_geom(vb.m, vb.n, 6);     // FP64 tiles
+      const Geom gs = pick_geom(vb.m, vb.n, 4);     // split-precision tiles
       // small blocks (up to 32 x 32) go to the per-wave kernel, except pairs
       // whose C block is larger than that (or whose A is the identity and B
       // is not): those run in the grouped kernel with its K pipeline
@@ -1538,14 +1548,12 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           }
           long ucost = 0;
           for (int q = 0; q < sg.count; ++q) ucost += pair_cost(g, vb.n, perm[sg.first + q]);
-          GUnit gu, gb;
-          gu.cost = gb.cost = ucost;
-          for (int tm = 0; tm < (vb.m + g.tm - 1) / g.tm; ++tm) {
+          GUnit gu;
+          gu.cost = ucost;
+          for (int tm = 0; tm < (vb.m + g.tm - 1) / g.tm; ++tm)
             for (int tn = 0; tn < (vb.n + g.jv - 1) / g.jv; ++tn) gu.tiles.push_back({si, tm, tn, g.code});
-            for (int tn = 0; tn < (vb.n + g.jc - 1) / g.jc; ++tn) gb.tiles.push_back({si, tm, tn, g.code});
-          }
-          if (mode != 2) gunits[g.cls].push_back(std::move(gu));
-          if (mode != 1) gunits_bf.push_back(std::move(gb));
+          if (mode != 2) gunits[g.cls].push_back(gu);
+          if (mode != 1) gunits_bf[g.cls].push_back(gu);
         }
         if (nseg > 1) {
           for (int64_t e0 = 0; e0 < mn; e0 += 256) {
@@ -1562,11 +1570,11 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           poff += nseg * mn;
         }
       };
-      if (gm.nv == PB) {
+      if (gm.code == gs.code && gm.nv == gs.nv) {
         emit(gm, 0);
       } else {
         emit(gm, 1);
-        emit(pick_geom(vb.m, vb.n, false), 2);
+        emit(gs, 2);
       }
     }
     pmax = std::max(pmax, poff);
@@ -1589,7 +1597,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     }
     std::vector<TileRef> gtiles[NCLS];
     for (int k = 0; k < NCLS; ++k) gtiles[k] = xcd_order(gunits[k]);
-    std::vector<TileRef> gtiles_bf = xcd_order(gunits_bf);
+    std::vector<TileRef> gtiles_bf[2] = {xcd_order(gunits_bf[0]), xcd_order(gunits_bf[1])};
     // small-block units, costliest first, four per workgroup
     std::vector<int4> sq[2];
     for (int k = 0; k < 2; ++k) {
@@ -1612,9 +1620,9 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       rc = up((void**)&pc.d_g[k], gtiles[k].data(), sizeof(TileRef) * gtiles[k].size());
       pc.n_g[k] = (int)gtiles[k].size();
     }
-    if (!rc) {
-      rc = up((void**)&pc.d_gbf, gtiles_bf.data(), sizeof(TileRef) * gtiles_bf.size());
-      pc.n_gbf = (int)gtiles_bf.size();
+    for (int k = 0; k < 2 && !rc; ++k) {
+      rc = up((void**)&pc.d_gbf[k], gtiles_bf[k].data(), sizeof(TileRef) * gtiles_bf[k].size());
+      pc.n_gbf[k] = (int)gtiles_bf[k].size();
     }
     if (rc) {
       p->chunks.push_back(pc);
@@ -1651,7 +1659,8 @@ int bsg_plan_destroy(bsg_plan* p) {
     (void)hipFree(c.d_red[0]);
     (void)hipFree(c.d_red[1]);
     for (int k = 0; k < NCLS; ++k) (void)hipFree(c.d_g[k]);
-    (void)hipFree(c.d_gbf);
+    (void)hipFree(c.d_gbf[0]);
+    (void)hipFree(c.d_gbf[1]);
     (void)hipFree(c.d_s[0]);
     (void)hipFree(c.d_s[1]);
   }
@@ -1706,10 +1715,11 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
                   double* v, hipEvent_t* ev, bool split) {
   hipStream_t st = ctx->stream;
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
-  // the side stream runs the small-block kernel (and, in FP64, the basic
-  // tiles) beside the wide tiles on the context stream; they write disjoint
-  // V blocks and partial tiles, and the reduction waits for both
-  const bool side = ch.n_s[0] > 0 || (!split && ch.n_g[0] > 0);
+  // the side stream runs the small-block kernel and the basic tiles beside
+  // the wide tiles on the context stream; they write disjoint V blocks and
+  // partial tiles, and the reduction waits for both
+  const int nbasic = split ? ch.n_gbf[0] : ch.n_g[0];
+  const bool side = ch.n_s[0] > 0 || nbasic > 0;
   if (side) {
     HIPCHK(hipEventRecord(ctx->fork, st));
     HIPCHK(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
@@ -1722,16 +1732,21 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
         hipLaunchKernelGGL((k_small<2, false>), dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
                            ch.d_s[0], ch.n_s[0], arena, c, v, p->part);
     }
-    if (!split && ch.n_g[0] > 0)
-      hipLaunchKernelGGL((k_sigma<2, 2>), dim3(ch.n_g[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
-                         ch.d_g[0], ch.n_g[0], arena, c, v, p->part);
+    if (nbasic > 0) {
+      if (split)
+        hipLaunchKernelGGL((k_sigma_bf<2, 2>), dim3(nbasic), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_ps,
+                           ch.d_seg, ch.d_gbf[0], nbasic, p->d_abs, p->d_cs, c, v, p->part);
+      else
+        hipLaunchKernelGGL((k_sigma<2, 2>), dim3(nbasic), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
+                           ch.d_g[0], nbasic, arena, c, v, p->part);
+    }
     if (ev) HIPCHK(hipEventRecord(ev[2], ctx->side));
     HIPCHK(hipEventRecord(ctx->join, ctx->side));
   }
   if (split) {
-    if (ch.n_gbf > 0)
-      hipLaunchKernelGGL(k_sigma_bf, dim3(ch.n_gbf), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_ps, ch.d_seg, ch.d_gbf,
-                         ch.n_gbf, p->d_abs, p->d_cs, c, v, p->part);
+    if (ch.n_gbf[1] > 0)
+      hipLaunchKernelGGL((k_sigma_bf<1, 4>), dim3(ch.n_gbf[1]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_ps, ch.d_seg,
+                         ch.d_gbf[1], ch.n_gbf[1], p->d_abs, p->d_cs, c, v, p->part);
   } else {
     if (ch.n_g[2] > 0)
       hipLaunchKernelGGL((k_sigma<1, 6>), dim3(ch.n_g[2]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g[2],
@@ -1811,7 +1826,7 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
   size_t k = 1;
   for (int r = 0; r < reps; ++r)
     for (auto& ch : p->chunks) {
-      const bool sidek = ch.n_s[0] > 0 || (!split && ch.n_g[0] > 0);
+      const bool sidek = ch.n_s[0] > 0 || (split ? ch.n_gbf[0] : ch.n_g[0]) > 0;
       // [0] fork to join of all fused kernels, [1] reduction, [2] the side
       // stream's kernels alone
       const int from[3] = {0, 3, 1}, to[3] = {3, 4, 2};
