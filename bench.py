@@ -164,12 +164,23 @@ def summarize_trace(d, last, pattern):
     kernels on smaller superblocks)."""
     print("| kernel | dispatches | average ms | min ms | max ms | grid | workgroup | VGPR | AGPR | SGPR | LDS B | scratch B |")
     print("|---|---|---|---|---|---|---|---|---|---|---|---|")
-    for k, rs in _last_by_kernel(_csv_rows(d, "kernel_trace.csv"), last, pattern).items():
+    by = _last_by_kernel(_csv_rows(d, "kernel_trace.csv"), last, pattern)
+    for k, rs in by.items():
         t = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in rs]
         r = rs[-1]
         print(f"| {k} | {len(t)} | {sum(t) / len(t):.4f} | {min(t):.4f} | {max(t):.4f} | {r['Grid_Size_X']} | "
               f"{r['Workgroup_Size_X']} | {r['VGPR_Count']} | {r['Accum_VGPR_Count']} | {r['SGPR_Count']} | "
               f"{r['LDS_Block_Size']} | {r['Scratch_Size']} |")
+    # the fused kernels of one H.C run concurrently on two streams: the span
+    # from the first start to the last end of the i-th last dispatch of each
+    for label, names in (("FP64", ("k_small<2, false>", "k_sigma<")), ("split", ("k_small<2, true>", "k_sigma_bf"))):
+        groups = [rs for k, rs in by.items() if any(n in k for n in names)]
+        if not groups or min(len(g) for g in groups) == 0:
+            continue
+        n = min(len(g) for g in groups)
+        spans = [(max(int(g[-1 - i]["End_Timestamp"]) for g in groups) -
+                  min(int(g[-1 - i]["Start_Timestamp"]) for g in groups)) / 1e6 for i in range(n)]
+        print(f"\n{label} fused kernels, first start to last end per call: average {sum(spans) / n:.4f} ms over {n} calls")
 
 
 def summarize_counters(d, last, pattern):
