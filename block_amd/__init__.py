@@ -69,6 +69,8 @@ _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
      [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp])
 
+_sig(_h, "blkh_rotation_check", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _lp])
+
 FP64, SPLIT_BF16 = 0, 1
 SWEEP_COLS = ("sweep", "forward", "M", "E0", "discarded", "seconds", "sigma_seconds", "sigma_flops",
               "sigma_calls", "split_calls", "build_seconds", "davidson_seconds", "renorm_seconds", "guessed_steps")
@@ -246,3 +248,15 @@ def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=
     if n < 0:
         raise _err()
     return e, sw[: nc * n].reshape(n, nc)
+
+
+def rotation_check(ints, nsys, M, device=0):
+    """Renormalise a Wilson-grown block of nsys sites plus one dot with a random
+    density matrix truncated to M, with the host rotations and with the GPU
+    rotator (transform_operators, save_load_block.C:267-319); returns (largest
+    relative difference over all rotated operators, operators compared)."""
+    worst = ctypes.c_double()
+    n = ctypes.c_long()
+    if host_lib.blkh_rotation_check(ints._h, nsys, M, device, ctypes.byref(worst), ctypes.byref(n)) != 0:
+        raise _err()
+    return worst.value, n.value

@@ -2,7 +2,9 @@
 // Python layer: integrals, superblock snapshots, the GPU multiply_h and the
 // sweep driver running its Davidson sigma on the GPU.  Errors are reported as
 // negative return codes / null handles with blkh_last_error() holding the text.
+#include <cmath>
 #include <cstring>
+#include <random>
 #include <string>
 
 #include "gpu_sigma.h"
@@ -93,6 +95,7 @@ void* blkh_snapshot_dmrg(void* ints, int nelec, int twoS, int irrep, int nroots,
     P.schedule = {{0, M, 1e-7, 0.0}};  // noise-free, as in converged sweeps
     P.verbose = verbose;
     GpuContext ctx(device);
+    P.rotator = std::make_shared<GpuRotator>(ctx);
     SuperBlock sb = dmrg_snapshot(I, P, [&](const SuperBlock& s) {
       return std::unique_ptr<DavidsonFunctor>(new GpuFunctor(ctx, s));
     });
@@ -209,6 +212,7 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
     P.ladder_from_sweep = ladder_from;
     P.transform_guess = guess != 0;
     GpuContext ctx(device);
+    P.rotator = std::make_shared<GpuRotator>(ctx);
     auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {
       return std::unique_ptr<DavidsonFunctor>(new GpuFunctor(ctx, sb));
     });
@@ -221,6 +225,66 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
 }  // extern "C"
 
 extern "C" {
+// Renormalisation of a Wilson-grown block sys(nsys sites) (x) dot with a
+// random density matrix truncated to M, once with the host rotations and
+// once with the GPU rotator; returns the largest relative difference of any
+// rotated operator (stored and complementary) and the number compared.
+int blkh_rotation_check(void* ints, int nsys, int M, int device, double* maxrel, long* nops) {
+  return guarded([&]() {
+    const Integrals& I = *static_cast<Integrals*>(ints);
+    Screen scr;
+    const Quantum target{I.nelec, 0, I.isym};
+    std::vector<int> order(I.norb);
+    for (int i = 0; i < I.norb; ++i) order[i] = i;
+    BlockPtr sys = warmup_block(I, order, nsys, target, M, true, scr);
+    BlockPtr dot = make_dot(I, nsys, scr);
+    BuildSpec sl;
+    sl.normal = true;
+    sl.comp = false;
+    BlockPtr L = build_sum_block(I, *sys, *dot, target, I.norb - nsys - 1, sl, scr);
+    std::mt19937_64 rng(7);
+    std::normal_distribution<double> gauss(0.0, 1.0);
+    std::vector<std::vector<double>> rho(L->si->nsec());
+    for (int q = 0; q < L->si->nsec(); ++q) {
+      const int d = L->si->dim[q];
+      std::vector<double> G((size_t)d * d);
+      for (auto& x : G) x = gauss(rng);
+      rho[q].assign((size_t)d * d, 0.0);
+      dgemm(false, true, d, d, d, 1.0, G.data(), d, G.data(), d, 0.0, rho[q].data(), d);
+    }
+    std::vector<std::vector<double>> rot;
+    std::vector<int> mkeep;
+    make_rotation(*L->si, rho, M, rot, mkeep);
+    Rotation R = make_rotation_info(*L->si, rot, mkeep);
+    GpuContext ctx(device);
+    GpuRotator gpu(ctx);
+    BlockPtr a = renormalise(*L, R, nullptr);
+    add_rotated_comp_ops(I, *L, *sys, *dot, scr, R, *a, nullptr);
+    BlockPtr b = renormalise(*L, R, &gpu);
+    add_rotated_comp_ops(I, *L, *sys, *dot, scr, R, *b, &gpu);
+    double worst = 0.0;
+    long n = 0;
+    auto cmp = [&](const OpPtr& x, const OpPtr& y) {
+      if (!x && !y) return;
+      if (!x || !y || x->data.size() != y->data.size()) throw std::runtime_error("rotation check: operator sets differ");
+      double num = 0.0, den = 0.0;
+      for (size_t e = 0; e < x->data.size(); ++e) {
+        num += (x->data[e] - y->data[e]) * (x->data[e] - y->data[e]);
+        den += x->data[e] * x->data[e];
+      }
+      if (den > 0.0) worst = std::max(worst, std::sqrt(num / den));
+      ++n;
+    };
+    cmp(a->ident, b->ident);
+    cmp(a->ham, b->ham);
+    for (auto m : {&SpinBlock::cre, &SpinBlock::ccd, &SpinBlock::cd, &SpinBlock::cc, &SpinBlock::cdc, &SpinBlock::ddc})
+      for (size_t i = 0; i < ((*a).*m).size(); ++i) cmp(((*a).*m)[i], ((*b).*m)[i]);
+    *maxrel = worst;
+    *nops = n;
+    return 0;
+  });
+}
+
 // SU(2) coefficients (doubled spins), exposed for the closed-form checks
 double blkh_cg(int j1, int m1, int j2, int m2, int J, int M) { return cg(j1, m1, j2, m2, J, M); }
 double blkh_sixj(int a, int b, int c, int d, int e, int f) { return sixj(a, b, c, d, e, f); }

@@ -14,11 +14,67 @@ GpuContext::GpuContext(int device) { bsg_check(bsg_init(&ctx, device), "bsg_init
 GpuContext::~GpuContext() {
   if (!ctx) return;
   if (arena) bsg_mfree(ctx, arena);
+  if (work) bsg_mfree(ctx, work);
   if (staging) bsg_host_free(ctx, staging);
   bsg_free(ctx);
 }
+double* GpuContext::ensure_staging(size_t n) {
+  if (staging_cap < n) {
+    if (staging) bsg_check(bsg_host_free(ctx, staging), "staging free");
+    staging = nullptr;
+    staging_cap = 0;
+    const size_t cap = n + n / 4;
+    bsg_check(bsg_host_alloc(ctx, sizeof(double) * cap, (void**)&staging), "staging");
+    staging_cap = cap;
+  }
+  return staging;
+}
+double* GpuContext::ensure_work(size_t n) {
+  if (work_cap < n) {
+    if (work) bsg_check(bsg_mfree(ctx, work), "work free");
+    work = nullptr;
+    work_cap = 0;
+    const size_t cap = n + n / 4;
+    bsg_check(bsg_malloc(ctx, sizeof(double) * cap, (void**)&work), "rotation scratch");
+    work_cap = cap;
+  }
+  return work;
+}
 
 namespace {
+// Upload operators to dst + offset: copied into page-locked staging by all
+// threads in batches of about 64 MB, each batch sent while the next one is
+// copied; device memory between the operators is left as it is.  n: doubles
+// spanned by the offsets.
+void upload_ops(GpuContext& g, const std::vector<std::pair<const SparseOp*, int64_t>>& ops, double* dst, size_t n) {
+  double* staging = g.ensure_staging(std::max<size_t>(1, n));
+  struct Piece {
+    const double* src;
+    int64_t off, n;
+  };
+  std::vector<Piece> pieces;
+  constexpr int64_t PIECE = 1 << 19;  // doubles (4 MB)
+  for (const auto& kv : ops)
+    for (int64_t x = 0; x < (int64_t)kv.first->data.size(); x += PIECE)
+      pieces.push_back({kv.first->data.data() + x, kv.second + x, std::min<int64_t>(PIECE, kv.first->data.size() - x)});
+  std::sort(pieces.begin(), pieces.end(), [](const Piece& a, const Piece& b) { return a.off < b.off; });
+  constexpr int64_t BATCH = 1 << 23;  // doubles (64 MB)
+  for (size_t p0 = 0; p0 < pieces.size();) {
+    // a batch is a gap-free run of pieces: the range sent is exactly theirs
+    size_t p1 = p0;
+    int64_t len = 0;
+    while (p1 < pieces.size() && (len == 0 || (len + pieces[p1].n <= BATCH && pieces[p1].off == pieces[p0].off + len)))
+      len += pieces[p1++].n;
+#pragma omp parallel for schedule(dynamic)
+    for (long i = (long)p0; i < (long)p1; ++i)
+      std::memcpy(staging + pieces[i].off, pieces[i].src, sizeof(double) * pieces[i].n);
+    const int64_t b0 = pieces[p0].off, b1 = pieces[p1 - 1].off + pieces[p1 - 1].n;
+    bsg_check(bsg_memcpy_h2d_async(g.ctx, dst + b0, staging + b0, sizeof(double) * (b1 - b0)), "operator upload");
+    p0 = p1;
+  }
+  bsg_check(bsg_synchronize(g.ctx), "operator upload");
+}
+
 SetupTimes g_setup;
 double secs() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 }  // namespace
@@ -146,39 +202,10 @@ GpuSigma::GpuSigma(GpuContext& g, const SuperBlock& sb, int max_vec) : g_(g) {
     bsg_check(bsg_malloc(g_.ctx, sizeof(double) * need, (void**)&d_arena_), "arena");
     owns_arena_ = true;
   }
-  // upload: operators copied into page-locked staging by all threads in
-  // batches of about 64 MB, each batch sent while the next one is copied
-  if (g_.staging_cap < need) {
-    if (g_.staging) bsg_check(bsg_host_free(g_.ctx, g_.staging), "staging free");
-    g_.staging = nullptr;
-    g_.staging_cap = 0;
-    const size_t cap = need + need / 4;
-    bsg_check(bsg_host_alloc(g_.ctx, sizeof(double) * cap, (void**)&g_.staging), "staging");
-    g_.staging_cap = cap;
-  }
-  struct Piece {
-    const double* src;
-    int64_t off, n;
-  };
-  std::vector<Piece> pieces;
-  constexpr int64_t PIECE = 1 << 19;  // doubles (4 MB)
-  for (const auto& kv : wl.arena_off)
-    for (int64_t x = 0; x < (int64_t)kv.first->data.size(); x += PIECE)
-      pieces.push_back({kv.first->data.data() + x, kv.second + x, std::min<int64_t>(PIECE, kv.first->data.size() - x)});
-  std::sort(pieces.begin(), pieces.end(), [](const Piece& a, const Piece& b) { return a.off < b.off; });
-  constexpr int64_t BATCH = 1 << 23;  // doubles (64 MB)
-  for (size_t p0 = 0; p0 < pieces.size();) {
-    size_t p1 = p0;
-    int64_t len = 0;
-    while (p1 < pieces.size() && (len == 0 || len + pieces[p1].n <= BATCH)) len += pieces[p1++].n;
-#pragma omp parallel for schedule(dynamic)
-    for (long i = (long)p0; i < (long)p1; ++i)
-      std::memcpy(g_.staging + pieces[i].off, pieces[i].src, sizeof(double) * pieces[i].n);
-    const int64_t b0 = pieces[p0].off, b1 = pieces[p1 - 1].off + pieces[p1 - 1].n;
-    bsg_check(bsg_memcpy_h2d_async(g_.ctx, d_arena_ + b0, g_.staging + b0, sizeof(double) * (b1 - b0)), "arena upload");
-    p0 = p1;
-  }
-  bsg_check(bsg_synchronize(g_.ctx), "arena upload");
+  std::vector<std::pair<const SparseOp*, int64_t>> ops(wl.arena_off.begin(), wl.arena_off.end());
+  g_.resident.clear();
+  upload_ops(g_, ops, d_arena_, (size_t)arena_size_);
+  if (!owns_arena_) g_.resident = wl.arena_off;
   double t2 = secs();
   size_t wbytes = sizeof(double) * std::max<long>(1, wave_size_) * std::max(1, max_vec);
   bsg_check(bsg_malloc(g_.ctx, wbytes, (void**)&d_c_), "c buffer");
@@ -235,6 +262,100 @@ std::vector<double> GpuSigma::diagonal() {
   bsg_check(bsg_diag_apply(g_.ctx, diag_terms_.data(), (int32_t)diag_terms_.size(), d_arena_, d_v_), "diag");
   std::vector<double> out(wave_size_);
   bsg_check(bsg_memcpy_d2h(g_.ctx, out.data(), d_v_, bytes), "diag download");
+  return out;
+}
+
+std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, const Rotation& R) const {
+  const StateInfo& os = *R.old;
+  const auto& rot = *R.rot;
+  // rotation matrices (the plan's operator arena): d_q x m_q row-major per old sector
+  std::vector<int64_t> uoff(os.nsec(), -1);
+  int64_t usize = 0;
+  for (int q = 0; q < os.nsec(); ++q)
+    if (!rot[q].empty()) {
+      uoff[q] = usize;
+      usize += (int64_t)rot[q].size();
+    }
+  // one device buffer: old operators, then the rotated ones, then the matrices
+  std::vector<OpPtr> out(ops.size());
+  std::vector<std::pair<const SparseOp*, int64_t>> src;
+  int64_t soff = 0;
+  for (const SparseOp* o : ops) {
+    src.push_back({o, soff});
+    soff += (int64_t)o->data.size();
+  }
+  std::vector<int64_t> ooff(ops.size());
+  int64_t osize = 0;
+  for (size_t w = 0; w < ops.size(); ++w) {
+    const SparseOp& o = *ops[w];
+    out[w] = std::make_shared<SparseOp>(get_pattern(R.ns, o.dn(), o.k2(), o.dirrep()), o.fermion);
+    ooff[w] = soff + osize;
+    osize += (int64_t)out[w]->data.size();
+  }
+  const int64_t total = soff + osize;
+  if (osize == 0) return out;
+  std::vector<bsg_vblock_t> vb;
+  std::vector<bsg_pair_t> pr;
+  for (size_t w = 0; w < ops.size(); ++w) {
+    const SparseOp& o = *ops[w];
+    const auto& P = *out[w]->pat;
+    for (const auto& bk : P.blocks) {
+      const int orow = R.oldidx[bk.r], ocol = R.oldidx[bk.c];
+      const int obi = o.pat->find(orow, ocol);
+      if (obi < 0 || bk.rows == 0 || bk.cols == 0) continue;
+      // new(r, c) = R_r^T O(orow, ocol) R_c: op(A) = R_r^T (stored d_r x m_r),
+      // C = the old block, B' = R_c (d_c x m_c, k-major)
+      bsg_pair_t p;
+      p.a_off = uoff[orow];
+      p.c_off = src[w].second + o.pat->blocks[obi].off;
+      p.b_off = uoff[ocol];
+      p.k1 = os.dim[orow];
+      p.k2 = os.dim[ocol];
+      p.a_trans = 1;
+      p.b_trans = 0;
+      p.alpha_a = 1.0;
+      p.alpha = 1.0;
+      vb.push_back({ooff[w] + bk.off, bk.rows, bk.cols, (int32_t)pr.size(), 1});
+      pr.push_back(p);
+    }
+  }
+  double* base = g_.ensure_work((size_t)(total + usize));
+  double* ubuf = base + total;
+  // operators still in the arena are copied on the device, the rest uploaded
+  std::vector<std::pair<const SparseOp*, int64_t>> host;
+  for (const auto& x : src) {
+    auto f = g_.resident.find(x.first);
+    if (f != g_.resident.end() && g_.arena && !g_.arena_leased)
+      bsg_check(bsg_copy(g_.ctx, g_.arena + f->second, base + x.second, (int64_t)x.first->data.size()), "operator copy");
+    else
+      host.push_back(x);
+  }
+  upload_ops(g_, host, base, (size_t)soff);
+  {
+    std::vector<double> u((size_t)std::max<int64_t>(1, usize));
+    for (int q = 0; q < os.nsec(); ++q)
+      if (uoff[q] >= 0) std::copy(rot[q].begin(), rot[q].end(), u.begin() + uoff[q]);
+    bsg_check(bsg_memcpy_h2d(g_.ctx, ubuf, u.data(), sizeof(double) * usize), "rotation upload");
+  }
+  bsg_check(bsg_memset_zero(g_.ctx, base + soff, sizeof(double) * osize), "rotation clear");
+  if (!pr.empty()) {
+    bsg_chunk_t chunk;
+    chunk.vblocks = vb.data();
+    chunk.n_vblocks = (int32_t)vb.size();
+    chunk.pairs = pr.data();
+    chunk.n_pairs = (int32_t)pr.size();
+    bsg_plan* plan = nullptr;
+    bsg_check(bsg_plan_create(g_.ctx, &chunk, 1, total, &plan), "rotation plan");
+    const int rc = bsg_sigma_apply(g_.ctx, plan, ubuf, base, base, 1, BSG_FP64);
+    bsg_plan_destroy(plan);
+    bsg_check(rc, "rotation");
+  }
+  // rotated operators back through the staging
+  double* staging = g_.ensure_staging((size_t)osize);
+  bsg_check(bsg_memcpy_d2h(g_.ctx, staging, base + soff, sizeof(double) * osize), "rotation download");
+#pragma omp parallel for schedule(dynamic)
+  for (long w = 0; w < (long)ops.size(); ++w)
+    std::memcpy(out[w]->data.data(), staging + (ooff[w] - soff), sizeof(double) * out[w]->data.size());
   return out;
 }
 

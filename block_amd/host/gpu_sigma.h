@@ -26,7 +26,16 @@ struct GpuContext {
   bool arena_leased = false;
   double* staging = nullptr;
   size_t staging_cap = 0;  // doubles
+  double* work = nullptr;  // scratch of the operator rotations
+  size_t work_cap = 0;     // doubles
+  // operators of the last arena upload and their offsets: valid until the
+  // next upload (the sweep rotates the superblock's operators right after
+  // its Davidson solve, while they are still in the arena)
+  std::unordered_map<const SparseOp*, int64_t> resident;
   explicit GpuContext(int device = 0);
+  // grow-only buffers (contents not kept when they grow)
+  double* ensure_staging(size_t n);
+  double* ensure_work(size_t n);
   ~GpuContext();
   GpuContext(const GpuContext&) = delete;
   GpuContext& operator=(const GpuContext&) = delete;
@@ -88,6 +97,21 @@ class GpuSigma {
   int precision_ = BSG_FP64;
   bool split_ready_ = false;
   std::vector<bsg_diag_t> diag_terms_;
+};
+
+// Operator rotations O' = R_r^T O R_c on the GPU (transform_operators,
+// save_load_block.C:267-319, through MatrixRotate, MatrixBLAS.C:553-572):
+// every new operator block is one GEMM pair of the sigma kernels, with the
+// rotation matrices as the operator arena and the old operator blocks as the
+// wavefunction.  Operators are uploaded through page-locked staging and the
+// rotated ones copied back.
+class GpuRotator : public OpRotator {
+ public:
+  explicit GpuRotator(GpuContext& g) : g_(g) {}
+  std::vector<OpPtr> rotate(const std::vector<const SparseOp*>& ops, const Rotation& R) const override;
+
+ private:
+  GpuContext& g_;
 };
 
 // Linear::block_davidson on device vectors with the sigma of h (same control

@@ -682,7 +682,23 @@ BlockPtr renormalise(const SpinBlock& B, const std::vector<std::vector<double>>&
   return renormalise(B, make_rotation_info(*B.si, rot, mkeep));
 }
 
-BlockPtr renormalise(const SpinBlock& B, const Rotation& R) {
+std::vector<OpPtr> OpRotator::rotate(const std::vector<const SparseOp*>& ops, const Rotation& R) const {
+  std::vector<OpPtr> out(ops.size());
+  std::string err;
+#pragma omp parallel for schedule(dynamic)
+  for (long w = 0; w < (long)ops.size(); ++w) {
+    try {
+      out[w] = rotate_op(*ops[w], R);
+    } catch (const std::exception& e) {
+#pragma omp critical(blk_task_err)
+      if (err.empty()) err = e.what();
+    }
+  }
+  if (!err.empty()) throw std::runtime_error(err);
+  return out;
+}
+
+BlockPtr renormalise(const SpinBlock& B, const Rotation& R, const OpRotator* rotator) {
   if (R.old != B.si.get()) throw std::runtime_error("renormalise: rotation built for another block");
   auto nb = std::make_shared<SpinBlock>();
   nb->norb = B.norb;
@@ -691,39 +707,52 @@ BlockPtr renormalise(const SpinBlock& B, const Rotation& R) {
   nb->si = R.ns;
   nb->has_normal = B.has_normal;
   nb->has_comp = B.has_comp;
-  nb->ident = rotate_op(*B.ident, R);
-  nb->ham = rotate_op(*B.ham, R);
   std::vector<std::pair<const std::vector<OpPtr>*, std::vector<OpPtr>*>> lists = {
       {&B.cre, &nb->cre}, {&B.ccd, &nb->ccd}, {&B.cd, &nb->cd}, {&B.cc, &nb->cc}, {&B.cdc, &nb->cdc}, {&B.ddc, &nb->ddc}};
-  std::vector<std::pair<const OpPtr*, OpPtr*>> work;
+  std::vector<const SparseOp*> src = {B.ident.get(), B.ham.get()};
+  std::vector<OpPtr*> dst = {&nb->ident, &nb->ham};
   for (auto& l : lists) {
     l.second->assign(l.first->size(), nullptr);
     for (size_t i = 0; i < l.first->size(); ++i)
-      if ((*l.first)[i]) work.push_back({&(*l.first)[i], &(*l.second)[i]});
+      if ((*l.first)[i]) {
+        src.push_back((*l.first)[i].get());
+        dst.push_back(&(*l.second)[i]);
+      }
   }
-  std::string err;
-#pragma omp parallel for schedule(dynamic)
-  for (long w = 0; w < (long)work.size(); ++w) {
-    try {
-      *work[w].second = rotate_op(**work[w].first, R);
-    } catch (const std::exception& e) {
-#pragma omp critical(blk_task_err)
-      if (err.empty()) err = e.what();
-    }
-  }
-  if (!err.empty()) throw std::runtime_error(err);
+  const OpRotator host;
+  std::vector<OpPtr> rot = (rotator ? rotator : &host)->rotate(src, R);
+  for (size_t w = 0; w < src.size(); ++w) *dst[w] = rot[w];
   return nb;
 }
 
 void add_rotated_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
-                          const Screen& scr, const Rotation& R, SpinBlock& out) {
+                          const Screen& scr, const Rotation& R, SpinBlock& out, const OpRotator* rotator) {
   const int d = dot.sites[0];
   Builder bd{I, B1, dot, d, B.si};
   std::vector<int> outside;
   for (int i = 0; i < I.norb; ++i)
     if (!B.in(i)) outside.push_back(i);
   auto tasks = comp_tasks(I, B, outside, scr);
-  run_tasks(bd, out, tasks, [&](OpPtr o) { return rotate_op(*o, R); });
+  if (!rotator) {
+    // built and rotated one at a time: only one composite operator per thread
+    run_tasks(bd, out, tasks, [&](OpPtr o) { return rotate_op(*o, R); });
+  } else {
+    // all composite operators built first, then rotated as one batch
+    run_tasks(bd, out, tasks, nullptr);
+    std::vector<OpPtr*> slot;
+    for (const auto& k : tasks) {
+      OpPtr* p = k.kind == 0   ? &out.cd[out.pidx(k.i, k.j, k.K)]
+                 : k.kind == 1 ? &out.cc[out.pidx(k.i, k.j, k.K)]
+                 : k.kind == 2 ? &out.cdc[out.pidx(k.i, k.j, k.K)]
+                 : k.kind == 3 ? &out.ddc[out.pidx(k.i, k.j, k.K)]
+                               : &out.ccd[k.i];
+      if (*p) slot.push_back(p);
+    }
+    std::vector<const SparseOp*> src;
+    for (OpPtr* p : slot) src.push_back(p->get());
+    std::vector<OpPtr> rot = rotator->rotate(src, R);
+    for (size_t w = 0; w < slot.size(); ++w) *slot[w] = rot[w];
+  }
   out.has_comp = true;
 }
 

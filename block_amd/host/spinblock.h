@@ -129,12 +129,19 @@ struct Rotation {
 Rotation make_rotation_info(const StateInfo& os, const std::vector<std::vector<double>>& rot,
                             const std::vector<int>& mkeep);
 OpPtr rotate_op(const SparseOp& o, const Rotation& R);
+// Rotation of a batch of operators of one block (renormalise_transform for
+// all of them).  The default runs rotate_op on the host threads; the sweep
+// can hand a device implementation to the renormalisation.
+struct OpRotator {
+  virtual ~OpRotator() = default;
+  virtual std::vector<OpPtr> rotate(const std::vector<const SparseOp*>& ops, const Rotation& R) const;
+};
 BlockPtr renormalise(const SpinBlock& B, const std::vector<std::vector<double>>& rot, const std::vector<int>& mkeep);
-BlockPtr renormalise(const SpinBlock& B, const Rotation& R);
+BlockPtr renormalise(const SpinBlock& B, const Rotation& R, const OpRotator* rotator = nullptr);
 // Build the complementary operators of the composite block B = B1 (x) dot one
 // at a time and store them rotated into out, the streaming form of
 // build_and_renormalise_transform (BaseOperator.C:341-459).
 void add_rotated_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
-                          const Screen& scr, const Rotation& R, SpinBlock& out);
+                          const Screen& scr, const Rotation& R, SpinBlock& out, const OpRotator* rotator = nullptr);
 
 }  // namespace blk

@@ -282,16 +282,17 @@ struct RenormTimes {
 RenormTimes g_rt;
 
 BlockPtr truncate_block(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
-                        const std::vector<std::vector<double>>& rho, int M, const Screen& scr, double* disc) {
+                        const std::vector<std::vector<double>>& rho, int M, const Screen& scr, double* disc,
+                        const OpRotator* rotator) {
   std::vector<std::vector<double>> rot;
   std::vector<int> mkeep;
   double t0 = now_s();
   *disc = make_rotation(*B.si, rho, M, rot, mkeep);
   Rotation Rot = make_rotation_info(*B.si, rot, mkeep);
   double t1 = now_s();
-  BlockPtr nb = renormalise(B, Rot);
+  BlockPtr nb = renormalise(B, Rot, rotator);
   double t2 = now_s();
-  if (!B.has_comp) add_rotated_comp_ops(I, B, B1, dot, scr, Rot, *nb);
+  if (!B.has_comp) add_rotated_comp_ops(I, B, B1, dot, scr, Rot, *nb, rotator);
   g_rt.eig += t1 - t0;
   g_rt.rotate += t2 - t1;
   g_rt.comp += now_s() - t2;
@@ -356,8 +357,8 @@ SuperBlock infinite_warmup(const Integrals& I, const DmrgParams& P, const SigmaF
     double d1 = 0, d2 = 0;
     auto rhoL = density_matrix(sb, sol, weights, true, noise, rng);
     auto rhoR = density_matrix(sb, sol, weights, false, noise, rng);
-    fwd[s + 1] = truncate_block(I, *L, *fwd[s], *dot1, rhoL, M, scr, &d1);
-    BlockPtr Rn = truncate_block(I, *R, *bwd[k - 1 - s], *dot2, rhoR, M, scr, &d2);
+    fwd[s + 1] = truncate_block(I, *L, *fwd[s], *dot1, rhoL, M, scr, &d1, P.rotator.get());
+    BlockPtr Rn = truncate_block(I, *R, *bwd[k - 1 - s], *dot2, rhoR, M, scr, &d2, P.rotator.get());
     bwd[k - 2 - s] = Rn;
     if (kp == k || 2 * (s + 3) > k) {
       last = sb;
@@ -472,7 +473,7 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       auto rho = density_matrix(sb, sol, weights, true, S->noise, rng);
       trho = now_s() - trho;
       double disc = 0.0;
-      BlockPtr nsys = truncate_block(I, *L, *sys, *dot1, rho, S->M, scr, &disc);
+      BlockPtr nsys = truncate_block(I, *L, *sys, *dot1, rho, S->M, scr, &disc, P.rotator.get());
       rec.max_discarded = std::max(rec.max_discarded, disc);
       rec.renorm_seconds += now_s() - ts;
       if (forward)

@@ -32,6 +32,7 @@ struct DmrgParams {
   double ladder = 0.0;  // > 0: split-precision sigma until the residual norm falls below this (FP64 after)
   int ladder_from_sweep = 0;  // first sweep that uses the ladder (0: also the warm-up)
   bool transform_guess = true;  // start each step from the previous wavefunction in the new basis
+  std::shared_ptr<const OpRotator> rotator;  // operator rotations of the renormalisation (null: host threads)
   uint64_t seed = 12345;
   int verbose = 0;
 };
