@@ -207,16 +207,18 @@ __device__ inline void mma_run(int nv, int nr, dbl4 (&acc)[NRM][NB], const doubl
 // GEMM 1 is PB blocks per wave (16 PB WC wide), so a V tile wider than the W
 // chunk forms every W column once and contracts it into NV / PB chunks' worth
 // of V columns.
-template <int NRM, int NV>
+template <int NRM, int NV, int PBW = PB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sigma(
     const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
     const TileRef* __restrict__ tiles, int ntiles, const double* __restrict__ arena, const double* __restrict__ c,
     double* __restrict__ v, double* __restrict__ part) {
-  static_assert(NV % PB == 0, "V tile width is a whole number of W chunks");
+  static_assert(NV % PBW == 0, "V tile width is a whole number of W chunks");
   constexpr int NT = 256;
   // GEMM 1 uses two op(A) chunk buffers, GEMM 2 the W chunk; they share the
   // LDS (barriers separate the phases).  Both k-major: X[k * P + row].
-  constexpr int LDSD = 2 * PKC * GP > GWS ? 2 * PKC * GP : GWS;
+  // W chunk: JC (TM + 1) doubles, JC = 16 PBW WC
+  constexpr int WSD = PBW == PB ? GWS : 16 * PBW * 4 * 17;
+  constexpr int LDSD = 2 * PKC * GP > WSD ? 2 * PKC * GP : WSD;
   __shared__ double lds[LDSD];
   double* const As0 = lds;
   double* const As1 = lds + PKC * GP;
@@ -229,7 +231,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const bsg_vblock_t vb = vbs[sg.vb];
   const int lwr = tr.pad & 3, nr = min(((tr.pad >> 2) & 1) + 1, NRM);
   const int WR = 1 << lwr, WC = 4 >> lwr;
-  const int TM = 16 * nr * WR, JC = 16 * PB * WC, JV = 16 * NV * WC, P = TM + 1;
+  const int TM = 16 * nr * WR, JC = 16 * PBW * WC, JV = 16 * NV * WC, P = TM + 1;
   const int row0 = tr.tm * TM, colg = tr.tn * JV;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       if (pr.b_off < 0 && (j0 < colg || j0 >= colg + JV)) continue;
       const int jw = min(JC, pr.k2 - j0);        // W columns in this chunk
       int nwb = 0;                               // W column blocks of this wave in the chunk
-      for (int t = 0; t < PB; ++t)
+      for (int t = 0; t < PBW; ++t)
         if ((wc + WC * t) * 16 < jw) nwb = t + 1;
       if (nrw == 0) nwb = 0;
       if (pr.a_off < 0) {
@@ -286,11 +288,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
       } else {
         const __amdgpu_buffer_rsrc_t rA = block_rsrc(arena + pr.a_off, (int64_t)vb.m * pr.k1);
-        dbl4 accW[NRM][PB];
+        dbl4 accW[NRM][PBW];
 #pragma unroll
         for (int r = 0; r < NRM; ++r)
 #pragma unroll
-          for (int t = 0; t < PB; ++t) accW[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+          for (int t = 0; t < PBW; ++t) accW[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
         // Pipeline over 32-deep chunks, each consumed in two 16-deep halves:
         // the op(A) elements of a chunk and the C operands of its first half
         // are loaded during the previous chunk's second half, the C operands
@@ -309,24 +311,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
           aoff[u] = e < TM * PKC ? (uint32_t)(pr.a_trans ? p * vb.m + gi : gi * pr.k1 + p) * 8u : OOB;
         }
         const uint32_t astep = (uint32_t)(pr.a_trans ? vb.m : 1) * 8u;  // bytes per K index
-        uint32_t coff[PB];
+        uint32_t coff[PBW];
 #pragma unroll
-        for (int t = 0; t < PB; ++t) {
+        for (int t = 0; t < PBW; ++t) {
           const int col = j0 + (wc + WC * t) * 16 + li;
           coff[t] = (t < nwb && col < pr.k2) ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
         }
         const uint32_t crow = (uint32_t)pr.k2 * 8u;  // bytes per C row
-        double an[AE], bn[4][PB];
+        // C operands in batches of S1 K steps (8 deep for the 256-wide W
+        // chunks: register budget); each batch is loaded while the previous
+        // one runs, and the last batch of a chunk overlaps the next chunk's
+        // op(A) and first C loads
+        constexpr int S1 = PBW > PB ? 2 : 4;
+        double an[AE], bn[S1][PBW];
         auto load_a = [&](int k0) {
 #pragma unroll
           for (int u = 0; u < AE; ++u)
             if (u * NT < TM * PKC) an[u] = bload2(rA, aoff[u], (uint32_t)k0 * astep);
         };
-        auto load_c = [&](int k0, int kh) {  // C rows k0 + kh .. k0 + kh + 15
+        auto load_c = [&](int k0, int kh) {  // C rows k0 + kh .. k0 + kh + 4 S1 - 1
 #pragma unroll
-          for (int s4 = 0; s4 < 4; ++s4)
+          for (int s4 = 0; s4 < S1; ++s4)
 #pragma unroll
-            for (int t = 0; t < PB; ++t) bn[s4][t] = bload2(rC, coff[t], (uint32_t)(k0 + kh + 4 * s4) * crow);
+            for (int t = 0; t < PBW; ++t) bn[s4][t] = bload2(rC, coff[t], (uint32_t)(k0 + kh + 4 * s4) * crow);
         };
         load_a(0);
         load_c(0, 0);
@@ -342,46 +349,66 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
               Ab[p * P + i] = pr.alpha_a * an[u];
             }
           }
-          double bc[4][PB];
+          double bc[S1][PBW];
 #pragma unroll
-          for (int s4 = 0; s4 < 4; ++s4)
+          for (int s4 = 0; s4 < S1; ++s4)
 #pragma unroll
-            for (int t = 0; t < PB; ++t) bc[s4][t] = bn[s4][t];
+            for (int t = 0; t < PBW; ++t) bc[s4][t] = bn[s4][t];
           __syncthreads();
-          if (kc > 16) load_c(k0, 16);
-          mma_run<NRM, PB>(nwb, nrw, accW, Ab, P, arow, 0, kc, li, lk, bc);
-          if (kc > 16) {
+          if constexpr (S1 == 4) {
+            // two 16-deep halves
+            if (kc > 16) load_c(k0, 16);
+            mma_run<NRM, PBW>(nwb, nrw, accW, Ab, P, arow, 0, kc, li, lk, bc);
+            if (kc > 16) {
 #pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4)
+              for (int s4 = 0; s4 < 4; ++s4)
 #pragma unroll
-              for (int t = 0; t < PB; ++t) bc[s4][t] = bn[s4][t];
+                for (int t = 0; t < PBW; ++t) bc[s4][t] = bn[s4][t];
+            }
+            if (k0 + PKC < pr.k1) {
+              load_a(k0 + PKC);
+              load_c(k0 + PKC, 0);
+            }
+            if (kc > 16) mma_run<NRM, PBW>(nwb, nrw, accW, Ab, P, arow, 16, kc, li, lk, bc);
+          } else {
+            for (int kh = 0; kh < kc; kh += 4 * S1) {
+              const bool more = kh + 4 * S1 < kc;
+              if (more) {
+                load_c(k0, kh + 4 * S1);
+              } else if (k0 + PKC < pr.k1) {
+                load_a(k0 + PKC);
+                load_c(k0 + PKC, 0);
+              }
+              mma_run<NRM, PBW, S1>(nwb, nrw, accW, Ab, P, arow, kh, kc, li, lk, bc);
+              if (more) {
+#pragma unroll
+                for (int s4 = 0; s4 < S1; ++s4)
+#pragma unroll
+                  for (int t = 0; t < PBW; ++t) bc[s4][t] = bn[s4][t];
+              }
+            }
           }
-          if (k0 + PKC < pr.k1) {
-            load_a(k0 + PKC);
-            load_c(k0 + PKC, 0);
-          }
-          if (kc > 16) mma_run<NRM, PB>(nwb, nrw, accW, Ab, P, arow, 16, kc, li, lk, bc);
         }
         __syncthreads();  // op(A) buffers done before W (same LDS) is written
         if (pr.b_off < 0) {
-          // identity B': the W blocks of this wave are V blocks t + ch PB of
-          // the tile, ch = the chunk's position in the tile
+          // identity B': the W blocks of this wave are V blocks t + ch PBW
+          // of the tile, ch = the chunk's position in the tile
           const int ch = (j0 - colg) / JC;
 #pragma unroll
-          for (int cc = 0; cc < NV / PB; ++cc)
+          for (int cc = 0; cc < NV / PBW; ++cc)
             if (cc == ch)
 #pragma unroll
               for (int r = 0; r < NRM; ++r)
 #pragma unroll
-                for (int t = 0; t < PB; ++t)
+                for (int t = 0; t < PBW; ++t)
 #pragma unroll
-                  for (int i = 0; i < 4; ++i) accV[r][cc * PB + t][i] += pr.alpha * accW[r][t][i];
+                  for (int i = 0; i < 4; ++i) accV[r][cc * PBW + t][i] += pr.alpha * accW[r][t][i];
           continue;
         }
 #pragma unroll
         for (int r = 0; r < NRM; ++r)
 #pragma unroll
-          for (int t = 0; t < PB; ++t) {
+          for (int t = 0; t < PBW; ++t) {
             const int cb = wc + WC * t;
             if (r < nr)
 #pragma unroll
@@ -394,7 +421,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       // next batch (16 deep, 8 for the widest tiles: register budget) are
       // loaded while the current one runs.
       {
-        constexpr int S2 = NV >= 6 ? 2 : 4;
+        constexpr int S2 = NV >= 6 || PBW > PB ? 2 : 4;
         const __amdgpu_buffer_rsrc_t rB = block_rsrc(arena + pr.b_off, (int64_t)vb.n * pr.k2);
         // B'(k, col) at k * ks + col * cs inside the block
         const int ks = pr.b_trans ? 1 : vb.n, cs = pr.b_trans ? pr.k2 : 1;

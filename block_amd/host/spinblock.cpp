@@ -2,6 +2,8 @@
 // Op::build recursions) and renormalisation.
 #include "spinblock.h"
 
+#include <chrono>
+
 #include <algorithm>
 #include <array>
 #include <cstdio>
@@ -725,6 +727,9 @@ BlockPtr renormalise(const SpinBlock& B, const Rotation& R, const OpRotator* rot
   return nb;
 }
 
+double g_comp_build_seconds = 0.0;
+double comp_build_seconds() { return g_comp_build_seconds; }
+
 void add_rotated_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
                           const Screen& scr, const Rotation& R, SpinBlock& out, const OpRotator* rotator) {
   const int d = dot.sites[0];
@@ -738,7 +743,9 @@ void add_rotated_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBloc
     run_tasks(bd, out, tasks, [&](OpPtr o) { return rotate_op(*o, R); });
   } else {
     // all composite operators built first, then rotated as one batch
+    const auto t0 = std::chrono::steady_clock::now();
     run_tasks(bd, out, tasks, nullptr);
+    g_comp_build_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::vector<OpPtr*> slot;
     for (const auto& k : tasks) {
       OpPtr* p = k.kind == 0   ? &out.cd[out.pidx(k.i, k.j, k.K)]

@@ -143,5 +143,7 @@ BlockPtr renormalise(const SpinBlock& B, const Rotation& R, const OpRotator* rot
 // build_and_renormalise_transform (BaseOperator.C:341-459).
 void add_rotated_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
                           const Screen& scr, const Rotation& R, SpinBlock& out, const OpRotator* rotator = nullptr);
+// seconds spent building complementary operators before a batch rotation (accumulated)
+double comp_build_seconds();
 
 }  // namespace blk
