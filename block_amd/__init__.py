@@ -70,6 +70,10 @@ _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp])
 
 _sig(_h, "blkh_rotation_check", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _lp])
+_ip = ctypes.POINTER(ctypes.c_int)
+_sig(_h, "blkh_fiedler_order", ctypes.c_int, [_vp, _ip])
+_sig(_h, "blkh_fiedler_order_matrix", ctypes.c_int, [ctypes.c_int, _dp, _ip])
+_sig(_h, "blkh_reorder_integrals", _vp, [_vp, _ip])
 
 FP64, SPLIT_BF16 = 0, 1
 SWEEP_COLS = ("sweep", "forward", "M", "E0", "discarded", "seconds", "sigma_seconds", "sigma_flops",
@@ -102,6 +106,25 @@ class Integrals:
     isym = property(lambda s: host_lib.blkh_isym(s._h))
     ms2 = property(lambda s: host_lib.blkh_ms2(s._h))
     ecore = property(lambda s: host_lib.blkh_ecore(s._h))
+
+    def fiedler_order(self):
+        """Orbital order from the Fiedler vector of the exchange-integral graph
+        (fiedler.C:47-81): order[a] is the orbital placed at position a."""
+        out = (ctypes.c_int * self.norb)()
+        if host_lib.blkh_fiedler_order(self._h, out) != 0:
+            raise _err()
+        return list(out)
+
+    def reordered(self, order):
+        """The integrals with orbital order[a] moved to position a."""
+        arr = (ctypes.c_int * self.norb)(*order)
+        h = host_lib.blkh_reorder_integrals(self._h, arr)
+        if not h:
+            raise _err()
+        obj = object.__new__(Integrals)
+        obj._h = h
+        obj.path = self.path
+        return obj
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -260,3 +283,13 @@ def rotation_check(ints, nsys, M, device=0):
     if host_lib.blkh_rotation_check(ints._h, nsys, M, device, ctypes.byref(worst), ctypes.byref(n)) != 0:
         raise _err()
     return worst.value, n.value
+
+
+def fiedler_order_matrix(m):
+    """fiedler_reorder (fiedler.C:47-81) on a symmetric interaction matrix."""
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    n = m.shape[0]
+    out = (ctypes.c_int * n)()
+    if host_lib.blkh_fiedler_order_matrix(n, _ptr(m), out) != 0:
+        raise _err()
+    return list(out)

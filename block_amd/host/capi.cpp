@@ -66,6 +66,32 @@ void* blkh_load_fcidump(const char* path) {
   return guarded_ptr([&]() -> void* { return new Integrals(read_fcidump(path)); });
 }
 void blkh_free_integrals(void* h) { delete static_cast<Integrals*>(h); }
+// Fiedler ordering of the orbitals (input.C:2366-2376) and the integrals in a
+// given order (new orbital a = old orbital perm[a]).
+int blkh_fiedler_order(void* ints, int* order) {
+  return guarded([&]() {
+    auto o = fiedler_order(*static_cast<Integrals*>(ints));
+    std::copy(o.begin(), o.end(), order);
+    return 0;
+  });
+}
+int blkh_fiedler_order_matrix(int n, const double* m, int* order) {
+  return guarded([&]() {
+    auto o = fiedler_order_matrix(n, m);
+    std::copy(o.begin(), o.end(), order);
+    return 0;
+  });
+}
+void* blkh_reorder_integrals(void* ints, const int* perm) {
+  return guarded_ptr([&]() -> void* {
+    const Integrals& I = *static_cast<Integrals*>(ints);
+    std::vector<int> p(perm, perm + I.norb), seen(I.norb, 0);
+    for (int x : p) {
+      if (x < 0 || x >= I.norb || seen[x]++) throw std::runtime_error("reorder: not a permutation");
+    }
+    return new Integrals(I.reordered(p));
+  });
+}
 int blkh_norb(void* h) { return static_cast<Integrals*>(h)->norb; }
 int blkh_nelec(void* h) { return static_cast<Integrals*>(h)->nelec; }
 int blkh_isym(void* h) { return static_cast<Integrals*>(h)->isym; }

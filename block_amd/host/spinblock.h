@@ -50,6 +50,11 @@ struct Integrals {
 // FCIDUMP reader/writer (the input side of input.C:1407-1560).
 Integrals read_fcidump(const std::string& path);
 void write_fcidump(const std::string& path, const Integrals& I);
+// Fiedler-vector orbital ordering (fiedler.C:47-81): order[a] = the orbital
+// placed at position a (Integrals::reordered's convention).  The matrix form
+// takes any symmetric n x n interaction matrix.
+std::vector<int> fiedler_order(const Integrals& I);
+std::vector<int> fiedler_order_matrix(int n, const double* m);
 
 struct SpinBlock;
 using BlockPtr = std::shared_ptr<SpinBlock>;
