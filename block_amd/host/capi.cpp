@@ -219,8 +219,8 @@ int blkh_gpu_sigma_timed(void* h, const double* c, int warmup, int reps, double*
 // Returns the number of sweeps run.
 int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
                   const double* sched, int max_sweeps, double sweep_tol, double ladder, int ladder_from, int guess,
-                  int device,
-                  int verbose, double* energies_out, double* sweep_out) {
+                  const char* checkpoint, const char* resume, int device, int verbose, double* energies_out,
+                  double* sweep_out) {
   return guarded([&]() {
     const Integrals& I = *static_cast<Integrals*>(ints);
     DmrgParams P;
@@ -237,6 +237,8 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
     P.ladder = ladder;
     P.ladder_from_sweep = ladder_from;
     P.transform_guess = guess != 0;
+    if (checkpoint) P.checkpoint = checkpoint;
+    if (resume) P.resume = resume;
     GpuContext ctx(device);
     P.rotator = std::make_shared<GpuRotator>(ctx);
     auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {

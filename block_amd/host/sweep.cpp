@@ -1,4 +1,5 @@
 #include "sweep.h"
+#include "checkpoint.h"
 #include "gpu_sigma.h"
 #include "guess.h"
 
@@ -8,6 +9,7 @@
 #include <map>
 #include <numeric>
 #include <random>
+#include <sstream>
 
 namespace blk {
 
@@ -388,7 +390,24 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
     std::vector<std::vector<double>> b;
     BlockPtr env, nsys;
   } prev;
-  {
+  double prev_full = 1e300;
+  int start_sweep = 0;
+  if (!P.resume.empty()) {
+    // continue from a saved sweep state (blocks, last wavefunction, records)
+    SweepState st = load_sweep_state(P.resume);
+    if ((int)st.fwd.size() != k || (int)st.bwd.size() != k)
+      throw std::runtime_error("run_dmrg: the saved state is for another number of orbitals");
+    fwd = std::move(st.fwd);
+    bwd = std::move(st.bwd);
+    prev.wave = st.wave;
+    prev.b = std::move(st.b);
+    res = std::move(st.res);
+    prev_full = st.prev_full;
+    start_sweep = st.next_sweep;
+    std::istringstream is(st.rng);
+    is >> rng;
+    if (P.verbose) std::fprintf(stderr, "resumed from %s at sweep %d\n", P.resume.c_str(), start_sweep);
+  } else {
     SuperBlock last = infinite_warmup(I, P, make_sigma, fwd, bwd, rng, &prev.b);
     prev.wave = last.wave;
   }
@@ -396,11 +415,11 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
   int fwd_max = 0;
   while (fwd_max + 1 < k && fwd[fwd_max + 1]) ++fwd_max;
   const int l0 = fwd_max - 1;  // first forward step (env = bwd[l0 + 3] exists)
-  if (P.verbose) std::fprintf(stderr, "warmup: infinite-system growth to %d|%d sites, %.2fs\n", fwd_max + 1,
-                              k - fwd_max - 1, now_s() - tw);
+  if (P.verbose && P.resume.empty())
+    std::fprintf(stderr, "warmup: infinite-system growth to %d|%d sites, %.2fs\n", fwd_max + 1, k - fwd_max - 1,
+                 now_s() - tw);
 
-  double prev_full = 1e300;
-  for (int sweep = 0; sweep < P.max_sweeps; ++sweep) {
+  for (int sweep = start_sweep; sweep < P.max_sweeps; ++sweep) {
     const ScheduleEntry* S = &P.schedule[0];
     for (const auto& e : P.schedule)
       if (sweep >= e.start_sweep) S = &e;
@@ -520,12 +539,24 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
     if (!forward) {
       double e = rec.energies[0];
       bool last_entry = sweep >= P.schedule.back().start_sweep;
-      if (last_entry && std::fabs(e - prev_full) < P.sweep_tol) {
-        res.converged = true;
-        break;
-      }
-      prev_full = e;
+      if (last_entry && std::fabs(e - prev_full) < P.sweep_tol) res.converged = true;
+      else prev_full = e;
     }
+    if (!P.checkpoint.empty()) {
+      SweepState st;
+      st.next_sweep = sweep + 1;
+      st.fwd = fwd;
+      st.bwd = bwd;
+      st.wave = prev.wave;
+      st.b = prev.b;
+      st.res = res;
+      st.prev_full = prev_full;
+      std::ostringstream os;
+      os << rng;
+      st.rng = os.str();
+      save_sweep_state(P.checkpoint, st);
+    }
+    if (res.converged) break;
   }
   return res;
 }

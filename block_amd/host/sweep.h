@@ -7,6 +7,7 @@
 #pragma once
 #include <functional>
 #include <memory>
+#include <string>
 
 #include "davidson.h"
 
@@ -33,6 +34,8 @@ struct DmrgParams {
   int ladder_from_sweep = 0;  // first sweep that uses the ladder (0: also the warm-up)
   bool transform_guess = true;  // start each step from the previous wavefunction in the new basis
   std::shared_ptr<const OpRotator> rotator;  // operator rotations of the renormalisation (null: host threads)
+  std::string checkpoint;  // file the sweep state is written to after every sweep (empty: none)
+  std::string resume;      // sweep-state file to continue from instead of the warm-up (empty: none)
   uint64_t seed = 12345;
   int verbose = 0;
 };

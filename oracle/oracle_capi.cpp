@@ -135,8 +135,8 @@ void* orc_exact_superblock(void* ints, int nelec, int twoS, int irrep, int nL) {
 // the sweep driver with the oracle sigma (CPU reference run); same outputs as
 // blkh_dmrg_gpu (sweep_out rows of kSweepCols)
 int orc_dmrg_cpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
-                 const double* sched, int max_sweeps, double sweep_tol, int guess, int nthreads, int verbose,
-                 double* energies_out, double* sweep_out) {
+                 const double* sched, int max_sweeps, double sweep_tol, int guess, const char* checkpoint,
+                 const char* resume, int nthreads, int verbose, double* energies_out, double* sweep_out) {
   return guarded([&]() {
     const Integrals& I = *static_cast<Integrals*>(ints);
     DmrgParams P;
@@ -151,6 +151,8 @@ int orc_dmrg_cpu(void* ints, int nelec, int twoS, int irrep, int nroots, const d
     P.sweep_tol = sweep_tol;
     P.verbose = verbose;
     P.transform_guess = guess != 0;
+    if (checkpoint) P.checkpoint = checkpoint;
+    if (resume) P.resume = resume;
     auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {
       return std::unique_ptr<DavidsonFunctor>(new OracleFunctor(sb, nthreads));
     });

@@ -1,0 +1,39 @@
+"""Checkpoint and resume of the sweep (the part of Block's block, rotation
+and sweep-state files, save_load_block.C:23-91, rotationmat.C:21-60,
+sweep_params.C:156-185): a run interrupted after some sweeps and resumed
+from its saved state continues exactly as the uninterrupted run."""
+import os
+
+import numpy as np
+import pytest
+
+import _oracle
+import block_amd
+from helpers import config_fcidump
+
+COLS = block_amd.SWEEP_COLS
+SCHED = [(0, 50, 1e-6, 1e-5), (2, 100, 1e-10, 0.0)]
+
+
+def test_resume_continues_the_sweeps(tmp_path):
+    ints = config_fcidump("hubbard12")
+    e_all, sw_all = _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=6, sweep_tol=0.0)
+    ck = str(tmp_path / "state.bin")
+    e3, sw3 = _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=3, sweep_tol=0.0, checkpoint=ck)
+    assert os.path.getsize(ck) > 0
+    e, sw = _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=6, sweep_tol=0.0, resume=ck)
+    # the resumed run carries the first three sweeps' records and runs 3..5
+    assert sw.shape == sw_all.shape
+    assert np.array_equal(sw[:3, COLS.index("sweep")], sw3[:, COLS.index("sweep")])
+    np.testing.assert_allclose(sw[:, COLS.index("E0")], sw_all[:, COLS.index("E0")], rtol=0, atol=1e-10)
+    assert abs(e[0] - e_all[0]) <= 1e-10
+    # every resumed step started from a carried wavefunction (the turn too)
+    assert np.all(sw[3:, COLS.index("guessed_steps")] == ints.norb - 3)
+
+
+def test_resume_rejects_garbage(tmp_path):
+    ints = config_fcidump("hubbard12")
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"not a state file at all")
+    with pytest.raises(RuntimeError):
+        _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=2, resume=str(bad))
