@@ -208,7 +208,7 @@ __device__ inline void mma_run(int nv, int nr, dbl4 (&acc)[NRM][NB], const doubl
 // chunk forms every W column once and contracts it into NV / PB chunks' worth
 // of V columns.
 template <int NRM, int NV, int PBW = PB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sigma(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NV == 4 ? 3 : 2))) void k_sigma(
     const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
     const TileRef* __restrict__ tiles, int ntiles, const double* __restrict__ arena, const double* __restrict__ c,
     double* __restrict__ v, double* __restrict__ part) {
