@@ -44,3 +44,20 @@ def test_gpu_dmrg_ladder_chain12():
     assert split_calls > 0
     assert abs(e[0] - e64[0]) <= 1e-8
     assert e[0] == pytest.approx(-6.526243384454524, abs=1e-6)
+
+
+def test_gpu_dmrg_checkpoint_resume(tmp_path):
+    """GPU sweeps resumed from a saved sweep state continue like the
+    uninterrupted run (blocks, rotations and the carried wavefunction are
+    restored; the first resumed step starts from the saved solution)."""
+    print("[gpu] dmrg checkpoint/resume chain 12", flush=True)
+    ints = config_fcidump("hubbard12")
+    sched = [(0, 50, 1e-6, 1e-5), (2, 100, 1e-10, 0.0)]
+    e_all, sw_all = block_amd.dmrg(ints, schedule=sched, max_sweeps=6, sweep_tol=0.0)
+    ck = str(tmp_path / "state.bin")
+    block_amd.dmrg(ints, schedule=sched, max_sweeps=3, sweep_tol=0.0, checkpoint=ck)
+    e, sw = block_amd.dmrg(ints, schedule=sched, max_sweeps=6, sweep_tol=0.0, resume=ck)
+    col = block_amd.SWEEP_COLS.index("E0")
+    assert sw.shape == sw_all.shape
+    assert abs(e[0] - e_all[0]) <= 1e-10
+    assert max(abs(sw[:, col] - sw_all[:, col])) <= 1e-10
