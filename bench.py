@@ -2,14 +2,17 @@
 
 A step is one H.C application (multiply_h, davidson.C:17-22) of the FP64 GPU
 path on the mid-chain two-site superblock of the 24-orbital / 24-electron C1
-synthetic Hamiltonian at M = 500, reached by the
-infinite-system DMRG warm-up (both halves truncated by density matrices).  Operator blocks
-and the wavefunction are resident in HBM before the timed region; the timed
-region is exactly --steps sigma applications bracketed by stream
-synchronisation, timed with HIP events on the sigma stream.
+synthetic Hamiltonian at M = 500, reached by the infinite-system DMRG warm-up
+(both halves truncated by density matrices).  Operator blocks and the
+wavefunction are resident in HBM before the timed region; the timed region is
+exactly --steps sigma applications bracketed by stream synchronisation, timed
+with HIP events on the sigma stream.  The same line reports the
+split-precision sigma on that superblock, one full sweep at M = 500 in FP64
+and with the precision ladder, and the CPU baseline.
 
 Prints one JSON line (see the fields below); with --gpus > 1 it exits with a
-message, the path runs on one GPU.
+message, the path runs on one GPU.  --summarize-trace / --summarize-traffic
+turn rocprofv3 CSVs of such a run into the tables kept with the profiles.
 """
 import argparse
 import ctypes
