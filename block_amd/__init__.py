@@ -68,7 +68,7 @@ _sig(_h, "blkh_gpu_sigma_set_precision", ctypes.c_int, [_vp, ctypes.c_int])
 _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
      [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
-      ctypes.c_int, _dp, _dp])
+      ctypes.c_int, ctypes.c_int, _dp, _dp])
 
 _sig(_h, "blkh_rotation_check", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _lp])
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -249,7 +249,8 @@ def _schedule_array(schedule):
 
 
 def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=None, max_sweeps=20,
-         sweep_tol=1e-8, device=0, verbose=0, ladder=0.0, guess=True, ladder_from=0, checkpoint=None, resume=None):
+         sweep_tol=1e-8, device=0, verbose=0, ladder=0.0, guess=True, ladder_from=0, checkpoint=None, resume=None,
+         host_ahead=True):
     """Two-site spin-adapted DMRG (Sweep::do_one, sweep.C:341) with the Davidson
     sigma on the GPU.  schedule: rows (start_sweep, M, davidson_tol, noise).
     ladder > 0: split-precision sigma until the Davidson residual norm falls
@@ -260,6 +261,9 @@ def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=
     checkpoint: file the sweep state is written to after every sweep; resume:
     such a file to continue from (blocks, last wavefunction, sweep records)
     instead of the warm-up.
+    host_ahead: build the host-side operators that do not depend on the
+    current solve (the new system block's complementary operators, the next
+    environment block) on a host thread while the GPU runs the Davidson solve.
     Returns (energies, sweeps) with sweeps rows named by SWEEP_COLS."""
     nelec = ints.nelec if nelec is None else nelec
     irrep = ints.isym if irrep is None else irrep
@@ -272,7 +276,8 @@ def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=
     n = host_lib.blkh_dmrg_gpu(ints._h, nelec, twoS, irrep, nroots, _ptr(w), len(schedule), _ptr(sched), max_sweeps,
                                sweep_tol, float(ladder), int(ladder_from), int(guess),
                                None if checkpoint is None else str(checkpoint).encode(),
-                               None if resume is None else str(resume).encode(), device, verbose, _ptr(e), _ptr(sw))
+                               None if resume is None else str(resume).encode(), int(bool(host_ahead)), device,
+                               verbose, _ptr(e), _ptr(sw))
     if n < 0:
         raise _err()
     return e, sw[: nc * n].reshape(n, nc)

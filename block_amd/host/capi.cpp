@@ -213,14 +213,15 @@ int blkh_gpu_sigma_timed(void* h, const double* c, int warmup, int reps, double*
 
 // Two-site DMRG with the sigma on the GPU.  schedule: n_sched rows of
 // (start_sweep, M, davidson_tol, noise).  ladder > 0: split-precision sigma
-// until the Davidson residual norm falls below it, FP64 after.
+// until the Davidson residual norm falls below it, FP64 after.  ahead: host
+// operator builds beside the Davidson solve (1 on, 0 off, -1 default: on).
 // energies_out[nroots]; sweep_out[max_sweeps * 10]: sweep, forward, M, E0,
 // discarded, seconds, sigma_seconds, sigma_flops, sigma calls, split calls.
 // Returns the number of sweeps run.
 int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
                   const double* sched, int max_sweeps, double sweep_tol, double ladder, int ladder_from, int guess,
-                  const char* checkpoint, const char* resume, int device, int verbose, double* energies_out,
-                  double* sweep_out) {
+                  const char* checkpoint, const char* resume, int ahead, int device, int verbose,
+                  double* energies_out, double* sweep_out) {
   return guarded([&]() {
     const Integrals& I = *static_cast<Integrals*>(ints);
     DmrgParams P;
@@ -239,6 +240,7 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
     P.transform_guess = guess != 0;
     if (checkpoint) P.checkpoint = checkpoint;
     if (resume) P.resume = resume;
+    P.host_ahead = ahead;
     GpuContext ctx(device);
     P.rotator = std::make_shared<GpuRotator>(ctx);
     auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {

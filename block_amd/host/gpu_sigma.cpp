@@ -10,7 +10,7 @@ void bsg_check(int rc, const char* what) {
   if (rc != BSG_OK) throw std::runtime_error(std::string(what) + ": " + bsg_last_error());
 }
 
-GpuContext::GpuContext(int device) { bsg_check(bsg_init(&ctx, device), "bsg_init"); }
+GpuContext::GpuContext(int dev) : device(dev) { bsg_check(bsg_init(&ctx, dev), "bsg_init"); }
 GpuContext::~GpuContext() {
   if (!ctx) return;
   if (arena) bsg_mfree(ctx, arena);
@@ -265,6 +265,21 @@ std::vector<double> GpuSigma::diagonal() {
   return out;
 }
 
+void GpuRotator::preload(const std::vector<OpPtr>& ops) const {
+  preloaded_.clear();
+  held_ = ops;
+  if (!side_) side_.reset(new GpuContext(g_.device));
+  std::vector<std::pair<const SparseOp*, int64_t>> at;
+  int64_t n = 0;
+  for (const OpPtr& o : ops) {
+    at.push_back({o.get(), n});
+    n += (int64_t)o->data.size();
+  }
+  double* dst = side_->ensure_work((size_t)std::max<int64_t>(1, n));
+  upload_ops(*side_, at, dst, (size_t)n);
+  for (const auto& x : at) preloaded_[x.first] = x.second;
+}
+
 std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, const Rotation& R) const {
   const StateInfo& os = *R.old;
   const auto& rot = *R.rot;
@@ -321,15 +336,23 @@ std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, c
   }
   double* base = g_.ensure_work((size_t)(total + usize));
   double* ubuf = base + total;
-  // operators still in the arena are copied on the device, the rest uploaded
+  // operators still in the arena or preloaded are copied on the device, the
+  // rest uploaded
   std::vector<std::pair<const SparseOp*, int64_t>> host;
   for (const auto& x : src) {
     auto f = g_.resident.find(x.first);
+    auto p = preloaded_.find(x.first);
     if (f != g_.resident.end() && g_.arena && !g_.arena_leased)
       bsg_check(bsg_copy(g_.ctx, g_.arena + f->second, base + x.second, (int64_t)x.first->data.size()), "operator copy");
-    else
+    else if (p != preloaded_.end()) {
+      bsg_check(bsg_copy(g_.ctx, side_->work + p->second, base + x.second, (int64_t)x.first->data.size()),
+                "preloaded operator copy");
+      preloaded_.erase(p);
+    } else {
       host.push_back(x);
+    }
   }
+  if (preloaded_.empty()) held_.clear();
   upload_ops(g_, host, base, (size_t)soff);
   {
     std::vector<double> u((size_t)std::max<int64_t>(1, usize));

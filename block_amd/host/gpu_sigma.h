@@ -5,6 +5,7 @@
 // term list is flattened into the two-phase GEMM-pair plan of
 // include/block_sigma.h.  apply() is multiply_h::operator() (v += H c).
 #pragma once
+#include <memory>
 #include <unordered_map>
 
 #include "../../include/block_sigma.h"
@@ -20,6 +21,7 @@ void bsg_check(int rc, const char* what);
 // the operator arena (leased by one GpuSigma at a time) and the page-locked
 // staging it is uploaded through, both grown on demand.
 struct GpuContext {
+  int device = 0;
   bsg_context* ctx = nullptr;
   double* arena = nullptr;
   size_t arena_cap = 0;  // doubles
@@ -105,13 +107,24 @@ class GpuSigma {
 // rotation matrices as the operator arena and the old operator blocks as the
 // wavefunction.  Operators are uploaded through page-locked staging and the
 // rotated ones copied back.
+// preload() uploads operators ahead of their rotation on a second device
+// context (its own stream, staging and buffer), so the copy runs beside the
+// sigma kernels of the solve; rotate() then copies them on the device.  A
+// preload is consumed by the next rotate() (the sweep joins the preloading
+// thread before it rotates).
 class GpuRotator : public OpRotator {
  public:
   explicit GpuRotator(GpuContext& g) : g_(g) {}
   std::vector<OpPtr> rotate(const std::vector<const SparseOp*>& ops, const Rotation& R) const override;
+  void preload(const std::vector<OpPtr>& ops) const override;
 
  private:
   GpuContext& g_;
+  mutable std::unique_ptr<GpuContext> side_;  // created on the first preload
+  // preloaded operators not yet rotated (kept alive, so no other operator can
+  // take one's address) and their offsets in side_->work
+  mutable std::vector<OpPtr> held_;
+  mutable std::unordered_map<const SparseOp*, int64_t> preloaded_;
 };
 
 // Linear::block_davidson on device vectors with the sigma of h (same control

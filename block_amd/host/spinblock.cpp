@@ -2,13 +2,14 @@
 // Op::build recursions) and renormalisation.
 #include "spinblock.h"
 
-#include <chrono>
-
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <fstream>
 #include <sstream>
+#include <utility>
 
 namespace blk {
 
@@ -727,8 +728,75 @@ BlockPtr renormalise(const SpinBlock& B, const Rotation& R, const OpRotator* rot
   return nb;
 }
 
-double g_comp_build_seconds = 0.0;
-double comp_build_seconds() { return g_comp_build_seconds; }
+std::atomic<double> g_comp_build_seconds{0.0};
+double comp_build_seconds() { return g_comp_build_seconds.load(); }
+
+namespace {
+// slots of out that the tasks fill, in task order
+template <class SB>
+std::vector<decltype(&std::declval<SB&>().cd[0])> task_slots(const std::vector<Task>& tasks, SB& out) {
+  std::vector<decltype(&out.cd[0])> slot;
+  for (const auto& k : tasks)
+    slot.push_back(k.kind == 0   ? &out.cd[out.pidx(k.i, k.j, k.K)]
+                   : k.kind == 1 ? &out.cc[out.pidx(k.i, k.j, k.K)]
+                   : k.kind == 2 ? &out.cdc[out.pidx(k.i, k.j, k.K)]
+                   : k.kind == 3 ? &out.ddc[out.pidx(k.i, k.j, k.K)]
+                                 : &out.ccd[k.i]);
+  return slot;
+}
+void add_seconds(std::atomic<double>& a, double x) {
+  double cur = a.load();
+  while (!a.compare_exchange_weak(cur, cur + x)) {
+  }
+}
+}  // namespace
+
+struct CompOpsAhead {
+  const StateInfo* si = nullptr;  // the composite basis they are built in
+  std::vector<Task> tasks;
+  SpinBlock ops;  // the built operators, in the slots of their tasks
+};
+
+std::shared_ptr<CompOpsAhead> build_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1,
+                                             const SpinBlock& dot, const Screen& scr) {
+  auto a = std::make_shared<CompOpsAhead>();
+  a->si = B.si.get();
+  Builder bd{I, B1, dot, dot.sites[0], B.si};
+  std::vector<int> outside;
+  for (int i = 0; i < I.norb; ++i)
+    if (!B.in(i)) outside.push_back(i);
+  a->tasks = comp_tasks(I, B, outside, scr);
+  a->ops.sites = B.sites;
+  init_arrays(a->ops, I.norb);
+  const auto t0 = std::chrono::steady_clock::now();
+  run_tasks(bd, a->ops, a->tasks, nullptr);
+  add_seconds(g_comp_build_seconds, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  return a;
+}
+
+std::vector<OpPtr> comp_ops_list(const CompOpsAhead& ahead) {
+  std::vector<OpPtr> v;
+  for (auto p : task_slots(ahead.tasks, ahead.ops))
+    if (*p) v.push_back(*p);
+  return v;
+}
+
+void add_rotated_comp_ops(const CompOpsAhead& ahead, const Rotation& R, SpinBlock& out, const OpRotator* rotator) {
+  if (R.old != ahead.si) throw std::runtime_error("add_rotated_comp_ops: operators built for another block");
+  const auto from = task_slots(ahead.tasks, ahead.ops);
+  const auto to = task_slots(ahead.tasks, out);
+  std::vector<const SparseOp*> src;
+  std::vector<OpPtr*> dst;
+  for (size_t t = 0; t < from.size(); ++t)
+    if (*from[t]) {
+      src.push_back(from[t]->get());
+      dst.push_back(to[t]);
+    }
+  const OpRotator host;
+  std::vector<OpPtr> rot = (rotator ? rotator : &host)->rotate(src, R);
+  for (size_t w = 0; w < dst.size(); ++w) *dst[w] = rot[w];
+  out.has_comp = true;
+}
 
 void add_rotated_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
                           const Screen& scr, const Rotation& R, SpinBlock& out, const OpRotator* rotator) {
@@ -745,16 +813,10 @@ void add_rotated_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBloc
     // all composite operators built first, then rotated as one batch
     const auto t0 = std::chrono::steady_clock::now();
     run_tasks(bd, out, tasks, nullptr);
-    g_comp_build_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    add_seconds(g_comp_build_seconds, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
     std::vector<OpPtr*> slot;
-    for (const auto& k : tasks) {
-      OpPtr* p = k.kind == 0   ? &out.cd[out.pidx(k.i, k.j, k.K)]
-                 : k.kind == 1 ? &out.cc[out.pidx(k.i, k.j, k.K)]
-                 : k.kind == 2 ? &out.cdc[out.pidx(k.i, k.j, k.K)]
-                 : k.kind == 3 ? &out.ddc[out.pidx(k.i, k.j, k.K)]
-                               : &out.ccd[k.i];
+    for (OpPtr* p : task_slots(tasks, out))
       if (*p) slot.push_back(p);
-    }
     std::vector<const SparseOp*> src;
     for (OpPtr* p : slot) src.push_back(p->get());
     std::vector<OpPtr> rot = rotator->rotate(src, R);

@@ -140,6 +140,10 @@ OpPtr rotate_op(const SparseOp& o, const Rotation& R);
 struct OpRotator {
   virtual ~OpRotator() = default;
   virtual std::vector<OpPtr> rotate(const std::vector<const SparseOp*>& ops, const Rotation& R) const;
+  // operators a later rotate() will take, handed over early (may be called
+  // from another host thread while the sweep's solve runs, never during a
+  // rotate(); the default keeps nothing)
+  virtual void preload(const std::vector<OpPtr>& ops) const {}
 };
 BlockPtr renormalise(const SpinBlock& B, const std::vector<std::vector<double>>& rot, const std::vector<int>& mkeep);
 BlockPtr renormalise(const SpinBlock& B, const Rotation& R, const OpRotator* rotator = nullptr);
@@ -148,6 +152,17 @@ BlockPtr renormalise(const SpinBlock& B, const Rotation& R, const OpRotator* rot
 // build_and_renormalise_transform (BaseOperator.C:341-459).
 void add_rotated_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
                           const Screen& scr, const Rotation& R, SpinBlock& out, const OpRotator* rotator = nullptr);
+// The same complementary operators built in B's own (composite) basis before
+// the rotation is known -- they depend only on B1, the dot and the integrals,
+// so the sweep builds them on a host thread while the GPU runs the Davidson
+// solve -- and then rotated into out as one batch.
+struct CompOpsAhead;
+std::shared_ptr<CompOpsAhead> build_comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1,
+                                             const SpinBlock& dot, const Screen& scr);
+void add_rotated_comp_ops(const CompOpsAhead& ahead, const Rotation& R, SpinBlock& out,
+                          const OpRotator* rotator = nullptr);
+// the operators built ahead (the ones add_rotated_comp_ops rotates)
+std::vector<OpPtr> comp_ops_list(const CompOpsAhead& ahead);
 // seconds spent building complementary operators before a batch rotation (accumulated)
 double comp_build_seconds();
 

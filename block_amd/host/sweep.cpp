@@ -10,6 +10,7 @@
 #include <numeric>
 #include <random>
 #include <sstream>
+#include <thread>
 
 namespace blk {
 
@@ -207,8 +208,11 @@ struct Solved {
 };
 
 // Solver::solve_wavefunction (solver.C:19-143): diagonal, guess, block Davidson
+// on_ready: called once the sigma functor and the diagonal are set up, just
+// before the Davidson iterations
 Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, const SigmaFactory& make_sigma,
-                        std::mt19937_64& rng, const std::vector<std::vector<double>>* guess, double ladder) {
+                        std::mt19937_64& rng, const std::vector<std::vector<double>>* guess, double ladder,
+                        const std::function<void()>& on_ready = nullptr) {
   std::normal_distribution<double> gauss(0.0, 1.0);
   const long N = sb.wave->size;
   if (N < P.nroots) throw std::runtime_error("run_dmrg: superblock smaller than the number of roots");
@@ -239,6 +243,7 @@ Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, c
     }
   }
   out.e = diag;
+  if (on_ready) on_ready();
   const int dmax = std::max(P.deflation_max, P.nroots + 2);
   if (!hfun->davidson(out.b, out.e, tol, true, P.deflation_min, dmax, &out.st, P.davidson_max_iter, ladder))
     block_davidson(out.b, out.e, tol, *hfun, true, P.deflation_min, dmax, &out.st, P.davidson_max_iter, ladder);
@@ -285,7 +290,7 @@ RenormTimes g_rt;
 
 BlockPtr truncate_block(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
                         const std::vector<std::vector<double>>& rho, int M, const Screen& scr, double* disc,
-                        const OpRotator* rotator) {
+                        const OpRotator* rotator, const CompOpsAhead* ahead = nullptr) {
   std::vector<std::vector<double>> rot;
   std::vector<int> mkeep;
   double t0 = now_s();
@@ -294,7 +299,12 @@ BlockPtr truncate_block(const Integrals& I, const SpinBlock& B, const SpinBlock&
   double t1 = now_s();
   BlockPtr nb = renormalise(B, Rot, rotator);
   double t2 = now_s();
-  if (!B.has_comp) add_rotated_comp_ops(I, B, B1, dot, scr, Rot, *nb, rotator);
+  if (!B.has_comp) {
+    if (ahead)
+      add_rotated_comp_ops(*ahead, Rot, *nb, rotator);
+    else
+      add_rotated_comp_ops(I, B, B1, dot, scr, Rot, *nb, rotator);
+  }
   g_rt.eig += t1 - t0;
   g_rt.rotate += t2 - t1;
   g_rt.comp += now_s() - t2;
@@ -306,6 +316,38 @@ BlockPtr truncate_block(const Integrals& I, const SpinBlock& B, const SpinBlock&
   nb->rotrec = rec;
   return nb;
 }
+
+// Host work run on a thread of its own beside the Davidson solve; the
+// destructor waits for it (an exception of the solve leaves no thread behind).
+class HostAhead {
+ public:
+  template <class F>
+  void start(F f) {
+    th_ = std::thread([this, f]() {
+      try {
+        f();
+      } catch (...) {
+        err_ = std::current_exception();
+      }
+    });
+  }
+  // wait; rethrows what the work threw
+  void join() {
+    if (th_.joinable()) th_.join();
+    if (err_) {
+      std::exception_ptr e = err_;
+      err_ = nullptr;
+      std::rethrow_exception(e);
+    }
+  }
+  ~HostAhead() {
+    if (th_.joinable()) th_.join();
+  }
+
+ private:
+  std::thread th_;
+  std::exception_ptr err_;
+};
 
 // target of a partial chain of kp of the k orbitals (infinite-system growth)
 Quantum partial_target(const Quantum& t, int kp, int k) {
@@ -432,6 +474,10 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
     double t0 = now_s();
     const int first = (sweep == 0) ? std::max(0, l0) : 0;
     const int nsteps = k - 3;
+    const bool use_ahead = P.host_ahead > 0 || (P.host_ahead < 0 && P.rotator);
+    BlockPtr next_R;  // the next step's R, built during this step's solve
+    const SpinBlock* next_env = nullptr;
+    int next_dot = -1;
     for (int step = first; step < nsteps; ++step) {
       // forward: sys = 0..l, dots l+1, l+2, env l+3..; backward mirrored
       int l = forward ? step : (k - 1 - step);
@@ -451,10 +497,41 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       sr.comp = true;
       double ts = now_s();
       BlockPtr L = build_sum_block(I, *sys, *dot1, target, nR, sl, scr);
-      BlockPtr R = build_sum_block(I, *env, *dot2, target, nL, sr, scr);
+      BlockPtr R = (next_R && next_env == env.get() && next_dot == d2)
+                       ? next_R
+                       : build_sum_block(I, *env, *dot2, target, nL, sr, scr);
+      next_R.reset();
+      next_env = nullptr;
       SuperBlock sb = make_superblock(I, L, R, target);
       const long N = sb.wave->size;
       rec.build_seconds += now_s() - ts;
+      // while the solve runs: L's complementary operators in its composite
+      // basis (renormalisation input, handed to the rotator, which may upload
+      // them beside the sigma kernels) and the next step's R = env' (x) dot
+      // (started once the solve's own host set-up -- work list, operator
+      // upload through the staging -- is done, so as not to compete with it)
+      std::shared_ptr<CompOpsAhead> compL;
+      HostAhead ahead;
+      std::function<void()> start_ahead;
+      if (use_ahead) {
+        const int ln = forward ? l + 1 : l - 1;
+        const int d2n = forward ? ln + 2 : ln - 2;
+        BlockPtr envn = step + 1 < nsteps ? (forward ? bwd[ln + 3] : fwd[ln - 3]) : nullptr;
+        start_ahead = [&, envn, d2n]() {
+          ahead.start([&, envn, d2n]() {
+            if (!L->has_comp) {
+              compL = build_comp_ops(I, *L, *sys, *dot1, scr);
+              if (P.rotator) P.rotator->preload(comp_ops_list(*compL));
+            }
+            if (envn) {
+              auto dn = make_dot(I, d2n, scr);
+              next_R = build_sum_block(I, *envn, *dn, target, nL + 1, sr, scr);
+              next_env = envn.get();
+              next_dot = d2n;
+            }
+          });
+        };
+      }
       ts = now_s();
       // guess: the previous wavefunction in this basis (same superblock after
       // the warm-up, the halves swapped at a turn, else the two-site transform)
@@ -476,7 +553,7 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       }
       rec.guess_seconds += now_s() - ts;
       Solved sol = solve_superblock(sb, P, S->davidson_tol, make_sigma, rng, have_guess ? &guess : nullptr,
-                                    sweep >= P.ladder_from_sweep ? P.ladder : 0.0);
+                                    sweep >= P.ladder_from_sweep ? P.ladder : 0.0, start_ahead);
       rec.guessed_steps += sol.guessed;
       rec.davidson_seconds += now_s() - ts - sol.st.sigma_seconds;
       rec.setup_seconds += sol.st.setup_seconds;
@@ -488,11 +565,15 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       rec.sigma_flops += sol.st.sigma_calls * sigma_flops(sb);
       for (int r = 0; r < P.nroots; ++r) rec.energies[r] = std::min(rec.energies[r], sol.e[r]);
       const RenormTimes rt0 = g_rt;
+      double twait = now_s();
+      ahead.join();
+      twait = now_s() - twait;
       double trho = now_s();
       auto rho = density_matrix(sb, sol, weights, true, S->noise, rng);
       trho = now_s() - trho;
       double disc = 0.0;
-      BlockPtr nsys = truncate_block(I, *L, *sys, *dot1, rho, S->M, scr, &disc, P.rotator.get());
+      BlockPtr nsys = truncate_block(I, *L, *sys, *dot1, rho, S->M, scr, &disc, P.rotator.get(), compL.get());
+      compL.reset();
       rec.max_discarded = std::max(rec.max_discarded, disc);
       rec.renorm_seconds += now_s() - ts;
       if (forward)
@@ -507,10 +588,10 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       }
       if (P.verbose > 1)
         std::fprintf(stderr,
-                     "  sweep %d step %d  N=%ld secL=%d secR=%d E0=%.12f  iters=%d%s  disc=%.2e  renorm: rho %.3f "
-                     "eig %.3f rotate %.3f comp %.3f\n",
+                     "  sweep %d step %d  N=%ld secL=%d secR=%d E0=%.12f  iters=%d%s  disc=%.2e  renorm: wait %.3f "
+                     "rho %.3f eig %.3f rotate %.3f comp %.3f\n",
                      sweep, step, N, L->si->nsec(), R->si->nsec(), sol.e[0], sol.st.iterations,
-                     sol.guessed ? " (guess)" : "", disc, trho, g_rt.eig - rt0.eig, g_rt.rotate - rt0.rotate,
+                     sol.guessed ? " (guess)" : "", disc, twait, trho, g_rt.eig - rt0.eig, g_rt.rotate - rt0.rotate,
                      g_rt.comp - rt0.comp);
     }
     // the sweep ends at the far edge: the last block is the single-site dot

@@ -36,6 +36,11 @@ struct DmrgParams {
   std::shared_ptr<const OpRotator> rotator;  // operator rotations of the renormalisation (null: host threads)
   std::string checkpoint;  // file the sweep state is written to after every sweep (empty: none)
   std::string resume;      // sweep-state file to continue from instead of the warm-up (empty: none)
+  // build host-side operators that do not depend on the current solve (the
+  // new system block's complementary operators, the next step's environment
+  // block) on a host thread during the Davidson solve: 1 on, 0 off, -1 on
+  // when a rotator is set (GPU runs, whose solve leaves the host idle)
+  int host_ahead = -1;
   uint64_t seed = 12345;
   int verbose = 0;
 };

@@ -136,7 +136,7 @@ void* orc_exact_superblock(void* ints, int nelec, int twoS, int irrep, int nL) {
 // blkh_dmrg_gpu (sweep_out rows of kSweepCols)
 int orc_dmrg_cpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
                  const double* sched, int max_sweeps, double sweep_tol, int guess, const char* checkpoint,
-                 const char* resume, int nthreads, int verbose, double* energies_out, double* sweep_out) {
+                 const char* resume, int ahead, int nthreads, int verbose, double* energies_out, double* sweep_out) {
   return guarded([&]() {
     const Integrals& I = *static_cast<Integrals*>(ints);
     DmrgParams P;
@@ -153,6 +153,7 @@ int orc_dmrg_cpu(void* ints, int nelec, int twoS, int irrep, int nroots, const d
     P.transform_guess = guess != 0;
     if (checkpoint) P.checkpoint = checkpoint;
     if (resume) P.resume = resume;
+    P.host_ahead = ahead;
     auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {
       return std::unique_ptr<DavidsonFunctor>(new OracleFunctor(sb, nthreads));
     });

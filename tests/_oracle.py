@@ -23,7 +23,7 @@ for name, res, args in [
     ("orc_exact_superblock", _vp, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("orc_dmrg_cpu", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int,
                                     _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_char_p,
-                                    ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+                                    ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
 ]:
     f = getattr(lib, name)
     f.restype, f.argtypes = res, args
@@ -93,7 +93,7 @@ def dense_h(sb, nthreads=4):
 
 
 def dmrg_cpu(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=None, max_sweeps=20,
-             sweep_tol=1e-8, nthreads=4, verbose=0, guess=True, checkpoint=None, resume=None):
+             sweep_tol=1e-8, nthreads=4, verbose=0, guess=True, checkpoint=None, resume=None, ahead=-1):
     nelec = ints.nelec if nelec is None else nelec
     irrep = ints.isym if irrep is None else irrep
     w = np.ascontiguousarray(weights if weights is not None else np.full(nroots, 1.0 / nroots), dtype=np.float64)
@@ -103,5 +103,6 @@ def dmrg_cpu(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, sched
     sw = np.zeros(nc * max_sweeps)
     n = _chk(lib.orc_dmrg_cpu(ints._h, nelec, twoS, irrep, nroots, _p(w), len(schedule), _p(sched), max_sweeps,
                               sweep_tol, int(guess), None if checkpoint is None else str(checkpoint).encode(),
-                              None if resume is None else str(resume).encode(), nthreads, verbose, _p(e), _p(sw)))
+                              None if resume is None else str(resume).encode(), int(ahead), nthreads, verbose, _p(e),
+                              _p(sw)))
     return e, sw[: nc * n].reshape(n, nc)

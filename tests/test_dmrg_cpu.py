@@ -25,3 +25,20 @@ def test_hubbard_chain_12_sites_M100():
     e, sweeps = _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=14, sweep_tol=1e-9)
     assert e[0] >= -6.526243384454524 - 1e-9
     assert e[0] == pytest.approx(-6.526243384454524, abs=1e-6)
+
+
+@pytest.mark.parametrize("nroots", [1, 2])
+def test_host_ahead_same_sweeps(golden, nroots):
+    # complementary operators and the next environment block built on a host
+    # thread beside the Davidson solve (the GPU runs' default) are the same
+    # operators: bitwise the same energies, truncation and Davidson work (one
+    # sigma thread, so that the sigma's own sums are in a fixed order)
+    ints = block_amd.Integrals(os.path.join(golden, "hubbard8_ring.FCIDUMP"))
+    sched = [(0, 20, 1e-7, 1e-5), (2, 30, 1e-10, 0.0)]
+    (e0, s0), (e1, s1) = (_oracle.dmrg_cpu(ints, nroots=nroots, schedule=sched, max_sweeps=4, sweep_tol=0.0,
+                                           nthreads=1, ahead=a) for a in (0, 1))
+    assert (e0 == e1).all()
+    cols = block_amd.SWEEP_COLS
+    for c in ("E0", "discarded", "sigma_calls", "guessed_steps"):
+        i = cols.index(c)
+        assert (s0[:, i] == s1[:, i]).all(), c
