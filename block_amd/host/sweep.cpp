@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <map>
 #include <numeric>
+#include <omp.h>
 #include <random>
 #include <sstream>
 #include <thread>
@@ -519,6 +520,8 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
         BlockPtr envn = step + 1 < nsteps ? (forward ? bwd[ln + 3] : fwd[ln - 3]) : nullptr;
         start_ahead = [&, envn, d2n]() {
           ahead.start([&, envn, d2n]() {
+            // leave two cores to the solve's own host thread
+            omp_set_num_threads(std::max(1, omp_get_num_procs() - 2));
             if (!L->has_comp) {
               compL = build_comp_ops(I, *L, *sys, *dot1, scr);
               if (P.rotator) P.rotator->preload(comp_ops_list(*compL));
