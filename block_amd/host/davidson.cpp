@@ -46,6 +46,9 @@ void block_davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_
   const double levelshift = 0.0;
   bool split = ladder > 0.0 && h.set_precision(1);
   const int nroots = (int)b.size();
+  // deflation keeps at least every root's vector
+  deflation_min = std::max(deflation_min, nroots);
+  deflation_max = std::max(deflation_max, deflation_min + 1);
   // orthonormalise the guesses (linear.C:190-200)
   for (int i = 0; i < nroots; ++i) {
     for (int j = 0; j < i; ++j) daxpy(-ddot(b[j], b[i]), b[j], b[i]);

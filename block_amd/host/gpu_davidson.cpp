@@ -34,7 +34,10 @@ void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std
   bsg_context* ctx = h.ctx();
   const int64_t N = h.wave_size();
   const int nroots = (int)b.size();
-  const int cap = std::max(deflation_max, nroots) + 1;
+  // deflation keeps at least every root's vector
+  deflation_min = std::max(deflation_min, nroots);
+  deflation_max = std::max(deflation_max, deflation_min + 1);
+  const int cap = deflation_max + 1;
   if (cap > 32) throw std::runtime_error("block_davidson_device: subspace larger than 32 vectors");
   const double levelshift = 0.0;
   DevBuf B(ctx, (size_t)cap * N), S(ctx, (size_t)cap * N), T(ctx, (size_t)cap * N), R(ctx, N), Wk(ctx, N),

@@ -245,9 +245,12 @@ Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, c
   }
   out.e = diag;
   if (on_ready) on_ready();
-  const int dmax = std::max(P.deflation_max, P.nroots + 2);
-  if (!hfun->davidson(out.b, out.e, tol, true, P.deflation_min, dmax, &out.st, P.davidson_max_iter, ladder))
-    block_davidson(out.b, out.e, tol, *hfun, true, P.deflation_min, dmax, &out.st, P.davidson_max_iter, ladder);
+  // with several roots the subspace never deflates below nroots vectors and
+  // grows to nroots + 20 (the nroots keyword, input.C:882-885)
+  const int dmin = std::max(P.deflation_min, P.nroots);
+  const int dmax = P.nroots > 1 ? std::max(P.deflation_max, P.nroots + 20) : std::max(P.deflation_max, dmin + 1);
+  if (!hfun->davidson(out.b, out.e, tol, true, dmin, dmax, &out.st, P.davidson_max_iter, ladder))
+    block_davidson(out.b, out.e, tol, *hfun, true, dmin, dmax, &out.st, P.davidson_max_iter, ladder);
   out.e.resize(P.nroots);
   out.st.setup_seconds = t1 - t0;
   out.st.diag_seconds = t2 - t1;

@@ -42,3 +42,18 @@ def test_host_ahead_same_sweeps(golden, nroots):
     for c in ("E0", "discarded", "sigma_calls", "guessed_steps"):
         i = cols.index(c)
         assert (s0[:, i] == s1[:, i]).all(), c
+
+
+def test_state_averaged_four_roots():
+    # four state-averaged roots (weights 1/4, the batched configuration's
+    # Davidson): untruncated halves of an 8-orbital synthetic system, so the
+    # sweep energies are the four lowest S=0 determinant-FCI energies.  With
+    # the subspace deflated below nroots vectors (deflation_min_size must be
+    # raised to nroots, input.C:882-885) roots 3 and 4 come out wrong.
+    from helpers import random_fcidump
+    ints = random_fcidump(8, 7)
+    e_fci, s2 = _oracle.fci(ints, 8, nroots=10, penalty=3.0)
+    singlets = sorted(e for e, s in zip(e_fci, s2) if s < 0.5)[:4]
+    sched = [(0, 200, 1e-9, 1e-5), (2, 200, 1e-11, 0.0)]
+    e, sw = _oracle.dmrg_cpu(ints, nroots=4, schedule=sched, max_sweeps=6, sweep_tol=0.0)
+    assert e == pytest.approx(singlets, abs=1e-8)

@@ -61,3 +61,21 @@ def test_gpu_dmrg_checkpoint_resume(tmp_path):
     assert sw.shape == sw_all.shape
     assert abs(e[0] - e_all[0]) <= 1e-10
     assert max(abs(sw[:, col] - sw_all[:, col])) <= 1e-10
+
+
+def test_gpu_dmrg_four_roots():
+    """Four state-averaged roots with the GPU sigma (the batched
+    configuration's Davidson): the CPU-oracle sweep energies to 1e-10 and the
+    four lowest S=0 FCI energies; the precision ladder within 1e-8."""
+    print("[gpu] dmrg four roots, 8-orbital synthetic", flush=True)
+    from helpers import random_fcidump
+    ints = random_fcidump(8, 7)
+    e_fci, s2 = _oracle.fci(ints, 8, nroots=10, penalty=3.0)
+    singlets = sorted(e for e, s in zip(e_fci, s2) if s < 0.5)[:4]
+    sched = [(0, 200, 1e-9, 1e-5), (2, 200, 1e-11, 0.0)]
+    e, sw = block_amd.dmrg(ints, nroots=4, schedule=sched, max_sweeps=6, sweep_tol=0.0)
+    e_cpu, _ = _oracle.dmrg_cpu(ints, nroots=4, schedule=sched, max_sweeps=6, sweep_tol=0.0)
+    assert e == pytest.approx(e_cpu, abs=1e-10)
+    assert e == pytest.approx(singlets, abs=1e-8)
+    e_l, _ = block_amd.dmrg(ints, nroots=4, schedule=sched, max_sweeps=6, sweep_tol=0.0, ladder=1e-5)
+    assert e_l == pytest.approx(e, abs=1e-8)
