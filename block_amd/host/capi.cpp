@@ -243,6 +243,7 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
     P.host_ahead = ahead;
     GpuContext ctx(device);
     P.rotator = std::make_shared<GpuRotator>(ctx);
+    P.preload_block = [&ctx](const SpinBlock& b) { preload_ops(ctx, 1, block_ops(b)); };
     auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {
       return std::unique_ptr<DavidsonFunctor>(new GpuFunctor(ctx, sb));
     });

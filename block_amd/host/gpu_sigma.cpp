@@ -13,6 +13,9 @@ void bsg_check(int rc, const char* what) {
 GpuContext::GpuContext(int dev) : device(dev) { bsg_check(bsg_init(&ctx, dev), "bsg_init"); }
 GpuContext::~GpuContext() {
   if (!ctx) return;
+  for (auto& p : pre)
+    if (p.buf && side) bsg_mfree(side->ctx, p.buf);
+  side.reset();
   if (arena) bsg_mfree(ctx, arena);
   if (work) bsg_mfree(ctx, work);
   if (staging) bsg_host_free(ctx, staging);
@@ -47,6 +50,22 @@ namespace {
 // copied; device memory between the operators is left as it is.  n: doubles
 // spanned by the offsets.
 void upload_ops(GpuContext& g, const std::vector<std::pair<const SparseOp*, int64_t>>& ops, double* dst, size_t n) {
+  // preloaded operators: device copies
+  std::vector<std::pair<const SparseOp*, int64_t>> rest;
+  for (const auto& kv : ops) {
+    bool done = false;
+    for (auto& p : g.pre) {
+      auto f = p.at.find(kv.first);
+      if (f == p.at.end()) continue;
+      bsg_check(bsg_copy(g.ctx, p.buf + f->second, dst + kv.second, (int64_t)kv.first->data.size()),
+                "preloaded operator copy");
+      p.at.erase(f);
+      if (p.at.empty()) p.held.clear();
+      done = true;
+      break;
+    }
+    if (!done) rest.push_back(kv);
+  }
   double* staging = g.ensure_staging(std::max<size_t>(1, n));
   struct Piece {
     const double* src;
@@ -54,7 +73,7 @@ void upload_ops(GpuContext& g, const std::vector<std::pair<const SparseOp*, int6
   };
   std::vector<Piece> pieces;
   constexpr int64_t PIECE = 1 << 19;  // doubles (4 MB)
-  for (const auto& kv : ops)
+  for (const auto& kv : rest)
     for (int64_t x = 0; x < (int64_t)kv.first->data.size(); x += PIECE)
       pieces.push_back({kv.first->data.data() + x, kv.second + x, std::min<int64_t>(PIECE, kv.first->data.size() - x)});
   std::sort(pieces.begin(), pieces.end(), [](const Piece& a, const Piece& b) { return a.off < b.off; });
@@ -265,19 +284,37 @@ std::vector<double> GpuSigma::diagonal() {
   return out;
 }
 
-void GpuRotator::preload(const std::vector<OpPtr>& ops) const {
-  preloaded_.clear();
-  held_ = ops;
-  if (!side_) side_.reset(new GpuContext(g_.device));
+void preload_ops(GpuContext& g, int slot, const std::vector<OpPtr>& ops) {
+  GpuContext::Preload& p = g.pre[slot];
+  p.at.clear();
+  p.held = ops;
+  if (!g.side) g.side.reset(new GpuContext(g.device));
   std::vector<std::pair<const SparseOp*, int64_t>> at;
   int64_t n = 0;
   for (const OpPtr& o : ops) {
     at.push_back({o.get(), n});
     n += (int64_t)o->data.size();
   }
-  double* dst = side_->ensure_work((size_t)std::max<int64_t>(1, n));
-  upload_ops(*side_, at, dst, (size_t)n);
-  for (const auto& x : at) preloaded_[x.first] = x.second;
+  if (p.cap < (size_t)n) {
+    if (p.buf) bsg_check(bsg_mfree(g.side->ctx, p.buf), "preload free");
+    p.buf = nullptr;
+    p.cap = 0;
+    const size_t cap = (size_t)n + (size_t)n / 4;
+    bsg_check(bsg_malloc(g.side->ctx, sizeof(double) * cap, (void**)&p.buf), "preload buffer");
+    p.cap = cap;
+  }
+  upload_ops(*g.side, at, p.buf, (size_t)n);
+  for (const auto& x : at) p.at[x.first] = x.second;
+}
+
+std::vector<OpPtr> block_ops(const SpinBlock& b) {
+  std::vector<OpPtr> v;
+  for (const OpPtr& o : {b.ident, b.ham})
+    if (o) v.push_back(o);
+  for (const auto* l : {&b.cre, &b.ccd, &b.cd, &b.cc, &b.cdc, &b.ddc})
+    for (const OpPtr& o : *l)
+      if (o) v.push_back(o);
+  return v;
 }
 
 std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, const Rotation& R) const {
@@ -336,23 +373,16 @@ std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, c
   }
   double* base = g_.ensure_work((size_t)(total + usize));
   double* ubuf = base + total;
-  // operators still in the arena or preloaded are copied on the device, the
-  // rest uploaded
+  // operators still in the arena are copied on the device, the rest uploaded
+  // (preloaded ones copied on the device by upload_ops)
   std::vector<std::pair<const SparseOp*, int64_t>> host;
   for (const auto& x : src) {
     auto f = g_.resident.find(x.first);
-    auto p = preloaded_.find(x.first);
     if (f != g_.resident.end() && g_.arena && !g_.arena_leased)
       bsg_check(bsg_copy(g_.ctx, g_.arena + f->second, base + x.second, (int64_t)x.first->data.size()), "operator copy");
-    else if (p != preloaded_.end()) {
-      bsg_check(bsg_copy(g_.ctx, side_->work + p->second, base + x.second, (int64_t)x.first->data.size()),
-                "preloaded operator copy");
-      preloaded_.erase(p);
-    } else {
+    else
       host.push_back(x);
-    }
   }
-  if (preloaded_.empty()) held_.clear();
   upload_ops(g_, host, base, (size_t)soff);
   {
     std::vector<double> u((size_t)std::max<int64_t>(1, usize));

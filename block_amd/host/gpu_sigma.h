@@ -34,6 +34,20 @@ struct GpuContext {
   // next upload (the sweep rotates the superblock's operators right after
   // its Davidson solve, while they are still in the arena)
   std::unordered_map<const SparseOp*, int64_t> resident;
+  // Operators uploaded ahead of use by another host thread, on a second
+  // context (its own stream, staging and buffers) so the copies run beside
+  // the sigma kernels: slot 0 for the next rotation, slot 1 for the next
+  // sigma arena.  An upload (upload_ops) that meets a preloaded operator
+  // copies it on the device instead.  Preloaded operators are kept alive
+  // until used, so no other operator can take one's address.
+  struct Preload {
+    std::vector<OpPtr> held;
+    std::unordered_map<const SparseOp*, int64_t> at;  // offsets in buf
+    double* buf = nullptr;
+    size_t cap = 0;  // doubles
+  };
+  Preload pre[2];
+  std::unique_ptr<GpuContext> side;  // created by the first preload
   explicit GpuContext(int device = 0);
   // grow-only buffers (contents not kept when they grow)
   double* ensure_staging(size_t n);
@@ -42,6 +56,13 @@ struct GpuContext {
   GpuContext(const GpuContext&) = delete;
   GpuContext& operator=(const GpuContext&) = delete;
 };
+
+// Upload ops to slot `slot` of g's preload store through g's side context
+// (replaces what the slot held).  Called from a host thread other than the
+// one using g, never while an upload of g runs.
+void preload_ops(GpuContext& g, int slot, const std::vector<OpPtr>& ops);
+// every operator of a block
+std::vector<OpPtr> block_ops(const SpinBlock& b);
 
 struct WorkList {
   std::vector<bsg_vblock_t> vb;    // V blocks with their pair ranges
@@ -107,24 +128,18 @@ class GpuSigma {
 // rotation matrices as the operator arena and the old operator blocks as the
 // wavefunction.  Operators are uploaded through page-locked staging and the
 // rotated ones copied back.
-// preload() uploads operators ahead of their rotation on a second device
-// context (its own stream, staging and buffer), so the copy runs beside the
-// sigma kernels of the solve; rotate() then copies them on the device.  A
-// preload is consumed by the next rotate() (the sweep joins the preloading
-// thread before it rotates).
+// preload() uploads operators ahead of their rotation into slot 0 of the
+// context's preload store, so the copy runs beside the sigma kernels of the
+// solve; rotate() then copies them on the device (the sweep joins the
+// preloading thread before it rotates).
 class GpuRotator : public OpRotator {
  public:
   explicit GpuRotator(GpuContext& g) : g_(g) {}
   std::vector<OpPtr> rotate(const std::vector<const SparseOp*>& ops, const Rotation& R) const override;
-  void preload(const std::vector<OpPtr>& ops) const override;
+  void preload(const std::vector<OpPtr>& ops) const override { preload_ops(g_, 0, ops); }
 
  private:
   GpuContext& g_;
-  mutable std::unique_ptr<GpuContext> side_;  // created on the first preload
-  // preloaded operators not yet rotated (kept alive, so no other operator can
-  // take one's address) and their offsets in side_->work
-  mutable std::vector<OpPtr> held_;
-  mutable std::unordered_map<const SparseOp*, int64_t> preloaded_;
 };
 
 // Linear::block_davidson on device vectors with the sigma of h (same control

@@ -534,6 +534,7 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
               next_R = build_sum_block(I, *envn, *dn, target, nL + 1, sr, scr);
               next_env = envn.get();
               next_dot = d2n;
+              if (P.preload_block) P.preload_block(*next_R);
             }
           });
         };

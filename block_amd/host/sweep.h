@@ -41,6 +41,9 @@ struct DmrgParams {
   // block) on a host thread during the Davidson solve: 1 on, 0 off, -1 on
   // when a rotator is set (GPU runs, whose solve leaves the host idle)
   int host_ahead = -1;
+  // called on that thread with the next step's environment-side block once
+  // it is built (the GPU runs upload its operators ahead of the next solve)
+  std::function<void(const SpinBlock&)> preload_block;
   uint64_t seed = 12345;
   int verbose = 0;
 };
