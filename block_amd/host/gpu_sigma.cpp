@@ -107,13 +107,17 @@ WorkList compile_worklist(const SuperBlock& sb) {
   const SparseOp* idL = sb.L->ident.get();
   const SparseOp* idR = sb.R->ident.get();
 
-  // operator arena: every distinct operator the term list references
-  for (const auto& t : sb.terms)
-    for (const SparseOp* o : {t.a.op, t.b.op})
+  // operator arena: every distinct operator the term list references, the
+  // L operators first, then the R ones (each half one contiguous run, so an
+  // upload that skips one half's preloaded operators stays in large batches)
+  for (int side = 0; side < 2; ++side)
+    for (const auto& t : sb.terms) {
+      const SparseOp* o = side ? t.b.op : t.a.op;
       if (!wl.arena_off.count(o)) {
         wl.arena_off[o] = wl.arena_size;
         wl.arena_size += (int64_t)o->data.size();
       }
+    }
 
   // pairs per V block (l, r), in term order and, within a term, in C block
   // order; V blocks are independent, so they are built in parallel
