@@ -240,6 +240,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision timing")
     ap.add_argument("--no-sweep", action="store_true", help="skip the sweep-time leg")
+    ap.add_argument("--sweep-only", action="store_true", help="only the sweep-time leg (its JSON object)")
     ap.add_argument("--wilson", action="store_true", help="Wilson-warmed blocks instead of the DMRG warm-up")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run a device copy of N doubles (HBM counter calibration)")
@@ -267,6 +268,9 @@ def main():
     fcidump = os.path.join(tmp, f"{args.config}.FCIDUMP")
     gen.make_config_fcidump(args.config, fcidump)
     ints = block_amd.Integrals(fcidump)
+    if args.sweep_only:
+        print(json.dumps({"sweep": sweep_leg(ints, cfg, M)}))
+        return 0
     t0 = time.time()
     if args.wilson:
         sb = block_amd.SuperBlock(ints, cfg["nelec"], 0, 0, nsys, M)

@@ -1171,6 +1171,12 @@ __global__ void k_copy(const double* __restrict__ x, double* __restrict__ y, int
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] = x[i];
 }
+// one workgroup per chunk of at most COPY_CHUNK doubles
+constexpr int64_t COPY_CHUNK = 16384;
+__global__ __launch_bounds__(256) void k_copy_chunks(const bsg_copy_seg_t* __restrict__ chunks) {
+  const bsg_copy_seg_t c = chunks[blockIdx.x];
+  for (int64_t i = threadIdx.x; i < c.n; i += 256) c.dst[i] = c.src[i];
+}
 
 // Gram matrix of two vector sets (m <= GMAX vectors of length n, stride ld):
 // per-block partial sums G[i m + j] = sum_x X_i[x] Y_j[x] over the block's
@@ -1280,6 +1286,8 @@ struct bsg_context {
   hipEvent_t fork = nullptr, join = nullptr;
   double* red = nullptr;  // RMAXB partials + 1 result
   double* gpart = nullptr;  // Gram partials (GBLK GMAX^2) + result (GMAX^2) + coefficients (GMAX^2)
+  bsg_copy_seg_t* copy_list = nullptr;  // chunk list of bsg_copy_segments (grown on demand)
+  size_t copy_cap = 0;
 };
 
 struct PlanChunk {
@@ -1363,6 +1371,7 @@ int bsg_free(bsg_context* ctx) {
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipFree(ctx->red));
+  if (ctx->copy_list) HIPCHK(hipFree(ctx->copy_list));
   HIPCHK(hipStreamSynchronize(ctx->side));
   HIPCHK(hipFree(ctx->gpart));
   HIPCHK(hipEventDestroy(ctx->fork));
@@ -1955,6 +1964,33 @@ int bsg_copy(bsg_context* ctx, const double* x, double* y, int64_t n) {
   if (n == 0) return BSG_OK;
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, ctx->stream, x, y, n);
   HIPCHK(hipGetLastError());
+  return BSG_OK;
+}
+int bsg_copy_segments(bsg_context* ctx, const bsg_copy_seg_t* segs, int32_t n_segs) {
+  if (!ctx || n_segs < 0 || (n_segs > 0 && !segs)) return fail(BSG_ERR_ARG, "bsg_copy_segments: bad args");
+  std::vector<bsg_copy_seg_t> chunks;
+  for (int32_t i = 0; i < n_segs; ++i) {
+    const bsg_copy_seg_t& s = segs[i];
+    if (s.n < 0 || (s.n > 0 && (!s.src || !s.dst))) return fail(BSG_ERR_ARG, "bsg_copy_segments: bad segment");
+    for (int64_t x = 0; x < s.n; x += COPY_CHUNK) chunks.push_back({s.src + x, s.dst + x, std::min(COPY_CHUNK, s.n - x)});
+  }
+  if (chunks.empty()) return BSG_OK;
+  if (chunks.size() > (size_t)0x7fffffff) return fail(BSG_ERR_ARG, "bsg_copy_segments: too many chunks");
+  HIPCHK(hipSetDevice(ctx->device));
+  // every call returns with its copies done, so no queued work reads the list
+  if (ctx->copy_cap < chunks.size()) {
+    if (ctx->copy_list) HIPCHK(hipFree(ctx->copy_list));
+    ctx->copy_list = nullptr;
+    ctx->copy_cap = 0;
+    const size_t cap = chunks.size() + chunks.size() / 2;
+    HIPCHK(hipMalloc((void**)&ctx->copy_list, sizeof(bsg_copy_seg_t) * cap));
+    ctx->copy_cap = cap;
+  }
+  HIPCHK(hipMemcpyAsync(ctx->copy_list, chunks.data(), sizeof(bsg_copy_seg_t) * chunks.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+  hipLaunchKernelGGL(k_copy_chunks, dim3((unsigned)chunks.size()), dim3(256), 0, ctx->stream, ctx->copy_list);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return BSG_OK;
 }
 int bsg_scal(bsg_context* ctx, double a, double* x, int64_t n) {

@@ -50,22 +50,24 @@ namespace {
 // copied; device memory between the operators is left as it is.  n: doubles
 // spanned by the offsets.
 void upload_ops(GpuContext& g, const std::vector<std::pair<const SparseOp*, int64_t>>& ops, double* dst, size_t n) {
-  // preloaded operators: device copies
+  // preloaded operators: device copies, in one launch
   std::vector<std::pair<const SparseOp*, int64_t>> rest;
+  std::vector<bsg_copy_seg_t> segs;
   for (const auto& kv : ops) {
     bool done = false;
     for (auto& p : g.pre) {
       auto f = p.at.find(kv.first);
       if (f == p.at.end()) continue;
-      bsg_check(bsg_copy(g.ctx, p.buf + f->second, dst + kv.second, (int64_t)kv.first->data.size()),
-                "preloaded operator copy");
+      segs.push_back({p.buf + f->second, dst + kv.second, (int64_t)kv.first->data.size()});
       p.at.erase(f);
-      if (p.at.empty()) p.held.clear();
       done = true;
       break;
     }
     if (!done) rest.push_back(kv);
   }
+  bsg_check(bsg_copy_segments(g.ctx, segs.data(), (int32_t)segs.size()), "preloaded operator copy");
+  for (auto& p : g.pre)
+    if (p.at.empty()) p.held.clear();
   double* staging = g.ensure_staging(std::max<size_t>(1, n));
   struct Piece {
     const double* src;
@@ -380,13 +382,15 @@ std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, c
   // operators still in the arena are copied on the device, the rest uploaded
   // (preloaded ones copied on the device by upload_ops)
   std::vector<std::pair<const SparseOp*, int64_t>> host;
+  std::vector<bsg_copy_seg_t> segs;
   for (const auto& x : src) {
     auto f = g_.resident.find(x.first);
     if (f != g_.resident.end() && g_.arena && !g_.arena_leased)
-      bsg_check(bsg_copy(g_.ctx, g_.arena + f->second, base + x.second, (int64_t)x.first->data.size()), "operator copy");
+      segs.push_back({g_.arena + f->second, base + x.second, (int64_t)x.first->data.size()});
     else
       host.push_back(x);
   }
+  bsg_check(bsg_copy_segments(g_.ctx, segs.data(), (int32_t)segs.size()), "operator copy");
   upload_ops(g_, host, base, (size_t)soff);
   {
     std::vector<double> u((size_t)std::max<int64_t>(1, usize));

@@ -148,6 +148,21 @@ int bsg_axpy(bsg_context* ctx, double a, const double* x, double* y, int64_t n);
 int bsg_scal(bsg_context* ctx, double a, double* x, int64_t n);
 /* y = x (the Wavefunction copies of linear.C:241-244) */
 int bsg_copy(bsg_context* ctx, const double* x, double* y, int64_t n);
+
+/* Many device-to-device copies in one launch: seg[i].dst[0..n) =
+   seg[i].src[0..n) (device pointers; segments must not overlap).  No
+   counterpart in the reference: the sweep moves operators between device
+   buffers one operator block list at a time (operators still in the arena
+   to the rotation buffer, operators uploaded ahead of use to the arena),
+   hundreds per step, which as single copies cost a launch each.  Runs on
+   the context's stream after the work queued there and returns when the
+   copies are done. */
+typedef struct {
+  const double* src;
+  double* dst;
+  int64_t n;
+} bsg_copy_seg_t;
+int bsg_copy_segments(bsg_context* ctx, const bsg_copy_seg_t* segs_host, int32_t n_segs);
 /* Subspace matrix of the block Davidson (linear.C:265-270): out_host[i*m+j]
  * = <X_i, Y_j> for m <= 32 device vectors of length n stored with stride ld. */
 int bsg_gram(bsg_context* ctx, const double* X, const double* Y, int32_t m, int64_t n, int64_t ld, double* out_host);

@@ -75,3 +75,48 @@ def test_split_sigma(name, make):
     assert abs(cn @ h(cn) - cn @ (ref / np.linalg.norm(c))) <= SPLIT_TOL * np.linalg.norm(ref) / np.linalg.norm(c)
     h.precision = block_amd.FP64
     assert np.array_equal(h(c), f64)
+
+
+def test_copy_segments():
+    """bsg_copy_segments: many device copies (segments longer than one
+    chunk, empty ones, reversed order) in one launch, through the C-ABI."""
+    import ctypes
+    print("[gpu] bsg_copy_segments", flush=True)
+    lib = block_amd.sigma_lib
+    vp = ctypes.c_void_p
+
+    class Seg(ctypes.Structure):
+        _fields_ = [("src", vp), ("dst", vp), ("n", ctypes.c_int64)]
+
+    lib.bsg_init.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+    lib.bsg_malloc.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(vp)]
+    lib.bsg_mfree.argtypes = [vp, vp]
+    lib.bsg_free.argtypes = [vp]
+    lib.bsg_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_size_t]
+    lib.bsg_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_size_t]
+    lib.bsg_memset_zero.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.bsg_copy_segments.argtypes = [vp, ctypes.POINTER(Seg), ctypes.c_int32]
+    n = 100_000
+    x = rng_vec(n, 5)
+    ctx, dx, dy = vp(), vp(), vp()
+    assert lib.bsg_init(ctypes.byref(ctx), 0) == 0
+    try:
+        assert lib.bsg_malloc(ctx, 8 * n, ctypes.byref(dx)) == 0
+        assert lib.bsg_malloc(ctx, 8 * n, ctypes.byref(dy)) == 0
+        assert lib.bsg_memcpy_h2d(ctx, dx, x.ctypes.data, 8 * n) == 0
+        assert lib.bsg_memset_zero(ctx, dy, 8 * n) == 0
+        # (src offset, dst offset, length): 40000 spans three 16384-double chunks
+        cuts = [(0, 50_000, 40_000), (40_000, 0, 7), (60_000, 10, 0), (90_000, 20, 10_000), (40_007, 95_000, 5_000)]
+        segs = (Seg * len(cuts))(*[Seg(dx.value + 8 * s, dy.value + 8 * d, m) for s, d, m in reversed(cuts)])
+        assert lib.bsg_copy_segments(ctx, segs, len(cuts)) == 0
+        y = np.empty(n)
+        assert lib.bsg_memcpy_d2h(ctx, y.ctypes.data, dy, 8 * n) == 0
+        want = np.zeros(n)
+        for s, d, m in cuts:
+            want[d:d + m] = x[s:s + m]
+        assert np.array_equal(y, want)
+    finally:
+        for p in (dx, dy):
+            if p.value:
+                lib.bsg_mfree(ctx, p)
+        lib.bsg_free(ctx)
