@@ -137,7 +137,11 @@ def copy_calibration(n, reps):
 
 
 def _short(name):
-    """k_sigma<false> from the demangled signature."""
+    """k_sigma<1, 4, 2> from the demangled signature (or from a mangled name
+    the profiler left as it was, e.g. a kernel with __bf16 arguments)."""
+    m = re.match(r"_ZN\w*?\d+(k_[a-z_]+)I((?:Li-?\d+E)+)E", name)
+    if m:
+        return f"{m.group(1)}<{', '.join(re.findall(r'Li(-?[0-9]+)E', m.group(2)))}>"
     name = name.replace("void ", "").replace("(anonymous namespace)::", "")
     return name.split("(")[0]
 
@@ -225,6 +229,32 @@ def summarize_traffic(dirs, last, pattern, copy_doubles):
     print(f"total per step: {tot:.4g} bytes")
 
 
+def summarize_sweep(d):
+    """GPU time of a whole profiled run (rocprofv3 --kernel-trace
+    --memory-copy-trace --stats CSVs) by kernel and by copy direction,
+    against the wall span of its kernel trace."""
+    fam = {}
+    for r in _csv_rows(d, "kernel_stats.csv"):
+        k = _short(r["Name"])
+        t, n = fam.get(k, (0, 0))
+        fam[k] = (t + int(r["TotalDurationNs"]), n + int(r["Calls"]))
+    trace = _csv_rows(d, "kernel_trace.csv")
+    span = (max(int(r["End_Timestamp"]) for r in trace) - min(int(r["Start_Timestamp"]) for r in trace)) / 1e9
+    ktot = sum(t for t, _ in fam.values()) / 1e9
+    print(f"wall span of the kernel trace: {span:.2f} s; kernels busy {ktot:.2f} s (summed over streams)")
+    print()
+    print("| kernel | calls | total s | share of kernel time |")
+    print("|---|---|---|---|")
+    for k, (t, n) in sorted(fam.items(), key=lambda x: -x[1][0]):
+        if t / 1e9 >= 0.01:
+            print(f"| {k} | {n} | {t / 1e9:.3f} | {t / 1e9 / ktot:.1%} |")
+    print()
+    print("| copies | calls | total s |")
+    print("|---|---|---|")
+    for r in _csv_rows(d, "memory_copy_stats.csv"):
+        print(f"| {r['Name']} | {r['Calls']} | {int(r['TotalDurationNs']) / 1e9:.3f} |")
+
+
 def fetch_val(d, name):
     return next(iter(d.values()))[name]
 
@@ -245,6 +275,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run a device copy of N doubles (HBM counter calibration)")
     ap.add_argument("--summarize-trace", metavar="DIR", help="table of a rocprofv3 kernel trace (no GPU run)")
+    ap.add_argument("--summarize-sweep", metavar="DIR",
+                    help="GPU time by kernel and copy direction of a profiled run (no GPU run)")
     ap.add_argument("--summarize-traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
                     help="HBM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (no GPU run)")
     ap.add_argument("--last", type=int, default=6, help="dispatches per kernel to summarise (the last ones)")
@@ -252,6 +284,8 @@ def main():
     args = ap.parse_args()
     if args.summarize_trace:
         return summarize_trace(args.summarize_trace, args.last, args.kernels) or 0
+    if args.summarize_sweep:
+        return summarize_sweep(args.summarize_sweep) or 0
     if args.summarize_traffic:
         return summarize_traffic(args.summarize_traffic, args.last, args.kernels, args.copy_calibration) or 0
     if args.copy_calibration:
