@@ -79,3 +79,28 @@ def test_gpu_dmrg_four_roots():
     assert e == pytest.approx(singlets, abs=1e-8)
     e_l, _ = block_amd.dmrg(ints, nroots=4, schedule=sched, max_sweeps=6, sweep_tol=0.0, ladder=1e-5)
     assert e_l == pytest.approx(e, abs=1e-8)
+
+
+def test_gpu_dmrg_reference_c2_d2h(golden):
+    """dmrg_tests/c2_d2h (runtest:12, fixture FCIDUMP and its dmrg.conf
+    schedule): 12 orbitals, 8 electrons, D2h, two state-averaged roots, M up
+    to 400; the reference asserts -75.4844085040 and -74.9778423596 to 1e-6
+    (measured here: -9e-8 and -1e-7 from them)."""
+    print("[gpu] dmrg reference fixture c2_d2h", flush=True)
+    ints = block_amd.Integrals(os.path.join(golden, "c2_d2h.FCIDUMP"))
+    sched = [(0, 50, 1e-6, 1e-6), (4, 150, 1e-6, 1e-6), (6, 400, 1e-6, 1e-6), (8, 400, 1e-10, 0.0)]
+    e, sw = block_amd.dmrg(ints, nroots=2, schedule=sched, max_sweeps=20, sweep_tol=1e-9)
+    assert e == pytest.approx([-75.4844085040, -74.9778423596], abs=1e-6)
+
+
+def test_gpu_dmrg_reference_h2o_nosym(golden):
+    """dmrg_tests/h2o_nosym (runtest:23): 13 orbitals, 10 electrons, no
+    symmetry, M up to 300, orbitals in Fiedler order (Block's default
+    reordering); the reference asserts -76.11460447 to 1e-6 (measured here:
+    2e-8 from it)."""
+    print("[gpu] dmrg reference fixture h2o_nosym", flush=True)
+    ints = block_amd.Integrals(os.path.join(golden, "h2o_nosym.FCIDUMP"))
+    ints = ints.reordered(ints.fiedler_order())
+    sched = [(0, 100, 1e-5, 1e-4), (4, 200, 1e-6, 1e-5), (8, 300, 1e-8, 1e-6), (12, 300, 1e-10, 0.0)]
+    e, sw = block_amd.dmrg(ints, schedule=sched, max_sweeps=24, sweep_tol=1e-9)
+    assert e[0] == pytest.approx(-76.11460447, abs=1e-6)
