@@ -19,6 +19,7 @@ import ctypes
 import json
 import os
 import re
+import resource
 import sys
 import tempfile
 import time
@@ -374,7 +375,8 @@ def main():
                                f"{warm}, FP64 sigma",
                    "wave_size": sb.wave_size, "flop_per_step": flops, "terms": sb.nterms,
                    "sectors_L_R_blocks": [nl, nr, nb], "gemm_pairs": h.pairs,
-                   "operator_arena_GB": round(h.arena_bytes / 1e9, 3), "superblock_build_s": round(t_build, 2)},
+                   "operator_arena_GB": round(h.arena_bytes / 1e9, 3), "superblock_build_s": round(t_build, 2),
+                   "host_peak_rss_GB": round(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1e6, 2)},
         "roofline": roofline,
     }
     if mixed:

@@ -54,6 +54,7 @@ _sig(_h, "blkh_nterms", ctypes.c_int, [_vp])
 _sig(_h, "blkh_nsectors", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int)])
 _sig(_h, "blkh_operator_bytes", ctypes.c_long, [_vp])
+_sig(_h, "blkh_wave_blocks", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int])
 _sig(_h, "blkh_diag_cpu", ctypes.c_int, [_vp, _dp])
 _sig(_h, "blkh_gpu_sigma_create", _vp, [_vp, ctypes.c_int, ctypes.c_long])
 _sig(_h, "blkh_gpu_sigma_free", None, [_vp])
@@ -163,6 +164,13 @@ class SuperBlock:
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         host_lib.blkh_nsectors(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         return a.value, b.value, c.value
+
+    def block_shapes(self):
+        """(rows, cols) of every wavefunction (V) block, in storage order."""
+        n = self.sectors()[2]
+        r, c = (ctypes.c_int * max(1, n))(), (ctypes.c_int * max(1, n))()
+        host_lib.blkh_wave_blocks(self._h, r, c, n)
+        return [(r[i], c[i]) for i in range(n)]
 
     def diagonal_cpu(self):
         out = np.zeros(self.wave_size)

@@ -207,17 +207,21 @@ __device__ inline void mma_run(int nv, int nr, dbl4 (&acc)[NRM][NB], const doubl
 // GEMM 1 is PB blocks per wave (16 PB WC wide), so a V tile wider than the W
 // chunk forms every W column once and contracts it into NV / PB chunks' worth
 // of V columns.
-template <int NRM, int NV, int PBW = PB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NV == 4 ? 3 : 2))) void k_sigma(
+// NWV: wavefronts per workgroup; with 8 they form one wave row of 8 columns
+// of waves (a 16 x 768 tile for NV = 6 with the registers of the 384-wide
+// one), so V blocks up to 768 wide form each W column once per row panel.
+template <int NRM, int NV, int PBW = PB, int NWV = 4>
+__global__ __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(NV == 4 ? 3 : 2))) void k_sigma(
     const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
     const TileRef* __restrict__ tiles, int ntiles, const double* __restrict__ arena, const double* __restrict__ c,
     double* __restrict__ v, double* __restrict__ part) {
   static_assert(NV % PBW == 0, "V tile width is a whole number of W chunks");
-  constexpr int NT = 256;
+  static_assert(NWV == 4 || (NWV == 8 && NRM == 1 && PBW == PB), "eight waves: one wave row, 16-row tiles");
+  constexpr int NT = 64 * NWV;
   // GEMM 1 uses two op(A) chunk buffers, GEMM 2 the W chunk; they share the
   // LDS (barriers separate the phases).  Both k-major: X[k * P + row].
   // W chunk: JC (TM + 1) doubles, JC = 16 PBW WC
-  constexpr int WSD = PBW == PB ? GWS : 16 * PBW * 4 * 17;
+  constexpr int WSD = NWV == 8 ? 16 * PBW * 8 * 17 : (PBW == PB ? GWS : 16 * PBW * 4 * 17);
   constexpr int LDSD = 2 * PKC * GP > WSD ? 2 * PKC * GP : WSD;
   __shared__ double lds[LDSD];
   double* const As0 = lds;
@@ -230,7 +234,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NV == 4 ? 3
   const Seg sg = segs[tr.item];
   const bsg_vblock_t vb = vbs[sg.vb];
   const int lwr = tr.pad & 3, nr = min(((tr.pad >> 2) & 1) + 1, NRM);
-  const int WR = 1 << lwr, WC = 4 >> lwr;
+  const int WR = NWV == 8 ? 1 : 1 << lwr, WC = NWV == 8 ? 8 : 4 >> lwr;
   const int TM = 16 * nr * WR, JC = 16 * PBW * WC, JV = 16 * NV * WC, P = TM + 1;
   const int row0 = tr.tm * TM, colg = tr.tn * JV;
   const int lane = threadIdx.x & 63;
@@ -1028,17 +1032,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // the MFMA issue time of the block's tiles (per-wave MFMAs per pair, summed
 // over tiles), which counts row padding, idle column slots and GEMM 1
 // repeated per column group.
-constexpr int NCLS = 3;
+constexpr int NCLS = 4;
 struct Geom {
   int lwr, nr, nv, tm, jc, jv, code, cls;
+  int wc = 4, nw = 4;  // wave columns, waves per workgroup
 };
+// MFMAs of all waves of all tiles of a V block, in units of a 4-wave tile's
 inline long geom_cost(const Geom& g, long m, long n, long k) {
   auto cdiv = [](long a, long b) { return (a + b - 1) / b; };
-  const int wc = 4 >> g.lwr;
+  const int wc = g.wc;
   const long panels = cdiv(m, g.tm), groups = cdiv(n, g.jv);
   const long wblk = cdiv(cdiv(k, 16), wc);  // W column blocks per wave (all chunks)
   const long vblk = std::max<long>(1, std::min<long>(g.nv, cdiv(std::min<long>(n, g.jv), 16 * wc)));
-  return panels * groups * g.nr * (wblk + vblk) * cdiv(k, 4);
+  return panels * groups * g.nr * (wblk + vblk) * cdiv(k, 4) * g.nw / 4;
 }
 inline Geom make_geom(int lwr, int nr, int nv) {
   Geom g;
@@ -1050,6 +1056,18 @@ inline Geom make_geom(int lwr, int nr, int nv) {
   g.jv = 16 * nv * (4 >> lwr);
   g.code = lwr | ((nr - 1) << 2);
   g.cls = nv / 2 - 1;
+  g.wc = 4 >> lwr;
+  return g;
+}
+// the 8-wave FP64 tile: 16 rows, 8 wave columns of 6 V column blocks (768)
+inline Geom make_geom_w8() {
+  Geom g = make_geom(0, 1, 6);
+  g.wc = 8;
+  g.nw = 8;
+  g.jc = 16 * PB * 8;
+  g.jv = 16 * 6 * 8;
+  g.code = 8;
+  g.cls = 3;
   return g;
 }
 inline Geom pick_geom(int m, int n, int max_nv) {
@@ -1059,6 +1077,7 @@ inline Geom pick_geom(int m, int n, int max_nv) {
   for (int lwr = 0; lwr <= 2; ++lwr)
     for (int nr = 1; nr <= 2; ++nr) cand.push_back(make_geom(lwr, nr, PB));
   for (int nv = 4; nv <= max_nv; nv += 2) cand.push_back(make_geom(0, 1, nv));
+  if (max_nv >= 6) cand.push_back(make_geom_w8());
   Geom best = cand[0];
   long bc = -1;
   for (const Geom& g : cand) {
@@ -1083,7 +1102,7 @@ inline long small_cost(int m, int n, const bsg_pair_t& pr) {
 // MFMA instructions per wavefront of one pair in a tile (segment cost)
 inline long pair_cost(const Geom& g, int n, const bsg_pair_t& pr) {
   auto cdiv = [](long a, long b) { return (a + b - 1) / b; };
-  const int wc = 4 >> g.lwr;
+  const int wc = g.wc;
   const long kw = pr.b_off < 0 ? std::min<long>(pr.k2, g.jv) : pr.k2;  // W columns formed per tile
   const long nch = cdiv(kw, g.jc);
   const long g1 = pr.a_off < 0 ? 0 : g.nr * cdiv(cdiv(kw, 16), wc) * cdiv(pr.k1, 4);
@@ -1295,8 +1314,8 @@ struct PlanChunk {
   bsg_pair_t* d_pr = nullptr;
   PairS* d_ps = nullptr;  // slice offsets of the grouped kernel's pairs (split precision)
   Seg* d_seg = nullptr;
-  TileRef* d_g[NCLS] = {nullptr, nullptr, nullptr};  // FP64 grouped-kernel tiles per kernel class
-  int n_g[NCLS] = {0, 0, 0};
+  TileRef* d_g[NCLS] = {};  // FP64 grouped-kernel tiles per kernel class
+  int n_g[NCLS] = {};
   TileRef* d_gbf[2] = {nullptr, nullptr};  // split-precision tiles (basic, 256 wide)
   int n_gbf[2] = {0, 0};
   int4* d_s[2] = {nullptr, nullptr};  // FP64 small-block units, 4 per workgroup (SB = 2, 3)
@@ -1589,7 +1608,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           for (int tm = 0; tm < (vb.m + g.tm - 1) / g.tm; ++tm)
             for (int tn = 0; tn < (vb.n + g.jv - 1) / g.jv; ++tn) gu.tiles.push_back({si, tm, tn, g.code});
           if (mode != 2) gunits[g.cls].push_back(gu);
-          if (mode != 1) gunits_bf[g.cls].push_back(gu);
+          if (mode != 1 && g.cls < 2) gunits_bf[g.cls].push_back(gu);
         }
         if (nseg > 1) {
           for (int64_t e0 = 0; e0 < mn; e0 += 256) {
@@ -1784,6 +1803,9 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
       hipLaunchKernelGGL((k_sigma_bf<1, 4>), dim3(ch.n_gbf[1]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_ps, ch.d_seg,
                          ch.d_gbf[1], ch.n_gbf[1], p->d_abs, p->d_cs, c, v, p->part);
   } else {
+    if (ch.n_g[3] > 0)
+      hipLaunchKernelGGL((k_sigma<1, 6, PB, 8>), dim3(ch.n_g[3]), dim3(512), 0, st, ch.d_vb, ch.d_pr, ch.d_seg,
+                         ch.d_g[3], ch.n_g[3], arena, c, v, p->part);
     if (ch.n_g[2] > 0)
       hipLaunchKernelGGL((k_sigma<1, 6>), dim3(ch.n_g[2]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g[2],
                          ch.n_g[2], arena, c, v, p->part);

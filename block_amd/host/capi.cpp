@@ -139,6 +139,17 @@ int blkh_nsectors(void* h, int* nl, int* nr, int* nblocks) {
   *nblocks = (int)sb->wave->blocks.size();
   return 0;
 }
+// rows and columns of the wavefunction's blocks (V blocks), at most max of them;
+// returns the number of blocks
+int blkh_wave_blocks(void* h, int* rows, int* cols, int max) {
+  auto* sb = static_cast<SuperBlock*>(h);
+  const int n = (int)sb->wave->blocks.size();
+  for (int i = 0; i < n && i < max; ++i) {
+    rows[i] = sb->wave->blocks[i].rows;
+    cols[i] = sb->wave->blocks[i].cols;
+  }
+  return n;
+}
 long blkh_operator_bytes(void* h) {
   auto* sb = static_cast<SuperBlock*>(h);
   return (long)(sb->L->nbytes() + sb->R->nbytes());

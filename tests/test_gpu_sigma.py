@@ -23,13 +23,22 @@ def _cases():
         # mid-chain superblock of the 24-orbital DMRG warm-up at M = 500: V
         # blocks up to ~350 wide, the wide-tile kernel classes
         ("syn24_M500_dmrg", lambda: block_amd.SuperBlock.from_dmrg_warmup(config_fcidump("syn24"), 24, 0, 0, 500)),
+        # the same at M = 700: a V block 479 wide, on the 768-wide FP64 tiles
+        # (eight waves in one wave row)
+        ("syn24_M700_dmrg", lambda: block_amd.SuperBlock.from_dmrg_warmup(config_fcidump("syn24"), 24, 0, 0, 700)),
     ]
+
+
+# cases that must contain a V block wider than the 384-wide tiles
+WIDE = {"syn24_M700_dmrg": 385}
 
 
 @pytest.mark.parametrize("name,make", _cases(), ids=[c[0] for c in _cases()])
 def test_sigma_parity(name, make):
     print(f"[gpu] sigma parity {name}", flush=True)
     sb = make()
+    if name in WIDE:
+        assert max(c for _, c in sb.block_shapes()) >= WIDE[name]
     h = block_amd.multiply_h(sb)
     c = rng_vec(sb.wave_size, 7)
     ref = _oracle.sigma(sb, c, nthreads=8)
