@@ -546,9 +546,12 @@ constexpr int SAB = 3 * GTM * SAP;      // one staged op(A) chunk, 3 slices
 constexpr int SWMAX = 3 * 64 * 72;      // W chunk slices: max over geometries of 3 TM (JC + 8)
 
 // NRM, NV: as for the FP64 kernel (row blocks per wave the accumulators
-// hold, V column blocks per wave); the widest split tiles are 256 columns.
-template <int NRM, int NV>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sigma_bf(
+// hold, V column blocks per wave).  NWV: wavefronts per workgroup; with 8
+// they form one wave row of 8 columns of waves (16 x 512 for NV = 4, the
+// registers of the 256-wide tile), so V blocks up to 512 wide form each W
+// column once per row panel.
+template <int NRM, int NV, int NWV = 4>
+__global__ __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(2))) void k_sigma_bf(
     const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const PairS* __restrict__ pss,
     const Seg* __restrict__ segs, const TileRef* __restrict__ tiles, int ntiles, const __bf16* __restrict__ slices,
     const __bf16* __restrict__ cslices, const double* __restrict__ c, double* __restrict__ v,
@@ -564,8 +567,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const Seg sg = segs[tr.item];
   const bsg_vblock_t vb = vbs[sg.vb];
   static_assert(NV % PB == 0, "V tile width is a whole number of W chunks");
+  static_assert(NWV == 4 || (NWV == 8 && NRM == 1), "eight waves: one wave row, 16-row tiles");
+  constexpr int NT = 64 * NWV;
   const int lwr = tr.pad & 3, nr = min(((tr.pad >> 2) & 1) + 1, NRM);
-  const int WR = 1 << lwr, WC = 4 >> lwr;
+  const int WR = NWV == 8 ? 1 : 1 << lwr, WC = NWV == 8 ? 8 : 4 >> lwr;
   const int TM = 16 * nr * WR, JC = 16 * PB * WC, JV = 16 * NV * WC, WP = JC + 8;
   const int row0 = tr.tm * TM, colg = tr.tn * JV;
   const int lane = threadIdx.x & 63;
@@ -587,7 +592,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   // op(A) staging: piece u (8 BF16) of a chunk is slice u / (4 TM), row
   // (u / 4) % TM, K quarter u % 4; three pieces per thread at most (one on
   // the wide tiles, which are 16 rows)
-  constexpr int SPT = ((NV > PB ? 16 : GTM) * 12 + 255) / 256;
+  constexpr int SPT = ((NV > PB ? 16 : GTM) * 12 + NT - 1) / NT;
   int abuf = 0;
   for (int q = 0; q < sg.count; ++q) {
     const bsg_pair_t pr = prs[sg.first + q];
@@ -620,7 +625,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       if (pr.a_off < 0) {
         const __amdgpu_buffer_rsrc_t rC = block_rsrc(c + pr.c_off, (int64_t)pr.k1 * pr.k2);
         // W = alpha C (panel rows), split into the W slices
-        for (int e = threadIdx.x; e < TM * JC; e += 256) {
+        for (int e = threadIdx.x; e < TM * JC; e += NT) {
           const int j = e % JC, i = e / JC;
           const int gi = row0 + i;
           const bool ok = gi < vb.m && j < jw;
@@ -643,7 +648,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         int alds[SPT], aq[SPT];
 #pragma unroll
         for (int x = 0; x < SPT; ++x) {
-          const int u = threadIdx.x + 256 * x;
+          const int u = threadIdx.x + NT * x;
           const int s = u / (4 * TM), i = (u >> 2) % TM, kq = u & 3;
           const bool ok = u < 12 * TM && row0 + i < vb.m;
           aoff[x] = ok ? (uint32_t)s * aslice + (uint32_t)((row0 + i) * k1p + 8 * kq) * 2u : OOB;
@@ -1059,15 +1064,17 @@ inline Geom make_geom(int lwr, int nr, int nv) {
   g.wc = 4 >> lwr;
   return g;
 }
-// the 8-wave FP64 tile: 16 rows, 8 wave columns of 6 V column blocks (768)
-inline Geom make_geom_w8() {
-  Geom g = make_geom(0, 1, 6);
+// the 8-wave tiles: 16 rows, 8 wave columns of nv V column blocks (768 wide
+// for the FP64 kernel's nv = 6, tile class 3; 512 for the split kernel's
+// nv = 4, split tile class 2)
+inline Geom make_geom_w8(int nv) {
+  Geom g = make_geom(0, 1, nv);
   g.wc = 8;
   g.nw = 8;
   g.jc = 16 * PB * 8;
-  g.jv = 16 * 6 * 8;
-  g.code = 8;
-  g.cls = 3;
+  g.jv = 16 * nv * 8;
+  g.code = nv == 6 ? 8 : 16;
+  g.cls = nv == 6 ? 3 : 2;
   return g;
 }
 inline Geom pick_geom(int m, int n, int max_nv) {
@@ -1077,7 +1084,7 @@ inline Geom pick_geom(int m, int n, int max_nv) {
   for (int lwr = 0; lwr <= 2; ++lwr)
     for (int nr = 1; nr <= 2; ++nr) cand.push_back(make_geom(lwr, nr, PB));
   for (int nv = 4; nv <= max_nv; nv += 2) cand.push_back(make_geom(0, 1, nv));
-  if (max_nv >= 6) cand.push_back(make_geom_w8());
+  cand.push_back(make_geom_w8(max_nv >= 6 ? 6 : 4));
   Geom best = cand[0];
   long bc = -1;
   for (const Geom& g : cand) {
@@ -1316,8 +1323,8 @@ struct PlanChunk {
   Seg* d_seg = nullptr;
   TileRef* d_g[NCLS] = {};  // FP64 grouped-kernel tiles per kernel class
   int n_g[NCLS] = {};
-  TileRef* d_gbf[2] = {nullptr, nullptr};  // split-precision tiles (basic, 256 wide)
-  int n_gbf[2] = {0, 0};
+  TileRef* d_gbf[3] = {};  // split-precision tiles (basic, 256 wide, 512 wide)
+  int n_gbf[3] = {};
   int4* d_s[2] = {nullptr, nullptr};  // FP64 small-block units, 4 per workgroup (SB = 2, 3)
   int n_s[2] = {0, 0};
   RedItem* d_red[2] = {nullptr, nullptr};  // after the FP64 / the split-precision kernels
@@ -1502,7 +1509,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
   for (int ci = 0; ci < n_chunks; ++ci) {
     const bsg_chunk_t& ch = chunks[ci];
     PlanChunk pc;
-    std::vector<GUnit> gunits[NCLS], gunits_bf[2];  // the tiles of each work unit (FP64, split)
+    std::vector<GUnit> gunits[NCLS], gunits_bf[3];  // the tiles of each work unit (FP64, split)
     std::vector<bsg_pair_t> perm;  // pairs in device order (grouped per V block as given)
     std::vector<PairS> pslices;    // their slice offsets
     std::vector<std::pair<long, int>> sunits[2];  // (cost, segment) of small-block units
@@ -1608,7 +1615,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           for (int tm = 0; tm < (vb.m + g.tm - 1) / g.tm; ++tm)
             for (int tn = 0; tn < (vb.n + g.jv - 1) / g.jv; ++tn) gu.tiles.push_back({si, tm, tn, g.code});
           if (mode != 2) gunits[g.cls].push_back(gu);
-          if (mode != 1 && g.cls < 2) gunits_bf[g.cls].push_back(gu);
+          if (mode != 1 && g.cls < 3) gunits_bf[g.cls].push_back(gu);
         }
         if (nseg > 1) {
           for (int64_t e0 = 0; e0 < mn; e0 += 256) {
@@ -1652,7 +1659,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     }
     std::vector<TileRef> gtiles[NCLS];
     for (int k = 0; k < NCLS; ++k) gtiles[k] = xcd_order(gunits[k]);
-    std::vector<TileRef> gtiles_bf[2] = {xcd_order(gunits_bf[0]), xcd_order(gunits_bf[1])};
+    std::vector<TileRef> gtiles_bf[3] = {xcd_order(gunits_bf[0]), xcd_order(gunits_bf[1]), xcd_order(gunits_bf[2])};
     // small-block units, costliest first, four per workgroup
     std::vector<int4> sq[2];
     for (int k = 0; k < 2; ++k) {
@@ -1675,7 +1682,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       rc = up((void**)&pc.d_g[k], gtiles[k].data(), sizeof(TileRef) * gtiles[k].size());
       pc.n_g[k] = (int)gtiles[k].size();
     }
-    for (int k = 0; k < 2 && !rc; ++k) {
+    for (int k = 0; k < 3 && !rc; ++k) {
       rc = up((void**)&pc.d_gbf[k], gtiles_bf[k].data(), sizeof(TileRef) * gtiles_bf[k].size());
       pc.n_gbf[k] = (int)gtiles_bf[k].size();
     }
@@ -1714,8 +1721,7 @@ int bsg_plan_destroy(bsg_plan* p) {
     (void)hipFree(c.d_red[0]);
     (void)hipFree(c.d_red[1]);
     for (int k = 0; k < NCLS; ++k) (void)hipFree(c.d_g[k]);
-    (void)hipFree(c.d_gbf[0]);
-    (void)hipFree(c.d_gbf[1]);
+    for (int k = 0; k < 3; ++k) (void)hipFree(c.d_gbf[k]);
     (void)hipFree(c.d_s[0]);
     (void)hipFree(c.d_s[1]);
   }
@@ -1799,6 +1805,9 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
     HIPCHK(hipEventRecord(ctx->join, ctx->side));
   }
   if (split) {
+    if (ch.n_gbf[2] > 0)
+      hipLaunchKernelGGL((k_sigma_bf<1, 4, 8>), dim3(ch.n_gbf[2]), dim3(512), 0, st, ch.d_vb, ch.d_pr, ch.d_ps,
+                         ch.d_seg, ch.d_gbf[2], ch.n_gbf[2], p->d_abs, p->d_cs, c, v, p->part);
     if (ch.n_gbf[1] > 0)
       hipLaunchKernelGGL((k_sigma_bf<1, 4>), dim3(ch.n_gbf[1]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_ps, ch.d_seg,
                          ch.d_gbf[1], ch.n_gbf[1], p->d_abs, p->d_cs, c, v, p->part);
