@@ -1187,6 +1187,21 @@ __global__ void k_scal(double a, double* __restrict__ x, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     x[i] *= a;
 }
+// the same with the scalar in device memory: x *= 1/sqrt(s[0]) (only if
+// s[0] > floor), y += -s[0] x
+__global__ void k_scal_rsqrt_dev(const double* __restrict__ s, double floor, double* __restrict__ x, int64_t n) {
+  const double n2 = s[0];
+  if (!(n2 > floor)) return;
+  const double a = 1.0 / sqrt(n2);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+__global__ void k_axpy_neg_dev(const double* __restrict__ s, const double* __restrict__ x, double* __restrict__ y,
+                               int64_t n) {
+  const double a = -s[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
 __global__ void k_precond(double* __restrict__ x, const double* __restrict__ d, double e, double shift, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double den = e - d[i];
@@ -1958,6 +1973,27 @@ int bsg_dot(bsg_context* ctx, const double* x, const double* y, int64_t n, doubl
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(result, ctx->red + RMAXB, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BSG_OK;
+}
+int bsg_gs_orthonormalise(bsg_context* ctx, double* r, const double* B, int32_t m, int64_t ld, int64_t n) {
+  if (!ctx || !r || n < 0 || m < 0 || (m > 0 && (!B || ld < n))) return fail(BSG_ERR_ARG, "bsg_gs_orthonormalise: bad args");
+  if (n == 0) return BSG_OK;
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(RMAXB, (n + RB - 1) / RB));
+  double* s = ctx->red + RMAXB;
+  auto dot = [&](const double* x, const double* y) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(RB), 0, ctx->stream, x, y, n, ctx->red);
+    hipLaunchKernelGGL(k_sum_partial, dim3(1), dim3(RB), 0, ctx->stream, ctx->red, nb, s);
+  };
+  for (int32_t j = 0; j < m; ++j) {
+    const double* bj = B + (int64_t)j * ld;
+    dot(r, r);
+    hipLaunchKernelGGL(k_scal_rsqrt_dev, dim3(grid_for(n)), dim3(256), 0, ctx->stream, s, -1.0, r, n);
+    dot(r, bj);
+    hipLaunchKernelGGL(k_axpy_neg_dev, dim3(grid_for(n)), dim3(256), 0, ctx->stream, s, bj, r, n);
+  }
+  dot(r, r);
+  hipLaunchKernelGGL(k_scal_rsqrt_dev, dim3(grid_for(n)), dim3(256), 0, ctx->stream, s, 1e-15, r, n);
+  HIPCHK(hipGetLastError());
   return BSG_OK;
 }
 int bsg_axpy(bsg_context* ctx, double a, const double* x, double* y, int64_t n) {

@@ -133,12 +133,8 @@ void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std
         m = deflation_min;
         nsig = std::min(nsig, m);
       }
-      for (int j = 0; j < m; ++j) {  // linear.C:358-366
-        const double nrm = dot(R.p, R.p);
-        bsg_check(bsg_scal(ctx, 1.0 / std::sqrt(nrm), R.p, N), "scal");
-        bsg_check(bsg_axpy(ctx, -dot(R.p, vec(Bp, j)), vec(Bp, j), R.p, N), "axpy");
-      }
-      normalise(R.p);
+      // linear.C:358-366 and the Normalise after it, scalars on the device
+      bsg_check(bsg_gs_orthonormalise(ctx, R.p, Bp, m, N, N), "orthonormalise");
       bsg_check(bsg_copy(ctx, R.p, vec(Bp, m), N), "copy");
       ++m;
     }

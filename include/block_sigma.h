@@ -176,6 +176,13 @@ int bsg_precondition(bsg_context* ctx, double* x, const double* diag, double e, 
  * then r = precond(r).  work: device scratch of n doubles. */
 int bsg_olsen_precondition(bsg_context* ctx, double* r, const double* c, const double* diag, double e, double shift,
                            double* work, int64_t n);
+/* Gram-Schmidt of the new Davidson vector (linear.C:358-366 and the
+ * Normalise that follows): for j < m, r /= |r|, r -= <r, b_j> b_j; then r
+ * is normalised unless |r|^2 <= 1e-15 (BaseOperator.C:282-287).  B: m
+ * device vectors at stride ld.  The scalars stay in device memory, so the
+ * loop runs without a host round trip per step; same reductions and order
+ * as bsg_dot / bsg_scal / bsg_axpy. */
+int bsg_gs_orthonormalise(bsg_context* ctx, double* r, const double* B, int32_t m, int64_t ld, int64_t n);
 
 #ifdef __cplusplus
 }
