@@ -7,12 +7,15 @@
 // k_sigma fuses both: a workgroup owns one V tile and walks a slice of that
 // block's pair list in order; for every pair it builds W in LDS chunk by
 // chunk (MFMA over the A.C contraction) and immediately contracts it with
-// B into the V accumulator, so W never travels to HBM.  Tiles are 64x64
-// (4 wavefronts) or 32x32 (1 wavefront), each wavefront owning a 32x32
-// quarter accumulated with v_mfma_f64_16x16x4f64 from LDS-staged 16-deep K
-// slices.  Long pair lists are cut into slices whose partial tiles k_reduce
-// adds in list order, so V is reduced without atomics and bitwise
-// reproducibly.
+// B into the V accumulator, so W never travels to HBM.  Tile classes: basic
+// tiles (up to 64 rows, 4 wavefronts of 32 x 32 or 16 x 64), and wide
+// 16-row tiles of 256, 384 (4 wavefronts) or 768 columns (8 wavefronts in
+// one row), so a wide V block forms each W column once per row panel;
+// v_mfma_f64_16x16x4f64 throughout.  k_sigma_bf is the same on pre-split
+// BF16 slices (v_mfma_f32_16x16x32_bf16, 256- and 512-wide tiles), k_small
+// gives V blocks up to 32 x 32 one wavefront each.  Long pair lists are cut
+// into work units whose partial tiles k_reduce adds in list order, so V is
+// reduced without atomics and bitwise reproducibly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
