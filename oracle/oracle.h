@@ -1,7 +1,7 @@
 // CPU oracle for the Davidson sigma path.  TEST INFRASTRUCTURE ONLY: nothing
 // in the product (block_amd/) may call, link or load this code; only tests/,
-// the smoke check and bench.py's cpu_baseline leg use it, and only as the
-// checker / reported CPU baseline.
+// the smoke check and bench.py's cpu_baseline leg use it, and only to check
+// results against or as the reported CPU baseline.
 //
 //  * oracle_multiply_h  -- direct restatement of SpinBlock::multiplyH
 //    (spinblock.C:722-789) looping over the operator arrays the way the opxop

@@ -1,5 +1,6 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so) -- test
-infrastructure: used by tests only as the checker."""
+infrastructure: tests use it only as the reference their results are
+checked against."""
 import ctypes
 import os
 
