@@ -71,6 +71,8 @@ _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
       ctypes.c_int, ctypes.c_int, _dp, _dp])
 
+_sig(_h, "blkh_davidson", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp,
+                                        ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.POINTER(ctypes.c_int)])
 _sig(_h, "blkh_rotation_check", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _lp])
 _ip = ctypes.POINTER(ctypes.c_int)
 _sig(_h, "blkh_fiedler_order", ctypes.c_int, [_vp, _ip])
@@ -289,6 +291,28 @@ def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=
     if n < 0:
         raise _err()
     return e, sw[: nc * n].reshape(n, nc)
+
+
+def davidson(sb, guesses, tol=1e-12, lower=None, ladder=0.0, device=0, host_loop=False):
+    """One Davidson solve on superblock sb with the GPU sigma
+    (Linear::block_davidson, linear.C:179-385), from the given guesses (one
+    row per root).  lower: states to project off (the reference's
+    lowerStates).  host_loop: run the host control flow with the GPU sigma
+    instead of the device-resident solve.  Returns (energies, vectors,
+    (iterations, sigma calls, split-precision sigma calls), device_solve)
+    where device_solve says whether the device-resident solve ran."""
+    g = np.ascontiguousarray(np.atleast_2d(guesses), dtype=np.float64)
+    nroots = g.shape[0]
+    low = None if lower is None or len(lower) == 0 else np.ascontiguousarray(np.atleast_2d(lower), dtype=np.float64)
+    e = np.zeros(nroots)
+    vecs = np.zeros_like(g)
+    st = (ctypes.c_int * 3)()
+    rc = host_lib.blkh_davidson(sb._h, device, int(bool(host_loop)), nroots, _ptr(g),
+                                0 if low is None else low.shape[0], None if low is None else _ptr(low), tol,
+                                float(ladder), _ptr(e), _ptr(vecs), st)
+    if rc < 0:
+        raise _err()
+    return e, vecs, (st[0], st[1], st[2]), (rc == 0 and not host_loop)
 
 
 def rotation_check(ints, nsys, M, device=0):

@@ -1222,30 +1222,31 @@ __global__ __launch_bounds__(256) void k_copy_chunks(const bsg_copy_seg_t* __res
   for (int64_t i = threadIdx.x; i < c.n; i += 256) c.dst[i] = c.src[i];
 }
 
-// Gram matrix of two vector sets (m <= GMAX vectors of length n, stride ld):
-// per-block partial sums G[i m + j] = sum_x X_i[x] Y_j[x] over the block's
-// chunks, then an ordered sum over blocks (deterministic).
-constexpr int GMAX = 32, GCH = 64, GBLK = 256;
-__global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__ X, const double* __restrict__ Y, int m,
-                                                      int64_t n, int64_t ld, double* __restrict__ part) {
+// Gram matrix tile of two vector sets (mx, my <= GMAX vectors of length n,
+// stride ld): per-block partial sums G[i my + j] = sum_x X_i[x] Y_j[x] over
+// the block's chunks, then an ordered sum over blocks (deterministic).
+// Larger sets (up to GRAM_MAX vectors) run as GMAX x GMAX tiles.
+constexpr int GMAX = 32, GCH = 64, GBLK = 256, GRAM_MAX = 128;
+__global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__ X, const double* __restrict__ Y, int mx,
+                                                      int my, int64_t n, int64_t ld, double* __restrict__ part) {
   __shared__ double xs[GMAX][GCH + 1], ys[GMAX][GCH + 1];
   double acc[GMAX * GMAX / 256];
-  const int npair = m * m;
+  const int npair = mx * my;
 #pragma unroll
   for (int u = 0; u < GMAX * GMAX / 256; ++u) acc[u] = 0.0;
   for (int64_t x0 = (int64_t)blockIdx.x * GCH; x0 < n; x0 += (int64_t)gridDim.x * GCH) {
-    for (int e = threadIdx.x; e < m * GCH; e += 256) {
+    for (int e = threadIdx.x; e < GMAX * GCH; e += 256) {
       const int i = e / GCH, x = e % GCH;
       const bool ok = x0 + x < n;
-      xs[i][x] = ok ? X[(int64_t)i * ld + x0 + x] : 0.0;
-      ys[i][x] = ok ? Y[(int64_t)i * ld + x0 + x] : 0.0;
+      if (i < mx) xs[i][x] = ok ? X[(int64_t)i * ld + x0 + x] : 0.0;
+      if (i < my) ys[i][x] = ok ? Y[(int64_t)i * ld + x0 + x] : 0.0;
     }
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < GMAX * GMAX / 256; ++u) {
       const int pidx = threadIdx.x + 256 * u;
       if (pidx < npair) {
-        const int i = pidx / m, j = pidx % m;
+        const int i = pidx / my, j = pidx % my;
         double s = 0.0;
         for (int x = 0; x < GCH; ++x) s += xs[i][x] * ys[j][x];
         acc[u] += s;
@@ -1259,20 +1260,23 @@ __global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__
     if (pidx < npair) part[(int64_t)blockIdx.x * npair + pidx] = acc[u];
   }
 }
-__global__ __launch_bounds__(256) void k_gram_sum(const double* __restrict__ part, int nblk, int npair,
-                                                  double* __restrict__ out) {
+// out[(i) ldo + j] = sum over blocks of the tile's partials (i < mx, j < my)
+__global__ __launch_bounds__(256) void k_gram_sum(const double* __restrict__ part, int nblk, int mx, int my,
+                                                  double* __restrict__ out, int ldo) {
   const int pidx = blockIdx.x * 256 + threadIdx.x;
+  const int npair = mx * my;
   if (pidx >= npair) return;
   double s = 0.0;
   for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * npair + pidx];
-  out[pidx] = s;
+  out[(int64_t)(pidx / my) * ldo + pidx % my] = s;
 }
-// Y_j[x] = sum_i coef[i mo + j] X_i[x] for j < mo (m, mo <= GMAX)
+// Y_j[x] (+)= sum_{i < m} coef[i ldc + j] X_i[x] for j < mo (m, mo <= GMAX);
+// accumulate = 0 overwrites Y
 __global__ __launch_bounds__(256) void k_lincomb(const double* __restrict__ X, int m, int64_t n, int64_t ld,
-                                                 const double* __restrict__ coef, int mo, double* __restrict__ Y,
-                                                 int64_t ldy) {
+                                                 const double* __restrict__ coef, int ldc, int mo,
+                                                 double* __restrict__ Y, int64_t ldy, int accumulate) {
   __shared__ double cs[GMAX * GMAX];
-  for (int e = threadIdx.x; e < m * mo; e += 256) cs[e] = coef[e];
+  for (int e = threadIdx.x; e < m * mo; e += 256) cs[e] = coef[(int64_t)(e / mo) * ldc + e % mo];
   __syncthreads();
   for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (int64_t)gridDim.x * 256) {
     double xv[GMAX];
@@ -1283,7 +1287,10 @@ __global__ __launch_bounds__(256) void k_lincomb(const double* __restrict__ X, i
 #pragma unroll
       for (int i = 0; i < GMAX; ++i)
         if (i < m) s += xv[i] * cs[i * mo + j];
-      Y[(int64_t)j * ldy + x] = s;
+      if (accumulate)
+        Y[(int64_t)j * ldy + x] += s;
+      else
+        Y[(int64_t)j * ldy + x] = s;
     }
   }
 }
@@ -1329,7 +1336,7 @@ struct bsg_context {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   double* red = nullptr;  // RMAXB partials + 1 result
-  double* gpart = nullptr;  // Gram partials (GBLK GMAX^2) + result (GMAX^2) + coefficients (GMAX^2)
+  double* gpart = nullptr;  // Gram partials (GBLK GMAX^2) + result (GRAM_MAX^2) + coefficients (GRAM_MAX^2)
   bsg_copy_seg_t* copy_list = nullptr;  // chunk list of bsg_copy_segments (grown on demand)
   size_t copy_cap = 0;
 };
@@ -1389,7 +1396,7 @@ int bsg_init(bsg_context** ctx, int device) {
     delete c;
     return fail(BSG_ERR_NOMEM, "bsg_init: reduction buffer");
   }
-  if (hipMalloc(&c->gpart, sizeof(double) * (GBLK + 2) * GMAX * GMAX) != hipSuccess) {
+  if (hipMalloc(&c->gpart, sizeof(double) * ((size_t)GBLK * GMAX * GMAX + 2 * GRAM_MAX * GRAM_MAX)) != hipSuccess) {
     (void)hipFree(c->red);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1978,7 +1985,8 @@ int bsg_dot(bsg_context* ctx, const double* x, const double* y, int64_t n, doubl
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return BSG_OK;
 }
-int bsg_gs_orthonormalise(bsg_context* ctx, double* r, const double* B, int32_t m, int64_t ld, int64_t n) {
+int bsg_gs_orthonormalise(bsg_context* ctx, double* r, const double* B, int32_t m, int64_t ld, int64_t n,
+                          int32_t normalise) {
   if (!ctx || !r || n < 0 || m < 0 || (m > 0 && (!B || ld < n))) return fail(BSG_ERR_ARG, "bsg_gs_orthonormalise: bad args");
   if (n == 0) return BSG_OK;
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(RMAXB, (n + RB - 1) / RB));
@@ -1994,8 +2002,10 @@ int bsg_gs_orthonormalise(bsg_context* ctx, double* r, const double* B, int32_t 
     dot(r, bj);
     hipLaunchKernelGGL(k_axpy_neg_dev, dim3(grid_for(n)), dim3(256), 0, ctx->stream, s, bj, r, n);
   }
-  dot(r, r);
-  hipLaunchKernelGGL(k_scal_rsqrt_dev, dim3(grid_for(n)), dim3(256), 0, ctx->stream, s, 1e-15, r, n);
+  if (normalise) {
+    dot(r, r);
+    hipLaunchKernelGGL(k_scal_rsqrt_dev, dim3(grid_for(n)), dim3(256), 0, ctx->stream, s, 1e-15, r, n);
+  }
   HIPCHK(hipGetLastError());
   return BSG_OK;
 }
@@ -2007,12 +2017,20 @@ int bsg_axpy(bsg_context* ctx, double a, const double* x, double* y, int64_t n) 
   return BSG_OK;
 }
 int bsg_gram(bsg_context* ctx, const double* X, const double* Y, int32_t m, int64_t n, int64_t ld, double* out) {
-  if (!ctx || !X || !Y || !out || m <= 0 || m > GMAX || n < 0 || ld < n) return fail(BSG_ERR_ARG, "bsg_gram: bad args");
+  if (!ctx || !X || !Y || !out || m <= 0 || m > GRAM_MAX || n < 0 || ld < n)
+    return fail(BSG_ERR_ARG, "bsg_gram: bad args");
   const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(GBLK, (n + GCH - 1) / GCH));
   double* part = ctx->gpart;
   double* res = ctx->gpart + (size_t)GBLK * GMAX * GMAX;
-  hipLaunchKernelGGL(k_gram_partial, dim3(nblk), dim3(256), 0, ctx->stream, X, Y, m, n, ld, part);
-  hipLaunchKernelGGL(k_gram_sum, dim3((m * m + 255) / 256), dim3(256), 0, ctx->stream, part, nblk, m * m, res);
+  // GMAX x GMAX tiles in stream order (the partials are reused tile by tile)
+  for (int i0 = 0; i0 < m; i0 += GMAX)
+    for (int j0 = 0; j0 < m; j0 += GMAX) {
+      const int mx = std::min(GMAX, m - i0), my = std::min(GMAX, m - j0);
+      hipLaunchKernelGGL(k_gram_partial, dim3(nblk), dim3(256), 0, ctx->stream, X + (int64_t)i0 * ld,
+                         Y + (int64_t)j0 * ld, mx, my, n, ld, part);
+      hipLaunchKernelGGL(k_gram_sum, dim3((mx * my + 255) / 256), dim3(256), 0, ctx->stream, part, nblk, mx, my,
+                         res + (int64_t)i0 * m + j0, m);
+    }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, res, sizeof(double) * m * m, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -2020,11 +2038,17 @@ int bsg_gram(bsg_context* ctx, const double* X, const double* Y, int32_t m, int6
 }
 int bsg_lincomb(bsg_context* ctx, const double* X, int32_t m, int64_t n, int64_t ld, const double* coef, int32_t mo,
                 double* Y, int64_t ldy) {
-  if (!ctx || !X || !coef || !Y || m <= 0 || m > GMAX || mo <= 0 || mo > GMAX || n < 0 || ld < n || ldy < n)
+  if (!ctx || !X || !coef || !Y || m <= 0 || m > GRAM_MAX || mo <= 0 || mo > GRAM_MAX || n < 0 || ld < n || ldy < n)
     return fail(BSG_ERR_ARG, "bsg_lincomb: bad args");
-  double* cd = ctx->gpart + (size_t)(GBLK + 1) * GMAX * GMAX;
+  double* cd = ctx->gpart + (size_t)GBLK * GMAX * GMAX + (size_t)GRAM_MAX * GRAM_MAX;
   HIPCHK(hipMemcpyAsync(cd, coef, sizeof(double) * m * mo, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n)), dim3(256), 0, ctx->stream, X, m, n, ld, cd, mo, Y, ldy);
+  // GMAX x GMAX coefficient tiles; the first row tile of each output tile
+  // overwrites Y, the others add in order
+  for (int j0 = 0; j0 < mo; j0 += GMAX)
+    for (int i0 = 0; i0 < m; i0 += GMAX)
+      hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n)), dim3(256), 0, ctx->stream, X + (int64_t)i0 * ld,
+                         std::min(GMAX, m - i0), n, ld, cd + (int64_t)i0 * mo + j0, mo, std::min(GMAX, mo - j0),
+                         Y + (int64_t)j0 * ldy, ldy, i0 > 0 ? 1 : 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->stream));  // coef is a host buffer the caller may reuse
   return BSG_OK;

@@ -51,8 +51,14 @@ struct GpuFunctor : DavidsonFunctor {
     return true;
   }
   bool davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_diag, double normtol, bool useprecond,
-                int deflation_min, int deflation_max, DavidsonStats* st, int max_iter, double ladder) override {
-    block_davidson_device(sig, b, h_diag, normtol, useprecond, deflation_min, deflation_max, st, max_iter, ladder);
+                int deflation_min, int deflation_max, DavidsonStats* st, int max_iter, double ladder,
+                const std::vector<std::vector<double>>* lower) override {
+    // a subspace beyond the device solve's limit runs in the host Davidson
+    // (block_davidson) with this functor's GPU sigma
+    const int dmin = std::max(deflation_min, (int)b.size());
+    if (std::max(deflation_max, dmin + 1) + 1 > kDeviceDavidsonMaxVectors) return false;
+    block_davidson_device(sig, b, h_diag, normtol, useprecond, deflation_min, deflation_max, st, max_iter, ladder,
+                          lower);
     return true;
   }
 };
@@ -222,13 +228,51 @@ int blkh_gpu_sigma_timed(void* h, const double* c, int warmup, int reps, double*
   });
 }
 
+// One Davidson solve on a superblock with the GPU sigma (Linear::
+// block_davidson, linear.C:179-385): mode 0 runs the device-resident solve
+// (vectors in HBM), mode 1 the host control flow (block_davidson) calling
+// the same GPU sigma.  guesses: nroots vectors of wave_size (the solve starts
+// from them); lower: nlow states to project off (lowerStates, null if 0).
+// Deflation as the sweep's (min max(2, nroots), max max(20, nroots + 20)).
+// energies[nroots], vecs[nroots * wave_size] (may be null),
+// stats[3] = iterations, sigma calls, split-precision sigma calls.
+int blkh_davidson(void* sbp, int device, int mode, int nroots, const double* guesses, int nlow, const double* lower,
+                  double tol, double ladder, double* energies, double* vecs, int* stats) {
+  return guarded([&]() {
+    const SuperBlock& sb = *static_cast<SuperBlock*>(sbp);
+    const long N = sb.wave->size;
+    if (nroots < 1 || nlow < 0 || !guesses || (nlow > 0 && !lower) || !energies || !stats)
+      throw std::runtime_error("blkh_davidson: bad arguments");
+    GpuContext ctx(device);
+    GpuFunctor h(ctx, sb);
+    std::vector<std::vector<double>> b(nroots), low(nlow);
+    for (int r = 0; r < nroots; ++r) b[r].assign(guesses + (size_t)r * N, guesses + (size_t)(r + 1) * N);
+    for (int r = 0; r < nlow; ++r) low[r].assign(lower + (size_t)r * N, lower + (size_t)(r + 1) * N);
+    std::vector<double> e;
+    h.diagonal(e);
+    const int dmin = std::max(2, nroots);
+    const int dmax = nroots > 1 ? std::max(20, nroots + 20) : std::max(20, dmin + 1);
+    DavidsonStats st;
+    bool done = false;
+    if (mode == 0) done = h.davidson(b, e, tol, true, dmin, dmax, &st, 200, ladder, nlow ? &low : nullptr);
+    if (!done) block_davidson(b, e, tol, h, true, dmin, dmax, &st, 200, ladder, nlow ? &low : nullptr);
+    for (int r = 0; r < nroots; ++r) {
+      energies[r] = e[r];
+      if (vecs) std::copy(b[r].begin(), b[r].end(), vecs + (size_t)r * N);
+    }
+    stats[0] = st.iterations;
+    stats[1] = st.sigma_calls;
+    stats[2] = st.sigma_calls_split;
+    return done || mode != 0 ? 0 : 1;
+  });
+}
+
 // Two-site DMRG with the sigma on the GPU.  schedule: n_sched rows of
 // (start_sweep, M, davidson_tol, noise).  ladder > 0: split-precision sigma
 // until the Davidson residual norm falls below it, FP64 after.  ahead: host
 // operator builds beside the Davidson solve (1 on, 0 off, -1 default: on).
-// energies_out[nroots]; sweep_out[max_sweeps * 10]: sweep, forward, M, E0,
-// discarded, seconds, sigma_seconds, sigma_flops, sigma calls, split calls.
-// Returns the number of sweeps run.
+// energies_out[nroots]; sweep_out[max_sweeps * kSweepCols]: one row per sweep
+// (columns as write_sweep_records).  Returns the number of rows written.
 int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
                   const double* sched, int max_sweeps, double sweep_tol, double ladder, int ladder_from, int guess,
                   const char* checkpoint, const char* resume, int ahead, int device, int verbose,
@@ -259,8 +303,7 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
       return std::unique_ptr<DavidsonFunctor>(new GpuFunctor(ctx, sb));
     });
     for (int r = 0; r < nroots; ++r) energies_out[r] = res.energies[r];
-    write_sweep_records(res, sweep_out);
-    return (int)res.sweeps.size();
+    return write_sweep_records(res, sweep_out, max_sweeps);
   });
 }
 

@@ -42,8 +42,13 @@ void olsen_precondition(std::vector<double>& r, const std::vector<double>& c0, d
 
 void block_davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_diag, double normtol,
                     DavidsonFunctor& h, bool useprecond, int deflation_min, int deflation_max, DavidsonStats* st,
-                    int max_iter, double ladder) {
+                    int max_iter, double ladder, const std::vector<std::vector<double>>* lower) {
   const double levelshift = 0.0;
+  // x -= <x,l>/<l,l> l for every lower state in turn
+  auto project_lower = [&](std::vector<double>& x) {
+    if (!lower) return;
+    for (const auto& l : *lower) daxpy(-ddot(x, l) / ddot(l, l), l, x);
+  };
   bool split = ladder > 0.0 && h.set_precision(1);
   const int nroots = (int)b.size();
   // deflation keeps at least every root's vector
@@ -53,6 +58,10 @@ void block_davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_
   for (int i = 0; i < nroots; ++i) {
     for (int j = 0; j < i; ++j) daxpy(-ddot(b[j], b[i]), b[j], b[i]);
     normalise(b[i]);
+  }
+  if (lower && !lower->empty()) {  // linear.C:203-208
+    project_lower(b[0]);
+    normalise(b[0]);
   }
   std::vector<std::vector<double>> sigma;
   int converged_roots = 0;
@@ -103,6 +112,7 @@ void block_davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_
     }
     r = sigma[converged_roots];
     daxpy(-eval[converged_roots], b[converged_roots], r);
+    project_lower(r);  // linear.C:311-316
     double rnorm = ddot(r, r);
     if (split && (rnorm < ladder * ladder || rnorm < normtol)) {
       // ladder step: continue from the current Ritz vectors with FP64 sigmas
@@ -132,6 +142,7 @@ void block_davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_
         dscal(1.0 / std::sqrt(nrm), r);
         daxpy(-ddot(r, b[j]), b[j], r);
       }
+      project_lower(r);  // linear.C:369-375
       normalise(r);
       b.push_back(r);
     }

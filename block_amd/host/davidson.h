@@ -22,7 +22,7 @@ struct DavidsonFunctor {
   // arguments of block_davidson; false: the caller runs block_davidson
   virtual bool davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_diag, double normtol,
                         bool useprecond, int deflation_min, int deflation_max, struct DavidsonStats* st, int max_iter,
-                        double ladder) {
+                        double ladder, const std::vector<std::vector<double>>* lower) {
     return false;
   }
 };
@@ -44,6 +44,12 @@ struct DavidsonStats {
 // expansion.  On return b holds the Ritz vectors and h_diag[0..nroots) the
 // eigenvalues.
 //
+// lower (state-specific excited states, the reference's lowerStates): when
+// given and not empty, the first guess, every residual and every new
+// subspace vector are projected off these vectors (x -= <x,l>/<l,l> l for
+// each in turn, linear.C:203-208,311-316,369-375), so the solve converges to
+// the lowest states orthogonal to them.
+//
 // Precision ladder (ladder > 0 and a functor with split precision): sigmas
 // start in split precision; once the residual norm of the current root falls
 // below `ladder`, or it meets normtol while still in split precision, the
@@ -53,7 +59,8 @@ struct DavidsonStats {
 // throughout.
 void block_davidson(std::vector<std::vector<double>>& b, std::vector<double>& h_diag, double normtol,
                     DavidsonFunctor& h, bool useprecond, int deflation_min, int deflation_max, DavidsonStats* st,
-                    int max_iter = 1000, double ladder = 0.0);
+                    int max_iter = 1000, double ladder = 0.0,
+                    const std::vector<std::vector<double>>* lower = nullptr);
 
 // vector kernels of BaseOperator.C:240-288 / linear.C:27-60 on host vectors
 double ddot(const std::vector<double>& x, const std::vector<double>& y);

@@ -5,7 +5,7 @@
 // preconditioner, convergence on the squared residual norm, deflation and
 // re-orthogonalised expansion, plus the precision ladder.  Every vector of
 // length N (b, sigma, residual, diagonal) stays in HBM and is handled by the
-// bsg_* vector kernels; only the m x m subspace matrix (m <= 32) comes to the
+// bsg_* vector kernels; only the m x m subspace matrix (m <= 128) comes to the
 // host for the eigen-decomposition.
 #include <chrono>
 #include <cmath>
@@ -30,7 +30,7 @@ struct DevBuf {
 
 void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std::vector<double>& h_diag,
                            double normtol, bool useprecond, int deflation_min, int deflation_max, DavidsonStats* st,
-                           int max_iter, double ladder) {
+                           int max_iter, double ladder, const std::vector<std::vector<double>>* lower) {
   bsg_context* ctx = h.ctx();
   const int64_t N = h.wave_size();
   const int nroots = (int)b.size();
@@ -38,7 +38,10 @@ void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std
   deflation_min = std::max(deflation_min, nroots);
   deflation_max = std::max(deflation_max, deflation_min + 1);
   const int cap = deflation_max + 1;
-  if (cap > 32) throw std::runtime_error("block_davidson_device: subspace larger than 32 vectors");
+  if (cap > kDeviceDavidsonMaxVectors)
+    throw std::runtime_error("block_davidson_device: subspace larger than " +
+                             std::to_string(kDeviceDavidsonMaxVectors) + " vectors");
+  const int nlow = lower ? (int)lower->size() : 0;
   const double levelshift = 0.0;
   DevBuf B(ctx, (size_t)cap * N), S(ctx, (size_t)cap * N), T(ctx, (size_t)cap * N), R(ctx, N), Wk(ctx, N),
       D(ctx, N);
@@ -58,10 +61,26 @@ void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std
   for (int i = 0; i < nroots; ++i)
     bsg_check(bsg_memcpy_h2d(ctx, vec(Bp, i), b[i].data(), sizeof(double) * N), "guess upload");
   bsg_check(bsg_memcpy_h2d(ctx, D.p, h_diag.data(), sizeof(double) * N), "diagonal upload");
+  // lower states (linear.C:203-208,311-316,369-375): x -= <x,l>/<l,l> l in turn
+  DevBuf Lw(ctx, (size_t)std::max(1, nlow) * N);
+  std::vector<double> lnorm2(nlow);
+  for (int i = 0; i < nlow; ++i) {
+    if ((int64_t)(*lower)[i].size() != N) throw std::runtime_error("block_davidson_device: lower state of wrong size");
+    bsg_check(bsg_memcpy_h2d(ctx, vec(Lw.p, i), (*lower)[i].data(), sizeof(double) * N), "lower state upload");
+    lnorm2[i] = dot(vec(Lw.p, i), vec(Lw.p, i));
+  }
+  auto project_lower = [&](double* x) {
+    for (int i = 0; i < nlow; ++i)
+      bsg_check(bsg_axpy(ctx, -dot(x, vec(Lw.p, i)) / lnorm2[i], vec(Lw.p, i), x, N), "lower projection");
+  };
   // orthonormalise the guesses (linear.C:190-200)
   for (int i = 0; i < nroots; ++i) {
     for (int j = 0; j < i; ++j) bsg_check(bsg_axpy(ctx, -dot(vec(Bp, j), vec(Bp, i)), vec(Bp, j), vec(Bp, i), N), "axpy");
     normalise(vec(Bp, i));
+  }
+  if (nlow > 0) {
+    project_lower(vec(Bp, 0));
+    normalise(vec(Bp, 0));
   }
   bool split = ladder > 0.0;
   h.set_precision(split ? BSG_SPLIT_BF16 : BSG_FP64);
@@ -107,6 +126,7 @@ void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std
     const int k = converged_roots;
     bsg_check(bsg_copy(ctx, vec(Sp, k), R.p, N), "copy");
     bsg_check(bsg_axpy(ctx, -eval[k], vec(Bp, k), R.p, N), "axpy");
+    project_lower(R.p);
     const double rnorm = dot(R.p, R.p);
     if (split && (rnorm < ladder * ladder || rnorm < normtol)) {
       // ladder step: continue from the current Ritz vectors with FP64 sigmas
@@ -134,7 +154,14 @@ void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std
         nsig = std::min(nsig, m);
       }
       // linear.C:358-366 and the Normalise after it, scalars on the device
-      bsg_check(bsg_gs_orthonormalise(ctx, R.p, Bp, m, N, N), "orthonormalise");
+      if (nlow > 0) {
+        // the lower-state projection sits between the loop and the Normalise
+        bsg_check(bsg_gs_orthonormalise(ctx, R.p, Bp, m, N, N, 0), "orthogonalise");
+        project_lower(R.p);
+        normalise(R.p);
+      } else {
+        bsg_check(bsg_gs_orthonormalise(ctx, R.p, Bp, m, N, N, 1), "orthonormalise");
+      }
       bsg_check(bsg_copy(ctx, R.p, vec(Bp, m), N), "copy");
       ++m;
     }

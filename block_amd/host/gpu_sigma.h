@@ -142,10 +142,12 @@ class GpuRotator : public OpRotator {
   GpuContext& g_;
 };
 
+// Largest Davidson subspace (deflation_max + 1 vectors) the device solve takes.
+constexpr int kDeviceDavidsonMaxVectors = 128;
 // Linear::block_davidson on device vectors with the sigma of h (same control
 // flow and outputs as blk::block_davidson; b, h_diag host vectors in/out).
 void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std::vector<double>& h_diag,
                            double normtol, bool useprecond, int deflation_min, int deflation_max, DavidsonStats* st,
-                           int max_iter, double ladder);
+                           int max_iter, double ladder, const std::vector<std::vector<double>>* lower = nullptr);
 
 }  // namespace blk

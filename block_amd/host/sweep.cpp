@@ -249,7 +249,7 @@ Solved solve_superblock(const SuperBlock& sb, const DmrgParams& P, double tol, c
   // grows to nroots + 20 (the nroots keyword, input.C:882-885)
   const int dmin = std::max(P.deflation_min, P.nroots);
   const int dmax = P.nroots > 1 ? std::max(P.deflation_max, P.nroots + 20) : std::max(P.deflation_max, dmin + 1);
-  if (!hfun->davidson(out.b, out.e, tol, true, dmin, dmax, &out.st, P.davidson_max_iter, ladder))
+  if (!hfun->davidson(out.b, out.e, tol, true, dmin, dmax, &out.st, P.davidson_max_iter, ladder, nullptr))
     block_davidson(out.b, out.e, tol, *hfun, true, dmin, dmax, &out.st, P.davidson_max_iter, ladder);
   out.e.resize(P.nroots);
   out.st.setup_seconds = t1 - t0;
@@ -443,6 +443,13 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
     SweepState st = load_sweep_state(P.resume);
     if ((int)st.fwd.size() != k || (int)st.bwd.size() != k)
       throw std::runtime_error("run_dmrg: the saved state is for another number of orbitals");
+    if (st.next_sweep > P.max_sweeps)
+      throw std::runtime_error("run_dmrg: the saved state is past max_sweeps (" + std::to_string(st.next_sweep) +
+                               " sweeps done)");
+    if ((int)st.b.size() != P.nroots || (int)st.res.energies.size() != P.nroots)
+      throw std::runtime_error("run_dmrg: the saved state has another number of roots");
+    if (!st.wave || !(st.wave->target == target))
+      throw std::runtime_error("run_dmrg: the saved state is for another target (N, 2S, irrep)");
     fwd = std::move(st.fwd);
     bwd = std::move(st.bwd);
     prev.wave = st.wave;
@@ -656,8 +663,9 @@ SuperBlock dmrg_snapshot(const Integrals& I, const DmrgParams& P, const SigmaFac
   return infinite_warmup(I, P, make_sigma, fwd, bwd, rng);
 }
 
-void write_sweep_records(const DmrgResult& res, double* out) {
-  for (size_t i = 0; i < res.sweeps.size(); ++i) {
+int write_sweep_records(const DmrgResult& res, double* out, int max_rows) {
+  const int n = std::min<int>((int)res.sweeps.size(), std::max(0, max_rows));
+  for (int i = 0; i < n; ++i) {
     const auto& s = res.sweeps[i];
     double* o = out + kSweepCols * i;
     o[0] = s.sweep;
@@ -675,6 +683,7 @@ void write_sweep_records(const DmrgResult& res, double* out) {
     o[12] = s.renorm_seconds;
     o[13] = (double)s.guessed_steps;
   }
+  return n;
 }
 
 }  // namespace blk

@@ -79,8 +79,9 @@ struct DmrgResult {
 constexpr int kSweepCols = 14;
 // sweep, forward, M, E0, max discarded weight, seconds, sigma seconds, sigma
 // flops, sigma calls, split-precision sigma calls, build seconds, other
-// Davidson seconds, renormalisation seconds, steps started from a guess
-void write_sweep_records(const DmrgResult& res, double* out);
+// Davidson seconds, renormalisation seconds, steps started from a guess.
+// out holds max_rows rows; returns the number of rows written.
+int write_sweep_records(const DmrgResult& res, double* out, int max_rows);
 
 using SigmaFactory = std::function<std::unique_ptr<DavidsonFunctor>(const SuperBlock&)>;
 

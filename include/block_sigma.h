@@ -115,9 +115,15 @@ int bsg_plan_phase_flops(const bsg_plan* plan, double* phase1, double* phase2);
 int64_t bsg_plan_arena_extent(const bsg_plan* plan);
 
 /* Split precision set-up: checks that the arena holds the plan's extent and
- * enables BSG_SPLIT_BF16 for this arena.  The kernels form the BF16 slices in
- * registers (x = x0 + x1 + x2 of the FP32 rounding of x, exact), so nothing
- * is allocated; call again after the arena moves.
+ * enables BSG_SPLIT_BF16 for this arena.  On its first call for a plan it
+ * allocates the plan's BF16 slice buffers: three slices x = x0 + x1 + x2 of
+ * the FP32 rounding of x (exact) for every operator block the grouped kernel
+ * reads, in each orientation it reads it (6 B per element and orientation,
+ * rows padded to a multiple of 8; about 0.75x the FP64 arena), and the
+ * transposed slices of the wavefunction's C blocks (6 B per element), which
+ * every split-precision application rewrites.  It then slices the arena's
+ * operator blocks; call again after the arena moves or its contents change.
+ * The small-block kernel forms its slices in registers.
  * No counterpart in the reference (its sigma is FP64 only). */
 int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* plan, const double* arena, int64_t n);
 
@@ -164,10 +170,12 @@ typedef struct {
 } bsg_copy_seg_t;
 int bsg_copy_segments(bsg_context* ctx, const bsg_copy_seg_t* segs_host, int32_t n_segs);
 /* Subspace matrix of the block Davidson (linear.C:265-270): out_host[i*m+j]
- * = <X_i, Y_j> for m <= 32 device vectors of length n stored with stride ld. */
+ * = <X_i, Y_j> for m <= 128 device vectors of length n stored with stride ld
+ * (the reference's subspace holds max(20, nroots+20) vectors, input.C:885). */
 int bsg_gram(bsg_context* ctx, const double* X, const double* Y, int32_t m, int64_t n, int64_t ld, double* out_host);
 /* Ritz rotation (linear.C:279-294): Y_j = sum_i coef_host[i*mo+j] X_i for
- * j < mo; X, Y device vector sets (strides ld, ldy; must not overlap). */
+ * j < mo, m and mo <= 128; X, Y device vector sets (strides ld, ldy; must not
+ * overlap). */
 int bsg_lincomb(bsg_context* ctx, const double* X, int32_t m, int64_t n, int64_t ld, const double* coef_host,
                 int32_t mo, double* Y, int64_t ldy);
 /* x[i] /= (e - diag[i] + shift) where |e - diag[i]| > 1e-12 (linear.C:27-47) */
@@ -177,12 +185,14 @@ int bsg_precondition(bsg_context* ctx, double* x, const double* diag, double e, 
 int bsg_olsen_precondition(bsg_context* ctx, double* r, const double* c, const double* diag, double e, double shift,
                            double* work, int64_t n);
 /* Gram-Schmidt of the new Davidson vector (linear.C:358-366 and the
- * Normalise that follows): for j < m, r /= |r|, r -= <r, b_j> b_j; then r
- * is normalised unless |r|^2 <= 1e-15 (BaseOperator.C:282-287).  B: m
- * device vectors at stride ld.  The scalars stay in device memory, so the
- * loop runs without a host round trip per step; same reductions and order
- * as bsg_dot / bsg_scal / bsg_axpy. */
-int bsg_gs_orthonormalise(bsg_context* ctx, double* r, const double* B, int32_t m, int64_t ld, int64_t n);
+ * Normalise that follows): for j < m, r /= |r|, r -= <r, b_j> b_j; then, if
+ * normalise != 0, r is normalised unless |r|^2 <= 1e-15 (BaseOperator.C:
+ * 282-287; a caller that projects r further before that Normalise passes 0).
+ * B: m device vectors at stride ld.  The scalars stay in device memory, so
+ * the loop runs without a host round trip per step; same reductions and
+ * order as bsg_dot / bsg_scal / bsg_axpy. */
+int bsg_gs_orthonormalise(bsg_context* ctx, double* r, const double* B, int32_t m, int64_t ld, int64_t n,
+                          int32_t normalise);
 
 #ifdef __cplusplus
 }

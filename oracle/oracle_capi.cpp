@@ -132,6 +132,33 @@ void* orc_exact_superblock(void* ints, int nelec, int twoS, int irrep, int nL) {
   }
 }
 
+// One Davidson solve (block_davidson, linear.C:179-385) with the oracle
+// sigma and diagonal; arguments as blkh_davidson (host control flow).
+int orc_davidson(void* sbp, int nroots, const double* guesses, int nlow, const double* lower, double tol,
+                 int nthreads, double* energies, double* vecs, int* stats) {
+  return guarded([&]() {
+    const SuperBlock& sb = *static_cast<SuperBlock*>(sbp);
+    const long N = sb.wave->size;
+    OracleFunctor h(sb, nthreads);
+    std::vector<std::vector<double>> b(nroots), low(nlow);
+    for (int r = 0; r < nroots; ++r) b[r].assign(guesses + (size_t)r * N, guesses + (size_t)(r + 1) * N);
+    for (int r = 0; r < nlow; ++r) low[r].assign(lower + (size_t)r * N, lower + (size_t)(r + 1) * N);
+    std::vector<double> e = oracle::oracle_diagonal(sb);
+    const int dmin = std::max(2, nroots);
+    const int dmax = nroots > 1 ? std::max(20, nroots + 20) : std::max(20, dmin + 1);
+    DavidsonStats st;
+    block_davidson(b, e, tol, h, true, dmin, dmax, &st, 200, 0.0, nlow ? &low : nullptr);
+    for (int r = 0; r < nroots; ++r) {
+      energies[r] = e[r];
+      if (vecs) std::copy(b[r].begin(), b[r].end(), vecs + (size_t)r * N);
+    }
+    stats[0] = st.iterations;
+    stats[1] = st.sigma_calls;
+    stats[2] = st.sigma_calls_split;
+    return 0;
+  });
+}
+
 // the sweep driver with the oracle sigma (CPU reference run); same outputs as
 // blkh_dmrg_gpu (sweep_out rows of kSweepCols)
 int orc_dmrg_cpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
@@ -158,8 +185,7 @@ int orc_dmrg_cpu(void* ints, int nelec, int twoS, int irrep, int nroots, const d
       return std::unique_ptr<DavidsonFunctor>(new OracleFunctor(sb, nthreads));
     });
     for (int r = 0; r < nroots; ++r) energies_out[r] = res.energies[r];
-    write_sweep_records(res, sweep_out);
-    return (int)res.sweeps.size();
+    return write_sweep_records(res, sweep_out, max_sweeps);
   });
 }
 

@@ -22,6 +22,8 @@ for name, res, args in [
     ("orc_exec_worklist", ctypes.c_int, [_vp, _dp, _dp]),
     ("orc_fci", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp]),
     ("orc_exact_superblock", _vp, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("orc_davidson", ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, _dp,
+                                    _dp, ctypes.POINTER(ctypes.c_int)]),
     ("orc_dmrg_cpu", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int,
                                     _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_char_p,
                                     ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
@@ -91,6 +93,20 @@ def dense_h(sb, nthreads=4):
         e[i] = 1.0
         H[:, i] = sigma(sb, e, nthreads)
     return H
+
+
+def davidson(sb, guesses, tol=1e-12, lower=None, nthreads=4):
+    """block_davidson with the oracle sigma from the given guesses (rows);
+    returns (energies, vectors, (iterations, sigma calls, split calls))."""
+    g = np.ascontiguousarray(np.atleast_2d(guesses), dtype=np.float64)
+    nroots = g.shape[0]
+    low = None if lower is None or len(lower) == 0 else np.ascontiguousarray(np.atleast_2d(lower), dtype=np.float64)
+    e = np.zeros(nroots)
+    vecs = np.zeros_like(g)
+    st = (ctypes.c_int * 3)()
+    _chk(lib.orc_davidson(sb._h, nroots, _p(g), 0 if low is None else low.shape[0], None if low is None else _p(low),
+                          tol, nthreads, _p(e), _p(vecs), st))
+    return e, vecs, (st[0], st[1], st[2])
 
 
 def dmrg_cpu(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=None, max_sweeps=20,

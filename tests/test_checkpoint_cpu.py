@@ -31,6 +31,24 @@ def test_resume_continues_the_sweeps(tmp_path):
     assert np.all(sw[3:, COLS.index("guessed_steps")] == ints.norb - 3)
 
 
+def test_resume_rejects_a_mismatched_run(tmp_path):
+    """A state saved after three sweeps cannot continue a run of fewer sweeps
+    (its records would not fit the caller's buffer), of another number of
+    roots or of another target."""
+    ints = config_fcidump("hubbard12")
+    ck = str(tmp_path / "state.bin")
+    _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=3, sweep_tol=0.0, checkpoint=ck)
+    with pytest.raises(RuntimeError, match="max_sweeps"):
+        _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=1, sweep_tol=0.0, resume=ck)
+    with pytest.raises(RuntimeError, match="roots"):
+        _oracle.dmrg_cpu(ints, nroots=2, schedule=SCHED, max_sweeps=6, sweep_tol=0.0, resume=ck)
+    with pytest.raises(RuntimeError, match="target"):
+        _oracle.dmrg_cpu(ints, twoS=2, schedule=SCHED, max_sweeps=6, sweep_tol=0.0, resume=ck)
+    # exactly as many sweeps as saved: nothing left to run, the saved records come back
+    e, sw = _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=3, sweep_tol=0.0, resume=ck)
+    assert sw.shape[0] == 3
+
+
 def test_resume_rejects_garbage(tmp_path):
     ints = config_fcidump("hubbard12")
     bad = tmp_path / "bad.bin"

@@ -20,6 +20,9 @@ def _cases():
         ("hubbard12_M100", lambda: block_amd.SuperBlock(config_fcidump("hubbard12"), 12, 0, 0, 5, 100)),
         ("syn16_M80", lambda: block_amd.SuperBlock(random_fcidump(16, 21), 16, 0, 0, 7, 80)),
         ("syn16d2h_M80", lambda: block_amd.SuperBlock(random_fcidump(16, 22, d2h=True), 16, 0, 0, 7, 80)),
+        # total spin S = 1: the St-dependent 9j factors and signs of the work list
+        ("exact6_S1", lambda: _oracle.exact_superblock(random_fcidump(6, 3), 6, 2, 0, 3)),
+        ("syn16_M80_S1", lambda: block_amd.SuperBlock(random_fcidump(16, 21), 16, 2, 0, 7, 80)),
         # mid-chain superblock of the 24-orbital DMRG warm-up at M = 500: V
         # blocks up to ~350 wide, the wide-tile kernel classes
         ("syn24_M500_dmrg", lambda: block_amd.SuperBlock.from_dmrg_warmup(config_fcidump("syn24"), 24, 0, 0, 500)),
