@@ -48,6 +48,8 @@ _sig(_h, "blkh_snapshot", _vp, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, c
 _sig(_h, "blkh_snapshot_dmrg", _vp, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int])
 _sig(_h, "blkh_free_superblock", None, [_vp])
+_sig(_h, "blkh_superblock_from_state", _vp, [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int])
 _sig(_h, "blkh_wave_size", ctypes.c_long, [_vp])
 _sig(_h, "blkh_flops", ctypes.c_double, [_vp])
 _sig(_h, "blkh_nterms", ctypes.c_int, [_vp])
@@ -64,12 +66,16 @@ _sig(_h, "blkh_gpu_sigma_flops", ctypes.c_double, [_vp])
 _sig(_h, "blkh_gpu_sigma_pairs", ctypes.c_long, [_vp])
 _sig(_h, "blkh_gpu_sigma_phase_flops", ctypes.c_int, [_vp, _dp, _dp])
 _sig(_h, "blkh_gpu_sigma_arena_bytes", ctypes.c_long, [_vp])
-_sig(_h, "blkh_gpu_sigma_timed", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _lp])
+_sig(_h, "blkh_gpu_sigma_timed", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _lp])
+_sig(_h, "blkh_gpu_sigma_set_batch", ctypes.c_int, [_vp, ctypes.c_int])
+_sig(_h, "blkh_gpu_sigma_apply_n", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int])
 _sig(_h, "blkh_gpu_sigma_set_precision", ctypes.c_int, [_vp, ctypes.c_int])
+_sig(_h, "blkh_gpu_sigma_raw", ctypes.c_int, [_vp] + [ctypes.POINTER(_vp)] * 5)
 _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
      [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
-      ctypes.c_int, ctypes.c_int, _dp, _dp])
+      ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp])
+_sig(_h, "blkh_build_check", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _lp])
 
 _sig(_h, "blkh_davidson", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp,
                                         ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.POINTER(ctypes.c_int)])
@@ -162,6 +168,19 @@ class SuperBlock:
             raise _err()
         return cls(ints, nelec, twoS, irrep, 0, M, _handle=h)
 
+    @classmethod
+    def from_sweep_state(cls, ints, path, nelec=None, twoS=0, irrep=None, step=-1):
+        """The two-site superblock of forward step `step` (sys = orbitals
+        0..step, dots step+1 and step+2, environment the rest; -1: the middle
+        of the chain) rebuilt from the blocks of a saved sweep state (a
+        checkpoint file of dmrg)."""
+        nelec = ints.nelec if nelec is None else nelec
+        irrep = ints.isym if irrep is None else irrep
+        h = host_lib.blkh_superblock_from_state(ints._h, str(path).encode(), nelec, twoS, irrep, step)
+        if not h:
+            raise _err()
+        return cls(ints, nelec, twoS, irrep, 0, 0, _handle=h)
+
     def sectors(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         host_lib.blkh_nsectors(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
@@ -193,12 +212,13 @@ class multiply_h:
     every operator block resident in HBM; calls run the fused MFMA kernels in
     FP64 or, with ``precision=SPLIT_BF16``, as split BF16 slice products."""
 
-    def __init__(self, sb, device=0, precision=FP64):
+    def __init__(self, sb, device=0, precision=FP64, max_vec=1):
         self.sb = sb
-        self._h = host_lib.blkh_gpu_sigma_create(sb._h, device, 0)
+        self._h = host_lib.blkh_gpu_sigma_create(sb._h, device, int(max_vec))
         if not self._h:
             raise _err()
         self.n = sb.wave_size
+        self.max_vec = int(max_vec)
         self.precision = precision
 
     @property
@@ -221,13 +241,32 @@ class multiply_h:
         return a.value, b.value
 
     def __call__(self, c):
+        """H c for one vector (shape (n,)) or for several (shape (k, n), k <=
+        max_vec: the roots run together in launches of the plan's batch)."""
         c = np.ascontiguousarray(c, dtype=np.float64)
-        if c.shape != (self.n,):
-            raise ValueError(f"wavefunction must have shape ({self.n},)")
-        v = np.zeros(self.n)
-        if host_lib.blkh_gpu_sigma_apply(self._h, _ptr(c), _ptr(v)) != 0:
+        if c.shape == (self.n,):
+            v = np.zeros(self.n)
+            if host_lib.blkh_gpu_sigma_apply(self._h, _ptr(c), _ptr(v)) != 0:
+                raise _err()
+            return v
+        if c.ndim != 2 or c.shape[1] != self.n or not 1 <= c.shape[0] <= self.max_vec:
+            raise ValueError(f"wavefunctions must have shape ({self.n},) or (k <= {self.max_vec}, {self.n})")
+        v = np.zeros_like(c)
+        if host_lib.blkh_gpu_sigma_apply_n(self._h, _ptr(c), _ptr(v), c.shape[0]) != 0:
             raise _err()
         return v
+
+    def set_batch(self, nroot):
+        """Roots per launch (bsg_plan_set_batch)."""
+        if host_lib.blkh_gpu_sigma_set_batch(self._h, int(nroot)) != 0:
+            raise _err()
+
+    def raw(self):
+        """The C-ABI handles (bsg_context*, bsg_plan*, arena, device c, device v
+        of wave_size doubles each) for calling libblock_sigma.so directly."""
+        out = [_vp() for _ in range(5)]
+        host_lib.blkh_gpu_sigma_raw(self._h, *[ctypes.byref(x) for x in out])
+        return tuple(x.value for x in out)
 
     def diagonal(self):
         out = np.zeros(self.n)
@@ -236,16 +275,18 @@ class multiply_h:
         return out
 
     def timed(self, c, warmup, reps):
-        """Device-resident repeated sigma in the current precision: returns
-        (wall_ms, [fused, reduce, split] kernel ms, [launches of each]) measured
-        with HIP events on the plan stream."""
+        """Device-resident repeated sigma in the current precision on one
+        vector (n,) or several (k, n): returns (wall_ms, [fused, reduce, side]
+        kernel ms, kernel dispatches) over the `reps` timed applications,
+        measured with HIP events on the plan stream."""
         c = np.ascontiguousarray(c, dtype=np.float64)
+        nvec = 1 if c.ndim == 1 else c.shape[0]
         wall = ctypes.c_double()
         km = np.zeros(3)
         la = (ctypes.c_long * 3)()
-        if host_lib.blkh_gpu_sigma_timed(self._h, _ptr(c), warmup, reps, ctypes.byref(wall), _ptr(km), la) != 0:
+        if host_lib.blkh_gpu_sigma_timed(self._h, _ptr(c), nvec, warmup, reps, ctypes.byref(wall), _ptr(km), la) != 0:
             raise _err()
-        return wall.value, km.tolist(), [la[0], la[1], la[2]]
+        return wall.value, km.tolist(), la[0]
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -260,9 +301,12 @@ def _schedule_array(schedule):
 
 def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=None, max_sweeps=20,
          sweep_tol=1e-8, device=0, verbose=0, ladder=0.0, guess=True, ladder_from=0, checkpoint=None, resume=None,
-         host_ahead=True):
+         host_ahead=True, device_blocks=True):
     """Two-site spin-adapted DMRG (Sweep::do_one, sweep.C:341) with the Davidson
     sigma on the GPU.  schedule: rows (start_sweep, M, davidson_tol, noise).
+    device_blocks: build the composite operators in HBM from device-resident
+    renormalised blocks (no operator crosses PCIe); False: host builds and
+    uploads (then host_ahead applies).
     ladder > 0: split-precision sigma until the Davidson residual norm falls
     below it, FP64 after (convergence is always decided on FP64 sigmas), from
     sweep ladder_from on (0: also in the warm-up).
@@ -286,11 +330,23 @@ def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=
     n = host_lib.blkh_dmrg_gpu(ints._h, nelec, twoS, irrep, nroots, _ptr(w), len(schedule), _ptr(sched), max_sweeps,
                                sweep_tol, float(ladder), int(ladder_from), int(guess),
                                None if checkpoint is None else str(checkpoint).encode(),
-                               None if resume is None else str(resume).encode(), int(bool(host_ahead)), device,
-                               verbose, _ptr(e), _ptr(sw))
+                               None if resume is None else str(resume).encode(), int(bool(host_ahead)),
+                               int(bool(device_blocks)), device, verbose, _ptr(e), _ptr(sw))
     if n < 0:
         raise _err()
     return e, sw[: nc * n].reshape(n, nc)
+
+
+def build_check(ints, nsys, M, device=0):
+    """Composite operators built in HBM against the host build (a Wilson-grown
+    block of nsys sites and the next dot, normal and complementary operator
+    sets, from a host block and from a device-only one); returns (largest
+    absolute element difference, operators compared)."""
+    worst = ctypes.c_double()
+    n = ctypes.c_long()
+    if host_lib.blkh_build_check(ints._h, nsys, M, device, ctypes.byref(worst), ctypes.byref(n)) != 0:
+        raise _err()
+    return worst.value, n.value
 
 
 def davidson(sb, guesses, tol=1e-12, lower=None, ladder=0.0, device=0, host_loop=False):

@@ -49,6 +49,25 @@ struct TileRef {
   int32_t item, tm, tn, pad;
 };
 
+// Several right-hand sides (roots) in one launch: root r reads c + r
+// wstride, accumulates into v + r wstride, keeps its partial tiles at part +
+// r pstride and its BF16 C slices at cslices + r cstride.
+struct RootSet {
+  int32_t n;
+  int32_t pad;
+  int64_t wstride, pstride, cstride;
+};
+// Workgroup L of a launch over rs.n roots -> (tile t, root): the roots of one
+// tile are dispatched next to each other, eight workgroups apart, so tile t
+// runs on XCD t mod 8 (workgroup L runs on XCD L mod 8) for every root, and
+// the roots share the tile's operator blocks in that XCD's L2.  The launch
+// has roundup(ntiles, 8) rs.n workgroups.
+__device__ inline int root_of(int L, int nroot, int& t) {
+  const int g = L / (8 * nroot), rem = L - g * 8 * nroot;
+  t = g * 8 + (rem & 7);
+  return rem >> 3;
+}
+
 
 
 // A work unit: pairs [first, first+count) of one V block.  p_off < 0: the
@@ -66,8 +85,10 @@ struct RedItem {
 };
 
 __global__ __launch_bounds__(256) void k_reduce(const RedItem* __restrict__ items, const double* __restrict__ part,
-                                                double* __restrict__ v) {
+                                                double* __restrict__ v, RootSet rs) {
   const RedItem it = items[blockIdx.x];
+  part += blockIdx.y * rs.pstride;
+  v += blockIdx.y * rs.wstride;
   const int e = threadIdx.x;
   if (e >= it.len) return;
   const int64_t x = it.e0 + e;
@@ -217,7 +238,7 @@ template <int NRM, int NV, int PBW = PB, int NWV = 4>
 __global__ __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(NV == 4 ? 3 : 2))) void k_sigma(
     const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
     const TileRef* __restrict__ tiles, int ntiles, const double* __restrict__ arena, const double* __restrict__ c,
-    double* __restrict__ v, double* __restrict__ part) {
+    double* __restrict__ v, double* __restrict__ part, RootSet rs) {
   static_assert(NV % PBW == 0, "V tile width is a whole number of W chunks");
   static_assert(NWV == 4 || (NWV == 8 && NRM == 1 && PBW == PB), "eight waves: one wave row, 16-row tiles");
   constexpr int NT = 64 * NWV;
@@ -230,8 +251,12 @@ __global__ __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(NV == 
   double* const As0 = lds;
   double* const As1 = lds + PKC * GP;
   double* const Ws = lds;
-  const int tid = blockIdx.x;
+  int tid;
+  const int root = root_of(blockIdx.x, rs.n, tid);
   if (tid >= ntiles) return;
+  c += root * rs.wstride;
+  v += root * rs.wstride;
+  part += root * rs.pstride;
   const TileRef tr = tiles[tid];
   if (tr.item < 0) return;  // padding of the XCD-interleaved order
   const Seg sg = segs[tr.item];
@@ -529,8 +554,10 @@ __device__ inline void mma6(dbl4& acc, const Frag& a, const Frag& b) {
 }
 
 __global__ __launch_bounds__(256) void k_slices(const SliceItem* __restrict__ items, const double* __restrict__ src,
-                                                __bf16* __restrict__ dst) {
+                                                __bf16* __restrict__ dst, int64_t src_stride, int64_t dst_stride) {
   const SliceItem it = items[blockIdx.x];
+  src += blockIdx.y * src_stride;
+  dst += blockIdx.y * dst_stride;
   const int64_t tot = (int64_t)it.rows * it.pitch;
   for (int64_t e = threadIdx.x; e < tot; e += 256) {
     const int r = (int)(e / it.pitch), k = (int)(e % it.pitch);
@@ -558,13 +585,18 @@ __global__ __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(2))) v
     const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const PairS* __restrict__ pss,
     const Seg* __restrict__ segs, const TileRef* __restrict__ tiles, int ntiles, const __bf16* __restrict__ slices,
     const __bf16* __restrict__ cslices, const double* __restrict__ c, double* __restrict__ v,
-    double* __restrict__ part) {
+    double* __restrict__ part, RootSet rs) {
   // two op(A) chunk buffers for GEMM 1, the W chunk for GEMM 2 (aliased,
   // barriers separate the phases)
   constexpr int LDSE = 2 * SAB > SWMAX ? 2 * SAB : SWMAX;
   __shared__ __attribute__((aligned(16))) __bf16 lds[LDSE];
-  const int tid = blockIdx.x;
+  int tid;
+  const int root = root_of(blockIdx.x, rs.n, tid);
   if (tid >= ntiles) return;
+  c += root * rs.wstride;
+  v += root * rs.wstride;
+  part += root * rs.pstride;
+  cslices += root * rs.cstride;
   const TileRef tr = tiles[tid];
   if (tr.item < 0) return;
   const Seg sg = segs[tr.item];
@@ -843,11 +875,15 @@ template <int SB, bool SPLIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_small(
     const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
     const int4* __restrict__ units, int nunits, const double* __restrict__ arena, const double* __restrict__ c,
-    double* __restrict__ v, double* __restrict__ part) {
+    double* __restrict__ v, double* __restrict__ part, RootSet roots) {
   constexpr int KF = 4 * SB;  // 4-deep K steps of a dimension <= 16 SB
   static_assert(KF == 8, "one 32-deep K group per product");
-  const int u = blockIdx.x;
+  int u;
+  const int root = root_of(blockIdx.x, roots.n, u);
   if (u >= nunits) return;
+  c += root * roots.wstride;
+  v += root * roots.wstride;
+  part += root * roots.pstride;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int4 un = units[u];
   const int si = w == 0 ? un.x : (w == 1 ? un.y : (w == 2 ? un.z : un.w));
@@ -1222,6 +1258,37 @@ __global__ __launch_bounds__(256) void k_copy_chunks(const bsg_copy_seg_t* __res
   for (int64_t i = threadIdx.x; i < c.n; i += 256) c.dst[i] = c.src[i];
 }
 
+// Composite-operator build: groups of scaled block copies.  Group g owns the
+// destination rectangle dst[r ld + c] (r < rows, c < cols) and the run of
+// sources [first, first + count); its elements are numbered from
+// estart[g] in one flat index space, 256 per workgroup, so small rectangles
+// share workgroups.  Each element sums its sources in list order,
+// (alpha a) beta per source, and adds the sum to the destination (zero
+// beforehand): bitwise the host build's accumulation.
+__global__ __launch_bounds__(256) void k_build(const bsg_build_group_t* __restrict__ groups,
+                                               const int64_t* __restrict__ estart, int ngroups,
+                                               const bsg_build_src_t* __restrict__ srcs, int64_t nelem) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= nelem) return;
+  // last group whose start is <= e
+  int lo = 0, hi = ngroups - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (estart[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  const bsg_build_group_t g = groups[lo];
+  const int64_t x = e - estart[lo];
+  const int r = (int)(x / g.cols), c = (int)(x % g.cols);
+  double s = 0.0;
+  for (int j = 0; j < g.count; ++j) {
+    const bsg_build_src_t sj = srcs[g.first + j];
+    const double a = sj.trans ? sj.src[(int64_t)c * sj.ld + r] : sj.src[(int64_t)r * sj.ld + c];
+    s += (sj.alpha * a) * sj.beta;
+  }
+  g.dst[(int64_t)r * g.ld + c] += s;
+}
+
 // Gram matrix tile of two vector sets (mx, my <= GMAX vectors of length n,
 // stride ld): per-block partial sums G[i my + j] = sum_x X_i[x] Y_j[x] over
 // the block's chunks, then an ordered sum over blocks (deterministic).
@@ -1359,7 +1426,10 @@ struct PlanChunk {
 struct bsg_plan {
   bsg_context* ctx = nullptr;
   std::vector<PlanChunk> chunks;
-  double* part = nullptr;
+  double* part = nullptr;    // partial tiles of `batch` roots, part_doubles apart
+  int64_t part_doubles = 0;  // partial tiles of split V blocks per root (largest chunk)
+  int32_t batch = 1;         // roots per launch (bsg_plan_set_batch)
+  bool cs_ready = false;     // d_cs holds C slices for `batch` roots
   int64_t wave_size = 0;
   int64_t arena_extent = 0;  // doubles of operator arena the pairs reach
   double flops = 0.0, flops1 = 0.0, flops2 = 0.0;
@@ -1720,8 +1790,9 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     pc.n_red[1] = (int)reds[1].size();
     p->chunks.push_back(pc);
   }
+  p->part_doubles = pmax;
   if (pmax > 0) {
-    hipError_t e = hipMalloc(&p->part, sizeof(double) * pmax);
+    hipError_t e = hipMalloc(&p->part, sizeof(double) * pmax * p->batch);
     if (e != hipSuccess) return cleanup(BSG_ERR_NOMEM, "bsg_plan_create: partial-tile scratch");
   }
   for (auto* v : {&p->ab_items, &p->c_items})
@@ -1760,25 +1831,52 @@ int bsg_plan_destroy(bsg_plan* p) {
 }
 
 int64_t bsg_plan_arena_extent(const bsg_plan* p) { return p ? p->arena_extent : 0; }
+int64_t bsg_plan_partial_doubles(const bsg_plan* p) { return p ? p->part_doubles : 0; }
 
 int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* p, const double* arena, int64_t n) {
   if (!ctx || !p || !arena || n < 0) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: bad args");
   if (p->ctx != ctx) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: plan belongs to another context");
   if (n < p->arena_extent) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: arena smaller than the plan's extent");
   // operator slices for the grouped kernel (the small-block kernel splits
-  // in registers), and room for the C slices of one application
+  // in registers), and room for the C slices of one launch's roots
   if (!p->d_abs && p->ab_elems > 0) {
     hipError_t e = hipMalloc((void**)&p->d_abs, 2 * p->ab_elems);
     if (e != hipSuccess) return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: operator slices");
   }
-  if (!p->d_cs && p->c_elems > 0) {
-    hipError_t e = hipMalloc((void**)&p->d_cs, 2 * p->c_elems);
+  if (!p->cs_ready && p->c_elems > 0) {
+    hipError_t e = hipMalloc((void**)&p->d_cs, 2 * p->c_elems * p->batch);
     if (e != hipSuccess) return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: wavefunction slices");
   }
+  p->cs_ready = true;
   if (!p->ab_items.empty())
-    hipLaunchKernelGGL(k_slices, dim3(p->ab_items.size()), dim3(256), 0, ctx->stream, p->d_ab_items, arena, p->d_abs);
+    hipLaunchKernelGGL(k_slices, dim3(p->ab_items.size()), dim3(256), 0, ctx->stream, p->d_ab_items, arena, p->d_abs,
+                       (int64_t)0, (int64_t)0);
   HIPCHK(hipGetLastError());
   p->split_src = arena;
+  return BSG_OK;
+}
+
+int bsg_plan_set_batch(bsg_context* ctx, bsg_plan* p, int32_t nroot) {
+  if (!ctx || !p || nroot < 1) return fail(BSG_ERR_ARG, "bsg_plan_set_batch: bad args");
+  if (p->ctx != ctx) return fail(BSG_ERR_ARG, "bsg_plan_set_batch: plan belongs to another context");
+  if (nroot == p->batch) return BSG_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // no queued work reads the old buffers
+  if (p->part_doubles > 0) {
+    double* np = nullptr;
+    if (hipMalloc(&np, sizeof(double) * p->part_doubles * nroot) != hipSuccess)
+      return fail(BSG_ERR_NOMEM, "bsg_plan_set_batch: partial-tile scratch");
+    (void)hipFree(p->part);
+    p->part = np;
+  }
+  if (p->cs_ready && p->c_elems > 0) {
+    __bf16* nc = nullptr;
+    if (hipMalloc((void**)&nc, 2 * p->c_elems * nroot) != hipSuccess)
+      return fail(BSG_ERR_NOMEM, "bsg_plan_set_batch: wavefunction slices");
+    (void)hipFree(p->d_cs);
+    p->d_cs = nc;
+  }
+  p->batch = nroot;
   return BSG_OK;
 }
 
@@ -1793,13 +1891,16 @@ int bsg_plan_phase_flops(const bsg_plan* p, double* f1, double* f2) {
 }  // extern "C"
 
 namespace {
-// Launch one chunk: the grouped kernels on the context stream with the
-// side-stream kernels beside them, then the ordered reduction.  ev
-// (optional) receives 5 events: before the fused kernels, around the side
-// stream's kernels, before and after the reduction.
+// Launch one chunk for rs.n roots: the grouped kernels on the context
+// stream with the side-stream kernels beside them, then the ordered
+// reduction.  ev (optional) receives 5 events: before the fused kernels,
+// around the side stream's kernels, before and after the reduction.
+// launches (optional) counts the kernel dispatches.
 int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, const double* arena, const double* c,
-                  double* v, hipEvent_t* ev, bool split) {
+                  double* v, hipEvent_t* ev, bool split, double* part, const RootSet& rs, long* launches = nullptr) {
   hipStream_t st = ctx->stream;
+  auto grid = [&](int n) { return dim3((unsigned)(((n + 7) / 8) * 8 * rs.n)); };
+  long nl = 0;
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   // the side stream runs the small-block kernel and the basic tiles beside
   // the wide tiles on the context stream; they write disjoint V blocks and
@@ -1812,47 +1913,62 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
     if (ev) HIPCHK(hipEventRecord(ev[1], ctx->side));
     if (ch.n_s[0] > 0) {
       if (split)
-        hipLaunchKernelGGL((k_small<2, true>), dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
-                           ch.d_s[0], ch.n_s[0], arena, c, v, p->part);
+        hipLaunchKernelGGL((k_small<2, true>), grid(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
+                           ch.d_s[0], ch.n_s[0], arena, c, v, part, rs);
       else
-        hipLaunchKernelGGL((k_small<2, false>), dim3(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
-                           ch.d_s[0], ch.n_s[0], arena, c, v, p->part);
+        hipLaunchKernelGGL((k_small<2, false>), grid(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
+                           ch.d_s[0], ch.n_s[0], arena, c, v, part, rs);
+      ++nl;
     }
     if (nbasic > 0) {
       if (split)
-        hipLaunchKernelGGL((k_sigma_bf<2, 2>), dim3(nbasic), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_ps,
-                           ch.d_seg, ch.d_gbf[0], nbasic, p->d_abs, p->d_cs, c, v, p->part);
+        hipLaunchKernelGGL((k_sigma_bf<2, 2>), grid(nbasic), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_ps,
+                           ch.d_seg, ch.d_gbf[0], nbasic, p->d_abs, p->d_cs, c, v, part, rs);
       else
-        hipLaunchKernelGGL((k_sigma<2, 2>), dim3(nbasic), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
-                           ch.d_g[0], nbasic, arena, c, v, p->part);
+        hipLaunchKernelGGL((k_sigma<2, 2>), grid(nbasic), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
+                           ch.d_g[0], nbasic, arena, c, v, part, rs);
+      ++nl;
     }
     if (ev) HIPCHK(hipEventRecord(ev[2], ctx->side));
     HIPCHK(hipEventRecord(ctx->join, ctx->side));
   }
   if (split) {
-    if (ch.n_gbf[2] > 0)
-      hipLaunchKernelGGL((k_sigma_bf<1, 4, 8>), dim3(ch.n_gbf[2]), dim3(512), 0, st, ch.d_vb, ch.d_pr, ch.d_ps,
-                         ch.d_seg, ch.d_gbf[2], ch.n_gbf[2], p->d_abs, p->d_cs, c, v, p->part);
-    if (ch.n_gbf[1] > 0)
-      hipLaunchKernelGGL((k_sigma_bf<1, 4>), dim3(ch.n_gbf[1]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_ps, ch.d_seg,
-                         ch.d_gbf[1], ch.n_gbf[1], p->d_abs, p->d_cs, c, v, p->part);
+    if (ch.n_gbf[2] > 0) {
+      hipLaunchKernelGGL((k_sigma_bf<1, 4, 8>), grid(ch.n_gbf[2]), dim3(512), 0, st, ch.d_vb, ch.d_pr, ch.d_ps,
+                         ch.d_seg, ch.d_gbf[2], ch.n_gbf[2], p->d_abs, p->d_cs, c, v, part, rs);
+      ++nl;
+    }
+    if (ch.n_gbf[1] > 0) {
+      hipLaunchKernelGGL((k_sigma_bf<1, 4>), grid(ch.n_gbf[1]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_ps, ch.d_seg,
+                         ch.d_gbf[1], ch.n_gbf[1], p->d_abs, p->d_cs, c, v, part, rs);
+      ++nl;
+    }
   } else {
-    if (ch.n_g[3] > 0)
-      hipLaunchKernelGGL((k_sigma<1, 6, PB, 8>), dim3(ch.n_g[3]), dim3(512), 0, st, ch.d_vb, ch.d_pr, ch.d_seg,
-                         ch.d_g[3], ch.n_g[3], arena, c, v, p->part);
-    if (ch.n_g[2] > 0)
-      hipLaunchKernelGGL((k_sigma<1, 6>), dim3(ch.n_g[2]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g[2],
-                         ch.n_g[2], arena, c, v, p->part);
-    if (ch.n_g[1] > 0)
-      hipLaunchKernelGGL((k_sigma<1, 4>), dim3(ch.n_g[1]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g[1],
-                         ch.n_g[1], arena, c, v, p->part);
+    if (ch.n_g[3] > 0) {
+      hipLaunchKernelGGL((k_sigma<1, 6, PB, 8>), grid(ch.n_g[3]), dim3(512), 0, st, ch.d_vb, ch.d_pr, ch.d_seg,
+                         ch.d_g[3], ch.n_g[3], arena, c, v, part, rs);
+      ++nl;
+    }
+    if (ch.n_g[2] > 0) {
+      hipLaunchKernelGGL((k_sigma<1, 6>), grid(ch.n_g[2]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g[2],
+                         ch.n_g[2], arena, c, v, part, rs);
+      ++nl;
+    }
+    if (ch.n_g[1] > 0) {
+      hipLaunchKernelGGL((k_sigma<1, 4>), grid(ch.n_g[1]), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_g[1],
+                         ch.n_g[1], arena, c, v, part, rs);
+      ++nl;
+    }
   }
   if (side) HIPCHK(hipStreamWaitEvent(st, ctx->join, 0));
   if (ev) HIPCHK(hipEventRecord(ev[3], st));
-  if (ch.n_red[split] > 0)
-    hipLaunchKernelGGL(k_reduce, dim3(ch.n_red[split]), dim3(256), 0, st, ch.d_red[split], p->part, v);
+  if (ch.n_red[split] > 0) {
+    hipLaunchKernelGGL(k_reduce, dim3(ch.n_red[split], rs.n), dim3(256), 0, st, ch.d_red[split], part, v, rs);
+    ++nl;
+  }
   if (ev) HIPCHK(hipEventRecord(ev[4], st));
   HIPCHK(hipGetLastError());
+  if (launches) *launches += nl;
   return BSG_OK;
 }
 
@@ -1863,6 +1979,38 @@ int check_precision(const bsg_plan* p, const double* arena, int32_t precision) {
     return fail(BSG_ERR_ARG, "split precision needs bsg_plan_split_arena on this arena first");
   return BSG_OK;
 }
+// v += H c for nvec vectors, at most `group` roots per launch, with the
+// partial tiles of split V blocks in part (roots part_doubles apart; written
+// before they are read within each launch, so no clearing).  launches
+// (optional) counts the kernel dispatches; ev (optional) receives 5 events
+// per chunk launch.
+int sigma_apply_with(bsg_context* ctx, bsg_plan* p, const double* arena, const double* c, double* v, int32_t nvec,
+                     int32_t precision, double* part, int group, long* launches = nullptr, hipEvent_t* ev = nullptr) {
+  int rc = check_precision(p, arena, precision);
+  if (rc) return rc;
+  const bool split = precision == BSG_SPLIT_BF16;
+  group = std::max(1, std::min(group, p->batch));
+  for (int iv = 0; iv < nvec; iv += group) {
+    RootSet rs;
+    rs.n = std::min(group, nvec - iv);
+    rs.pad = 0;
+    rs.wstride = p->wave_size;
+    rs.pstride = p->part_doubles;
+    rs.cstride = p->c_elems;
+    const double* cv = c + (size_t)iv * p->wave_size;
+    double* vv = v + (size_t)iv * p->wave_size;
+    if (split && !p->c_items.empty()) {
+      hipLaunchKernelGGL(k_slices, dim3(p->c_items.size(), rs.n), dim3(256), 0, ctx->stream, p->d_c_items, cv,
+                         p->d_cs, rs.wstride, rs.cstride);
+      if (launches) ++*launches;
+    }
+    for (auto& ch : p->chunks) {
+      if ((rc = enqueue_chunk(ctx, p, ch, arena, cv, vv, ev, split, part, rs, launches))) return rc;
+      if (ev) ev += 5;
+    }
+  }
+  return BSG_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -1871,41 +2019,28 @@ int bsg_sigma_apply(bsg_context* ctx, bsg_plan* p, const double* arena, const do
                     int32_t precision) {
   if (!ctx || !p || !arena || !c || !v || nvec <= 0) return fail(BSG_ERR_ARG, "bsg_sigma_apply: bad args");
   if (p->ctx != ctx) return fail(BSG_ERR_ARG, "bsg_sigma_apply: plan belongs to another context");
-  int rc = check_precision(p, arena, precision);
-  if (rc) return rc;
-  const bool split = precision == BSG_SPLIT_BF16;
-  for (int iv = 0; iv < nvec; ++iv) {
-    const double* cv = c + (size_t)iv * p->wave_size;
-    double* vv = v + (size_t)iv * p->wave_size;
-    if (split && !p->c_items.empty())
-      hipLaunchKernelGGL(k_slices, dim3(p->c_items.size()), dim3(256), 0, ctx->stream, p->d_c_items, cv, p->d_cs);
-    for (auto& ch : p->chunks)
-      if ((rc = enqueue_chunk(ctx, p, ch, arena, cv, vv, nullptr, split))) return rc;
-  }
-  return BSG_OK;
+  return sigma_apply_with(ctx, p, arena, c, v, nvec, precision, p->part, p->batch);
 }
 
-int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const double* c, double* v, int32_t reps,
-                      int32_t precision, double* wall_ms, double* kernel_ms, long* launches) {
-  if (!ctx || !p || !arena || !c || !v || reps <= 0 || !wall_ms || !kernel_ms || !launches)
+int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const double* c, double* v, int32_t nvec,
+                      int32_t reps, int32_t precision, double* wall_ms, double* kernel_ms, long* launches) {
+  if (!ctx || !p || !arena || !c || !v || nvec <= 0 || reps <= 0 || !wall_ms || !kernel_ms || !launches)
     return fail(BSG_ERR_ARG, "bsg_sigma_profile: bad args");
   int rc = check_precision(p, arena, precision);
   if (rc) return rc;
   const bool split = precision == BSG_SPLIT_BF16;
   const size_t nch = p->chunks.size();
+  const int ngroups = (nvec + p->batch - 1) / p->batch;
   constexpr size_t per = 5;  // events per chunk launch (enqueue_chunk)
-  std::vector<hipEvent_t> ev((size_t)reps * nch * per + 2);
+  std::vector<hipEvent_t> ev((size_t)reps * ngroups * nch * per + 2);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  long disp = 0;
   HIPCHK(hipEventRecord(ev[0], ctx->stream));
-  size_t q = 1;
-  for (int r = 0; r < reps; ++r) {
-    if (split && !p->c_items.empty())
-      hipLaunchKernelGGL(k_slices, dim3(p->c_items.size()), dim3(256), 0, ctx->stream, p->d_c_items, c, p->d_cs);
-    for (auto& ch : p->chunks) {
-      if ((rc = enqueue_chunk(ctx, p, ch, arena, c, v, &ev[q], split))) return rc;
-      q += per;
-    }
-  }
+  for (int r = 0; r < reps; ++r)
+    if ((rc = sigma_apply_with(ctx, p, arena, c, v, nvec, precision, p->part, p->batch, &disp,
+                               &ev[1 + (size_t)r * ngroups * nch * per])))
+      return rc;
+  const size_t q = 1 + (size_t)reps * ngroups * nch * per;
   HIPCHK(hipEventRecord(ev[q], ctx->stream));
   HIPCHK(hipEventSynchronize(ev[q]));
   float ms = 0.f;
@@ -1916,7 +2051,7 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
     launches[i] = 0;
   }
   size_t k = 1;
-  for (int r = 0; r < reps; ++r)
+  for (int r = 0; r < reps * ngroups; ++r)
     for (auto& ch : p->chunks) {
       const bool sidek = ch.n_s[0] > 0 || (split ? ch.n_gbf[0] : ch.n_g[0]) > 0;
       // [0] fork to join of all fused kernels, [1] reduction, [2] the side
@@ -1927,10 +2062,11 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
         if (!present[i]) continue;
         HIPCHK(hipEventElapsedTime(&ms, ev[k + from[i]], ev[k + to[i]]));
         kernel_ms[i] += ms;
-        launches[i] += 1;
       }
       k += per;
     }
+  // kernel dispatches of all applications (every kernel launch counted)
+  launches[0] = disp;
   for (auto& e : ev) HIPCHK(hipEventDestroy(e));
   return BSG_OK;
 }
@@ -2085,6 +2221,43 @@ int bsg_copy_segments(bsg_context* ctx, const bsg_copy_seg_t* segs, int32_t n_se
   hipLaunchKernelGGL(k_copy_chunks, dim3((unsigned)chunks.size()), dim3(256), 0, ctx->stream, ctx->copy_list);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BSG_OK;
+}
+int bsg_build_apply(bsg_context* ctx, const bsg_build_group_t* groups, int32_t ngroups, const bsg_build_src_t* srcs,
+                    int32_t nsrcs) {
+  if (!ctx || ngroups < 0 || nsrcs < 0 || (ngroups > 0 && (!groups || !srcs))) return fail(BSG_ERR_ARG, "bsg_build_apply: bad args");
+  if (ngroups == 0) return BSG_OK;
+  std::vector<int64_t> estart(ngroups);
+  int64_t nelem = 0;
+  for (int32_t g = 0; g < ngroups; ++g) {
+    const bsg_build_group_t& gr = groups[g];
+    if (!gr.dst || gr.rows <= 0 || gr.cols <= 0 || gr.ld < gr.cols || gr.first < 0 || gr.count < 0 ||
+        gr.first + gr.count > nsrcs)
+      return fail(BSG_ERR_ARG, "bsg_build_apply: bad group");
+    estart[g] = nelem;
+    nelem += (int64_t)gr.rows * gr.cols;
+  }
+  for (int32_t j = 0; j < nsrcs; ++j)
+    if (!srcs[j].src || srcs[j].ld <= 0) return fail(BSG_ERR_ARG, "bsg_build_apply: bad source");
+  if (nelem == 0) return BSG_OK;
+  if ((nelem + 255) / 256 > 0x7fffffffLL) return fail(BSG_ERR_ARG, "bsg_build_apply: too many elements");
+  HIPCHK(hipSetDevice(ctx->device));
+  bsg_build_group_t* dg = nullptr;
+  bsg_build_src_t* ds = nullptr;
+  int64_t* de = nullptr;
+  HIPCHK(hipMalloc(&dg, sizeof(bsg_build_group_t) * ngroups));
+  HIPCHK(hipMalloc(&ds, sizeof(bsg_build_src_t) * std::max(1, nsrcs)));
+  HIPCHK(hipMalloc(&de, sizeof(int64_t) * ngroups));
+  HIPCHK(hipMemcpyAsync(dg, groups, sizeof(bsg_build_group_t) * ngroups, hipMemcpyHostToDevice, ctx->stream));
+  if (nsrcs > 0) HIPCHK(hipMemcpyAsync(ds, srcs, sizeof(bsg_build_src_t) * nsrcs, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(de, estart.data(), sizeof(int64_t) * ngroups, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_build, dim3((unsigned)((nelem + 255) / 256)), dim3(256), 0, ctx->stream, dg, de, ngroups, ds,
+                     nelem);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipFree(dg));
+  HIPCHK(hipFree(ds));
+  HIPCHK(hipFree(de));
   return BSG_OK;
 }
 int bsg_scal(bsg_context* ctx, double a, double* x, int64_t n) {

@@ -104,20 +104,71 @@ struct OpPattern {
 using PatPtr = std::shared_ptr<const OpPattern>;
 PatPtr get_pattern(const SIPtr& si, int dn, int k2, int dirrep);
 
-struct SparseOp {
+// An operator's elements live in host memory (`data`), in device memory
+// (`dev`, kept alive by `dev_owner`), or both.  Operators built or rotated on
+// the GPU are device-only (host_valid false, `data` empty) until something on
+// the host asks for them (materialise_host).
+struct SparseOp : std::enable_shared_from_this<SparseOp> {
   PatPtr pat;
   bool fermion = false;
   std::vector<double> data;
+  double* dev = nullptr;
+  std::shared_ptr<void> dev_owner;
+  bool host_valid = true;
   SparseOp() = default;
-  SparseOp(PatPtr p, bool f) : pat(std::move(p)), fermion(f), data(pat->size, 0.0) {}
+  SparseOp(PatPtr p, bool f);
+  struct DeviceOnly {};
+  SparseOp(PatPtr p, bool f, DeviceOnly) : pat(std::move(p)), fermion(f), host_valid(false) {}
   double* blk(int id) { return data.data() + pat->blocks[id].off; }
   const double* blk(int id) const { return data.data() + pat->blocks[id].off; }
   int dn() const { return pat->dn; }
   int k2() const { return pat->k2; }
   int dirrep() const { return pat->dirrep; }
+  long size() const { return pat->size; }
   void axpy(double a, const SparseOp& x);
 };
 using OpPtr = std::shared_ptr<SparseOp>;
+
+// Download of device-only operators (set by the GPU layer); materialise_host
+// makes `data` valid for host readers (a no-op for host operators).
+using HostMaterialiser = void (*)(const std::vector<SparseOp*>& ops);
+void set_host_materialiser(HostMaterialiser f);
+void materialise_host(const std::vector<SparseOp*>& ops);
+
+// ---------------------------------------------------------------- build recording
+// While a BuildSession is open, tensor_product and SparseOp::axpy on the
+// session's threads do not compute: they append the scaled block copies they
+// stand for, and operators created meanwhile get no host storage.  The GPU
+// layer then executes the items (composite operators built in HBM from
+// device-resident block operators).  Item: dst[r dst_ld + c] += (alpha *
+// (trans ? src[c src_ld + r] : src[r src_ld + c])) * beta for r < rows,
+// c < cols (the host build's product order), offsets in doubles from the
+// operators' first element; level 0 items (linear combinations of whole
+// operators) run before level 1 items.
+struct BuildItem {
+  const SparseOp* dst;
+  const SparseOp* src;
+  int64_t dst_off, src_off;
+  int32_t dst_ld, src_ld, rows, cols, trans, level;
+  double alpha, beta;
+};
+struct BuildSession {
+  BuildSession();
+  ~BuildSession();
+  BuildSession(const BuildSession&) = delete;
+  BuildSession& operator=(const BuildSession&) = delete;
+  // all threads' items (call after the recorded work returned)
+  std::vector<BuildItem> items() const;
+  // every operator an item writes, kept alive until the session ends
+  std::vector<OpPtr> written() const;
+  struct Impl;
+  std::unique_ptr<Impl> impl;
+};
+bool build_recording();
+// The calling thread's open session (null if none), and entering one on a
+// worker thread (OpenMP workers of recorded work inherit their parent's).
+BuildSession::Impl* current_build_session();
+void enter_build_session(BuildSession::Impl* s);
 
 // Read-only view: op itself (dag=false) or its adjoint tensor (dag=true),
 // times coef.  The adjoint's block (r,c) is dagger_phase(S_r,S_c,k) *

@@ -7,6 +7,7 @@
 #include <random>
 #include <string>
 
+#include "checkpoint.h"
 #include "gpu_sigma.h"
 #include "sweep.h"
 
@@ -134,6 +135,33 @@ void* blkh_snapshot_dmrg(void* ints, int nelec, int twoS, int irrep, int nroots,
     return new SuperBlock(std::move(sb));
   });
 }
+// The two-site superblock of forward step l (sys = sites 0..l, dots l+1 and
+// l+2, env = sites l+3..) from the blocks of a saved sweep state
+// (checkpoint file of blkh_dmrg_gpu): the workload a sweep at that state's
+// bond dimension solves at that step.  l < 0: the middle of the chain.
+void* blkh_superblock_from_state(void* ints, const char* path, int nelec, int twoS, int irrep, int l) {
+  return guarded_ptr([&]() -> void* {
+    const Integrals& I = *static_cast<Integrals*>(ints);
+    const int k = I.norb;
+    if (l < 0) l = k / 2 - 2;
+    if (l < 0 || l + 3 >= k) throw std::runtime_error("blkh_superblock_from_state: step out of range");
+    SweepState st = load_sweep_state(path);
+    if ((int)st.fwd.size() != k || !st.fwd[l] || !st.bwd[l + 3])
+      throw std::runtime_error("blkh_superblock_from_state: the state has no blocks for this step");
+    Screen scr;
+    const Quantum target{nelec, twoS, irrep};
+    BuildSpec sl;
+    sl.normal = true;
+    sl.comp = false;
+    BuildSpec sr;
+    sr.normal = false;
+    sr.comp = true;
+    const int nL = (int)st.fwd[l]->sites.size() + 1, nR = (int)st.bwd[l + 3]->sites.size() + 1;
+    BlockPtr L = build_sum_block(I, *st.fwd[l], *make_dot(I, l + 1, scr), target, nR, sl, scr);
+    BlockPtr R = build_sum_block(I, *st.bwd[l + 3], *make_dot(I, l + 2, scr), target, nL, sr, scr);
+    return new SuperBlock(make_superblock(I, L, R, target));
+  });
+}
 void blkh_free_superblock(void* h) { delete static_cast<SuperBlock*>(h); }
 long blkh_wave_size(void* h) { return static_cast<SuperBlock*>(h)->wave->size; }
 double blkh_flops(void* h) { return sigma_flops(*static_cast<SuperBlock*>(h)); }
@@ -168,12 +196,13 @@ int blkh_diag_cpu(void* h, double* out) {
   });
 }
 
-void* blkh_gpu_sigma_create(void* sb, int device, long reserved) {
+// max_vec: vectors the handle's device buffers hold, and the plan's batch
+void* blkh_gpu_sigma_create(void* sb, int device, long max_vec) {
   return guarded_ptr([&]() -> void* {
     auto* h = new GpuSigmaHandle();
     try {
       h->ctx.reset(new GpuContext(device));
-      h->sig.reset(new GpuSigma(*h->ctx, *static_cast<SuperBlock*>(sb)));
+      h->sig.reset(new GpuSigma(*h->ctx, *static_cast<SuperBlock*>(sb), (int)std::max(1L, max_vec)));
     } catch (...) {
       delete h;
       throw;
@@ -205,25 +234,53 @@ int blkh_gpu_sigma_set_precision(void* h, int mode) {
     return 0;
   });
 }
+// the C-ABI handles behind a GPU sigma handle (context, plan, operator arena,
+// device c and v buffers of wave_size doubles), for callers that drive
+// libblock_sigma.so directly
+int blkh_gpu_sigma_raw(void* h, void** ctx, void** plan, const double** arena, double** dev_c, double** dev_v) {
+  auto* g = static_cast<GpuSigmaHandle*>(h);
+  *ctx = g->sig->ctx();
+  *plan = g->sig->plan();
+  *arena = g->sig->dev_arena();
+  *dev_c = g->sig->dev_c();
+  *dev_v = g->sig->dev_v();
+  return 0;
+}
 long blkh_gpu_sigma_pairs(void* h) { return (long)static_cast<GpuSigmaHandle*>(h)->sig->n_pairs(); }
 long blkh_gpu_sigma_arena_bytes(void* h) { return (long)static_cast<GpuSigmaHandle*>(h)->sig->arena_bytes(); }
 // device-resident loop for timing: c is uploaded once, v accumulates `reps` sigmas.
 // kernel_ms[0..2] receive the summed fused-kernel / reduction / wavefunction
 // split times measured with HIP events on the plan's stream, wall_ms the
 // whole loop; the handle's current precision is used.
-int blkh_gpu_sigma_timed(void* h, const double* c, int warmup, int reps, double* wall_ms, double* kernel_ms,
+// c: nvec vectors (nvec <= the handle's max_vec), batched by the plan's
+// batch size (blkh_gpu_sigma_set_batch).
+int blkh_gpu_sigma_timed(void* h, const double* c, int nvec, int warmup, int reps, double* wall_ms, double* kernel_ms,
                          long* launches) {
   return guarded([&]() {
     auto* g = static_cast<GpuSigmaHandle*>(h);
     GpuSigma& s = *g->sig;
-    size_t bytes = sizeof(double) * s.wave_size();
+    if (nvec < 1 || nvec > s.max_vec()) throw std::runtime_error("blkh_gpu_sigma_timed: nvec beyond the handle's max_vec");
+    size_t bytes = sizeof(double) * s.wave_size() * nvec;
     bsg_check(bsg_memcpy_h2d(s.ctx(), s.dev_c(), c, bytes), "c upload");
     bsg_check(bsg_memset_zero(s.ctx(), s.dev_v(), bytes), "v clear");
-    for (int i = 0; i < warmup; ++i) s.apply_device(s.dev_c(), s.dev_v());
+    for (int i = 0; i < warmup; ++i) s.apply_device(s.dev_c(), s.dev_v(), nvec);
     bsg_check(bsg_synchronize(s.ctx()), "sync");
-    bsg_check(bsg_sigma_profile(s.ctx(), s.plan(), s.dev_arena(), s.dev_c(), s.dev_v(), reps, s.precision(), wall_ms,
-                                kernel_ms, launches),
+    bsg_check(bsg_sigma_profile(s.ctx(), s.plan(), s.dev_arena(), s.dev_c(), s.dev_v(), nvec, reps, s.precision(),
+                                wall_ms, kernel_ms, launches),
               "profile");
+    return 0;
+  });
+}
+int blkh_gpu_sigma_set_batch(void* h, int nroot) {
+  return guarded([&]() {
+    static_cast<GpuSigmaHandle*>(h)->sig->set_batch(nroot);
+    return 0;
+  });
+}
+// v = H c for nvec host vectors (nvec <= max_vec) in the plan's batches
+int blkh_gpu_sigma_apply_n(void* h, const double* c, double* v, int nvec) {
+  return guarded([&]() {
+    static_cast<GpuSigmaHandle*>(h)->sig->apply(c, v, nvec);
     return 0;
   });
 }
@@ -271,11 +328,13 @@ int blkh_davidson(void* sbp, int device, int mode, int nroots, const double* gue
 // (start_sweep, M, davidson_tol, noise).  ladder > 0: split-precision sigma
 // until the Davidson residual norm falls below it, FP64 after.  ahead: host
 // operator builds beside the Davidson solve (1 on, 0 off, -1 default: on).
+// device_blocks: composite operators built in HBM and renormalised blocks
+// kept there (no operator crosses PCIe; host builds beside the solve off).
 // energies_out[nroots]; sweep_out[max_sweeps * kSweepCols]: one row per sweep
 // (columns as write_sweep_records).  Returns the number of rows written.
 int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
                   const double* sched, int max_sweeps, double sweep_tol, double ladder, int ladder_from, int guess,
-                  const char* checkpoint, const char* resume, int ahead, int device, int verbose,
+                  const char* checkpoint, const char* resume, int ahead, int device_blocks, int device, int verbose,
                   double* energies_out, double* sweep_out) {
   return guarded([&]() {
     const Integrals& I = *static_cast<Integrals*>(ints);
@@ -297,13 +356,103 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
     if (resume) P.resume = resume;
     P.host_ahead = ahead;
     GpuContext ctx(device);
-    P.rotator = std::make_shared<GpuRotator>(ctx);
-    P.preload_block = [&ctx](const SpinBlock& b) { preload_ops(ctx, 1, block_ops(b)); };
+    // device-only operators are downloaded through ctx while it lives
+    struct MatGuard {
+      explicit MatGuard(GpuContext* g) { set_materialiser_context(g); }
+      ~MatGuard() { set_materialiser_context(nullptr); }
+    } mat(&ctx);
+    P.rotator = std::make_shared<GpuRotator>(ctx, device_blocks != 0);
+    if (device_blocks) {
+      P.builder = std::make_shared<GpuBlockBuilder>(ctx);
+    } else {
+      P.preload_block = [&ctx](const SpinBlock& b) { preload_ops(ctx, 1, block_ops(b)); };
+    }
     auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {
       return std::unique_ptr<DavidsonFunctor>(new GpuFunctor(ctx, sb));
     });
     for (int r = 0; r < nroots; ++r) energies_out[r] = res.energies[r];
     return write_sweep_records(res, sweep_out, max_sweeps);
+  });
+}
+
+// Device builds beside host builds: a Wilson-grown block sys (nsys sites) and
+// the dot nsys; sys (x) dot with normal operators, with complementary
+// operators, and sys (x) dot's complementary operators in its own basis, each
+// built on the host and in HBM; then the same from a device-only sys (its
+// operators rotated on the GPU and left there, so the build reads HBM only).
+// maxdiff: largest |host - device| element over all operators, nops: the
+// number compared.
+int blkh_build_check(void* ints, int nsys, int M, int device, double* maxdiff, long* nops) {
+  return guarded([&]() {
+    const Integrals& I = *static_cast<Integrals*>(ints);
+    Screen scr;
+    const Quantum target{I.nelec, 0, I.isym};
+    std::vector<int> order(I.norb);
+    for (int i = 0; i < I.norb; ++i) order[i] = i;
+    BlockPtr sys = warmup_block(I, order, nsys, target, M, true, scr);
+    BlockPtr dot = make_dot(I, nsys, scr);
+    GpuContext ctx(device);
+    struct MatGuard {
+      explicit MatGuard(GpuContext* g) { set_materialiser_context(g); }
+      ~MatGuard() { set_materialiser_context(nullptr); }
+    } mat(&ctx);
+    GpuBlockBuilder gb(ctx);
+    double worst = 0.0;
+    long n = 0;
+    auto cmp_lists = [&](const std::vector<OpPtr>& a, const std::vector<OpPtr>& b) {
+      if (a.size() != b.size()) throw std::runtime_error("build check: operator sets differ");
+      std::vector<SparseOp*> dev;
+      for (const OpPtr& o : b) dev.push_back(o.get());
+      materialise_host(dev);
+      for (size_t i = 0; i < a.size(); ++i) {
+        if (a[i]->size() != b[i]->size() || a[i]->pat != b[i]->pat) throw std::runtime_error("build check: patterns differ");
+        for (long e = 0; e < a[i]->size(); ++e) worst = std::max(worst, std::fabs(a[i]->data[e] - b[i]->data[e]));
+        ++n;
+      }
+    };
+    // device builds first (B1 may be device-only), then B1 in host memory for the host builds
+    auto check_from = [&](const SpinBlock& B1) {
+      const int nrem = I.norb - (int)B1.sites.size() - 1;
+      BlockPtr d[2];
+      std::shared_ptr<CompOpsAhead> da;
+      for (int comp = 0; comp < 2; ++comp) {
+        BuildSpec sp;
+        sp.normal = comp == 0;
+        sp.comp = comp == 1;
+        d[comp] = gb.sum_block(I, B1, *dot, target, nrem, sp, scr);
+        if (comp == 0) da = gb.comp_ops(I, *d[0], B1, *dot, scr);
+      }
+      std::vector<SparseOp*> b1;
+      for (const OpPtr& o : block_ops(B1)) b1.push_back(o.get());
+      materialise_host(b1);
+      for (int comp = 0; comp < 2; ++comp) {
+        BuildSpec sp;
+        sp.normal = comp == 0;
+        sp.comp = comp == 1;
+        BlockPtr h = build_sum_block(I, B1, *dot, target, nrem, sp, scr);
+        cmp_lists(block_ops(*h), block_ops(*d[comp]));
+        if (comp == 0) cmp_lists(comp_ops_list(*build_comp_ops(I, *h, B1, *dot, scr)), comp_ops_list(*da));
+      }
+    };
+    check_from(*sys);
+    // sys with every operator device-only (an identity rotation on the GPU)
+    std::vector<std::vector<double>> rot(sys->si->nsec());
+    std::vector<int> keep(sys->si->nsec());
+    for (int q = 0; q < sys->si->nsec(); ++q) {
+      const int d = sys->si->dim[q];
+      keep[q] = d;
+      rot[q].assign((size_t)d * d, 0.0);
+      for (int i = 0; i < d; ++i) rot[q][(size_t)i * d + i] = 1.0;
+    }
+    Rotation R = make_rotation_info(*sys->si, rot, keep);
+    GpuRotator dev_rot(ctx, true);
+    BlockPtr dsys = renormalise(*sys, R, &dev_rot);
+    for (const OpPtr& o : block_ops(*dsys))
+      if (o->host_valid || !o->dev) throw std::runtime_error("build check: rotated block not device-only");
+    check_from(*dsys);
+    *maxdiff = worst;
+    *nops = n;
+    return 0;
   });
 }
 

@@ -226,7 +226,13 @@ void save_sweep_state(const std::string& path, const SweepState& s) {
       o.vec(t);
     }
   }
-  // operators: pattern key and data
+  // operators: pattern key and data (device-only ones downloaded first)
+  {
+    std::vector<SparseOp*> dev;
+    for (const SparseOp* p : w.ops)
+      if (!p->host_valid) dev.push_back(const_cast<SparseOp*>(p));
+    materialise_host(dev);
+  }
   o.pod<uint64_t>(w.ops.size());
   for (const SparseOp* p : w.ops) {
     o.pod<int32_t>(w.si_id.at(p->pat->si.get()));

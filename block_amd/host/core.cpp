@@ -2,6 +2,7 @@
 // tensor product and small dense kernels of the host library.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <tuple>
 
@@ -99,8 +100,105 @@ PatPtr get_pattern(const SIPtr& si, int dn, int k2, int dirrep) {
   return p;
 }
 
+// ---------------------------------------------------------------- device-only operators
+namespace {
+HostMaterialiser g_materialiser = nullptr;
+}
+void set_host_materialiser(HostMaterialiser f) { g_materialiser = f; }
+void materialise_host(const std::vector<SparseOp*>& ops) {
+  std::vector<SparseOp*> need;
+  for (SparseOp* o : ops)
+    if (o && !o->host_valid) need.push_back(o);
+  if (need.empty()) return;
+  if (!g_materialiser) throw std::runtime_error("device-only operators and no GPU layer to download them");
+  g_materialiser(need);
+}
+
+// ---------------------------------------------------------------- build recording
+namespace {
+struct Recorder {
+  std::vector<BuildItem> items;
+  std::vector<OpPtr> keep;
+};
+}  // namespace
+struct BuildSession::Impl {
+  std::mutex mu;
+  std::vector<std::unique_ptr<Recorder>> recs;
+};
+namespace {
+// the session the calling thread records into, and its recorder there
+thread_local BuildSession::Impl* tl_session = nullptr;
+thread_local BuildSession::Impl* tl_rec_session = nullptr;
+thread_local Recorder* tl_rec = nullptr;
+Recorder* recorder() {
+  BuildSession::Impl* s = tl_session;
+  if (!s) return nullptr;
+  if (tl_rec_session != s) {
+    auto r = std::make_unique<Recorder>();
+    tl_rec = r.get();
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->recs.push_back(std::move(r));
+    tl_rec_session = s;
+  }
+  return tl_rec;
+}
+// an OpenMP worker records into its parent's session while in scope
+struct SessionScope {
+  BuildSession::Impl* prev;
+  explicit SessionScope(BuildSession::Impl* s) : prev(tl_session) { tl_session = s; }
+  ~SessionScope() { tl_session = prev; }
+};
+void keep_written(Recorder* r, const SparseOp& o) {
+  // operators are owned by shared pointers; keep the written ones alive so
+  // no later allocation can reuse an address an item refers to
+  r->keep.push_back(std::const_pointer_cast<SparseOp>(o.shared_from_this()));
+}
+}  // namespace
+BuildSession::BuildSession() : impl(new Impl) {
+  if (tl_session) throw std::runtime_error("BuildSession: a session is already open on this thread");
+  tl_session = impl.get();
+}
+BuildSession::~BuildSession() {
+  if (tl_session == impl.get()) tl_session = nullptr;
+  if (tl_rec_session == impl.get()) {
+    tl_rec_session = nullptr;
+    tl_rec = nullptr;
+  }
+}
+std::vector<BuildItem> BuildSession::items() const {
+  std::vector<BuildItem> out;
+  std::lock_guard<std::mutex> lk(impl->mu);
+  for (const auto& r : impl->recs) out.insert(out.end(), r->items.begin(), r->items.end());
+  return out;
+}
+std::vector<OpPtr> BuildSession::written() const {
+  std::vector<OpPtr> out;
+  std::lock_guard<std::mutex> lk(impl->mu);
+  for (const auto& r : impl->recs) out.insert(out.end(), r->keep.begin(), r->keep.end());
+  return out;
+}
+bool build_recording() { return tl_session != nullptr; }
+BuildSession::Impl* current_build_session() { return tl_session; }
+void enter_build_session(BuildSession::Impl* s) { tl_session = s; }
+
+SparseOp::SparseOp(PatPtr p, bool f) : pat(std::move(p)), fermion(f) {
+  if (tl_session)
+    host_valid = false;  // built in HBM from the recorded items
+  else
+    data.assign(pat->size, 0.0);
+}
+
 void SparseOp::axpy(double a, const SparseOp& x) {
   if (x.pat.get() != pat.get()) throw std::runtime_error("SparseOp::axpy: pattern mismatch");
+  if (Recorder* r = recorder()) {
+    if (a != 0.0 && pat->size > 0) {
+      if (pat->size > 0x7fffffffL) throw std::runtime_error("SparseOp::axpy: operator too large to record");
+      r->items.push_back({this, &x, 0, 0, (int32_t)pat->size, (int32_t)pat->size, 1, (int32_t)pat->size, 0, 0, a, 1.0});
+      keep_written(r, *this);
+    }
+    return;
+  }
+  if (!host_valid || !x.host_valid) throw std::runtime_error("SparseOp::axpy: operator not in host memory");
   const double* xs = x.data.data();
   double* ys = data.data();
   for (size_t i = 0; i < data.size(); ++i) ys[i] += a * xs[i];
@@ -123,6 +221,52 @@ void tensor_product(const OpView& a, const OpView& b, SparseOp& c, double scale)
   if (a.dn() + b.dn() != c.dn() || irrep_mul(a.dirrep(), b.dirrep()) != c.dirrep() || !triangle(ka, kb, K))
     throw std::runtime_error("tensor_product: delta quanta do not combine");
   const auto& blocks = c.pat->blocks;
+  if (recorder()) {
+    // composite (block (x) dot) operators: every sub-block term is a scaled
+    // copy of one block of a, the dot factor a 1 x 1 block of b
+    if (!b.op->host_valid) throw std::runtime_error("tensor_product: dot operator not in host memory");
+    Recorder* r = recorder();
+    bool any = false;
+    for (int ib = 0; ib < (int)blocks.size(); ++ib) {
+      const auto& B = blocks[ib];
+      const Quantum &Q = cs.q[B.r], &Qp = cs.q[B.c];
+      for (const auto& s : cs.subs[B.r])
+        for (const auto& sp : cs.subs[B.c]) {
+          int ia = a.find(s.l, sp.l);
+          if (ia < 0) continue;
+          int ibb = b.find(s.r, sp.r);
+          if (ibb < 0) continue;
+          const Quantum &ql = ls.q[s.l], &qlp = ls.q[sp.l], &qr = rs.q[s.r], &qrp = rs.q[sp.r];
+          double f = scale * wninej(qlp.s2, qrp.s2, Qp.s2, ka, kb, K, ql.s2, qr.s2, Q.s2);
+          if (f == 0.0) continue;
+          if (b.fermion() && (qlp.n & 1)) f = -f;
+          f *= a.scale(s.l, sp.l) * b.scale(s.r, sp.r);
+          const int dl = ls.dim[s.l], dlp = ls.dim[sp.l], dr = rs.dim[s.r], drp = rs.dim[sp.r];
+          if (dr != 1 || drp != 1) throw std::runtime_error("tensor_product: recording needs a single-site second factor");
+          const double bval = b.op->blk(ibb)[0];
+          if (bval == 0.0 || dl == 0 || dlp == 0) continue;
+          BuildItem it;
+          it.dst = &c;
+          it.src = a.op;
+          it.dst_off = B.off + (int64_t)s.off * B.cols + sp.off;
+          it.dst_ld = B.cols;
+          it.src_off = a.op->pat->blocks[ia].off;
+          it.src_ld = a.dag ? dl : dlp;  // the stored block is dlp x dl when a is the adjoint view
+          it.rows = dl;
+          it.cols = dlp;
+          it.trans = a.dag ? 1 : 0;
+          it.level = 1;
+          it.alpha = f;
+          it.beta = bval;
+          r->items.push_back(it);
+          any = true;
+        }
+    }
+    if (any) keep_written(r, c);
+    return;
+  }
+  if (!a.op->host_valid || !b.op->host_valid || !c.host_valid)
+    throw std::runtime_error("tensor_product: operator not in host memory");
 #pragma omp parallel for schedule(dynamic)
   for (int ib = 0; ib < (int)blocks.size(); ++ib) {
     const auto& B = blocks[ib];

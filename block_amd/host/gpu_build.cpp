@@ -1,0 +1,198 @@
+// Composite operators built in HBM: the host records the scaled block copies
+// of build_sum_block / build_comp_ops (the TensorProduct recursions of
+// Operators.C:624-691,1059-1196,1472-1616,1905-1966,2322-2395 for a block
+// (x) dot, operatorfunctions.C:146-254 with MatrixBLAS.C:138-162's copies)
+// and bsg_build_apply executes them from the device copies of the block's
+// operators, so a sweep's blocks never cross PCIe.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+
+#include "gpu_sigma.h"
+
+namespace blk {
+
+namespace {
+double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+BuildTimes g_build;
+std::mutex g_mat_mu;
+GpuContext* g_mat_ctx = nullptr;
+
+// downloads device-only operators into their host storage
+void download_ops(const std::vector<SparseOp*>& ops) {
+  std::lock_guard<std::mutex> lk(g_mat_mu);
+  if (!g_mat_ctx) throw std::runtime_error("materialise_host: no GPU context to download operators");
+  for (SparseOp* o : ops) {
+    if (o->host_valid) continue;
+    if (!o->dev) throw std::runtime_error("materialise_host: operator in neither host nor device memory");
+    o->data.assign(o->size(), 0.0);
+    bsg_check(bsg_memcpy_d2h(g_mat_ctx->ctx, o->data.data(), o->dev, sizeof(double) * o->size()), "operator download");
+    o->host_valid = true;
+  }
+}
+}  // namespace
+
+const BuildTimes& gpu_build_times() { return g_build; }
+
+void set_materialiser_context(GpuContext* g) {
+  std::lock_guard<std::mutex> lk(g_mat_mu);
+  g_mat_ctx = g;
+  set_host_materialiser(g ? download_ops : nullptr);
+}
+
+std::shared_ptr<void> device_buffer(GpuContext& g, size_t n, double** ptr) {
+  double* p = nullptr;
+  bsg_check(bsg_malloc(g.ctx, sizeof(double) * std::max<size_t>(1, n), (void**)&p), "device block storage");
+  bsg_context* ctx = g.ctx;
+  *ptr = p;
+  return std::shared_ptr<void>(p, [ctx](void* q) { bsg_mfree(ctx, q); });
+}
+
+void execute_build(GpuContext& g, const BuildSession& session, const std::vector<SparseOp*>& targets) {
+  const double t0 = now_s();
+  const std::vector<BuildItem> items = session.items();
+  const std::vector<OpPtr> written = session.written();
+  // destinations: the targets in one buffer they keep, other written
+  // operators (linear-combination intermediates) in scratch
+  std::unordered_map<const SparseOp*, double*> where;
+  int64_t tsize = 0;
+  std::vector<std::pair<SparseOp*, int64_t>> tlist;
+  for (SparseOp* o : targets) {
+    if (!o || where.count(o)) continue;
+    where[o] = nullptr;
+    tlist.push_back({o, tsize});
+    tsize += o->size();
+  }
+  std::vector<std::pair<const SparseOp*, int64_t>> slist;  // scratch: intermediates, then uploaded sources
+  int64_t ssize = 0;
+  for (const OpPtr& o : written) {
+    if (where.count(o.get())) continue;
+    where[o.get()] = nullptr;
+    slist.push_back({o.get(), ssize});
+    ssize += o->size();
+  }
+  const int64_t inter = ssize;
+  std::vector<std::pair<const SparseOp*, int64_t>> up;  // host-only sources
+  for (const BuildItem& it : items) {
+    const SparseOp* s = it.src;
+    if (where.count(s) || s->dev) continue;
+    if (!s->host_valid) throw std::runtime_error("execute_build: source operator in neither host nor device memory");
+    where[s] = nullptr;
+    up.push_back({s, ssize - inter});
+    ssize += s->size();
+  }
+  double* tbuf = nullptr;
+  auto owner = device_buffer(g, (size_t)tsize, &tbuf);
+  double* sbuf = nullptr;
+  std::shared_ptr<void> sowner;
+  if (ssize > 0) sowner = device_buffer(g, (size_t)ssize, &sbuf);
+  for (auto& x : tlist) where[x.first] = tbuf + x.second;
+  for (auto& x : slist) where[x.first] = sbuf + x.second;
+  for (auto& x : up) where[x.first] = sbuf + inter + x.second;
+  bsg_check(bsg_memset_zero(g.ctx, tbuf, sizeof(double) * std::max<int64_t>(1, tsize)), "build clear");
+  if (inter > 0) bsg_check(bsg_memset_zero(g.ctx, sbuf, sizeof(double) * inter), "build clear");
+  if (!up.empty()) {
+    // small host operators (the dot's) through a plain copy each
+    std::vector<double> h;
+    for (auto& x : up) h.insert(h.end(), x.first->data.begin(), x.first->data.end());
+    bsg_check(bsg_memcpy_h2d(g.ctx, sbuf + inter, h.data(), sizeof(double) * h.size()), "build sources");
+  }
+  auto addr = [&](const SparseOp* o) -> const double* {
+    auto f = where.find(o);
+    if (f != where.end() && f->second) return f->second;
+    if (o->dev) return o->dev;
+    throw std::runtime_error("execute_build: operator without an address");
+  };
+  // groups: items with the same level and destination rectangle, in the
+  // order recorded (one recording thread per destination operator)
+  std::vector<int> order(items.size());
+  for (size_t i = 0; i < items.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const BuildItem &x = items[a], &y = items[b];
+    if (x.level != y.level) return x.level < y.level;
+    if (x.dst != y.dst) return std::less<const SparseOp*>()(x.dst, y.dst);
+    return x.dst_off < y.dst_off;
+  });
+  double bytes = 0.0;
+  for (int level = 0; level < 2; ++level) {
+    std::vector<bsg_build_group_t> groups;
+    std::vector<bsg_build_src_t> srcs;
+    for (size_t k = 0; k < order.size();) {
+      const BuildItem& h = items[order[k]];
+      if (h.level != level) {
+        ++k;
+        continue;
+      }
+      bsg_build_group_t gr;
+      gr.dst = const_cast<double*>(addr(h.dst)) + h.dst_off;
+      gr.ld = h.dst_ld;
+      gr.rows = h.rows;
+      gr.cols = h.cols;
+      gr.first = (int32_t)srcs.size();
+      gr.pad = 0;
+      size_t e = k;
+      for (; e < order.size(); ++e) {
+        const BuildItem& it = items[order[e]];
+        if (it.level != level || it.dst != h.dst || it.dst_off != h.dst_off) break;
+        if (it.rows != h.rows || it.cols != h.cols || it.dst_ld != h.dst_ld)
+          throw std::runtime_error("execute_build: overlapping destination rectangles");
+        bsg_build_src_t s;
+        s.src = addr(it.src) + it.src_off;
+        s.ld = it.src_ld;
+        s.trans = it.trans;
+        s.alpha = it.alpha;
+        s.beta = it.beta;
+        srcs.push_back(s);
+      }
+      gr.count = (int32_t)(e - k);
+      bytes += 8.0 * h.rows * h.cols * (gr.count + 2);
+      groups.push_back(gr);
+      k = e;
+    }
+    if (groups.size() > 0x7fffffff || srcs.size() > 0x7fffffff) throw std::runtime_error("execute_build: too many items");
+    bsg_check(bsg_build_apply(g.ctx, groups.data(), (int32_t)groups.size(), srcs.data(), (int32_t)srcs.size()),
+              "composite build");
+  }
+  for (auto& x : tlist) {
+    x.first->data = std::vector<double>();
+    x.first->host_valid = false;
+    x.first->dev = tbuf + x.second;
+    x.first->dev_owner = owner;
+  }
+  g_build.execute += now_s() - t0;
+  g_build.bytes += bytes;
+}
+
+BlockPtr GpuBlockBuilder::sum_block(const Integrals& I, const SpinBlock& B1, const SpinBlock& dot,
+                                    const Quantum& target, int nrem, const BuildSpec& spec, const Screen& scr) const {
+  BlockPtr b;
+  {
+    BuildSession session;
+    const double t0 = now_s();
+    b = build_sum_block(I, B1, dot, target, nrem, spec, scr);
+    g_build.record += now_s() - t0;
+    std::vector<SparseOp*> ops;
+    for (const OpPtr& o : block_ops(*b)) ops.push_back(o.get());
+    execute_build(g_, session, ops);
+  }
+  return b;
+}
+
+std::shared_ptr<CompOpsAhead> GpuBlockBuilder::comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1,
+                                                        const SpinBlock& dot, const Screen& scr) const {
+  std::shared_ptr<CompOpsAhead> a;
+  {
+    BuildSession session;
+    const double t0 = now_s();
+    a = build_comp_ops(I, B, B1, dot, scr);
+    g_build.record += now_s() - t0;
+    std::vector<SparseOp*> ops;
+    for (const OpPtr& o : comp_ops_list(*a)) ops.push_back(o.get());
+    execute_build(g_, session, ops);
+  }
+  return a;
+}
+
+}  // namespace blk

@@ -84,18 +84,21 @@ void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std
   }
   bool split = ladder > 0.0;
   h.set_precision(split ? BSG_SPLIT_BF16 : BSG_FP64);
+  // the first iteration's (and a ladder restart's) nroots sigmas in one launch set
+  h.set_batch(nroots);
   int m = nroots, nsig = 0, converged_roots = 0, iter = 0;
   std::vector<double> G, sub, eval, coef;
   while (true) {
     ++iter;
     if (st) st->iterations = iter;
     auto t0 = std::chrono::steady_clock::now();
-    for (int i = nsig; i < m; ++i) {
-      bsg_check(bsg_memset_zero(ctx, vec(Sp, i), sizeof(double) * N), "sigma clear");
-      h.apply_device(vec(Bp, i), vec(Sp, i));
+    // sigma for the new vectors only (linear.C:234-258), batched
+    if (m > nsig) {
+      bsg_check(bsg_memset_zero(ctx, vec(Sp, nsig), sizeof(double) * N * (m - nsig)), "sigma clear");
+      h.apply_device(vec(Bp, nsig), vec(Sp, nsig), m - nsig);
       if (st) {
-        st->sigma_calls++;
-        if (split) st->sigma_calls_split++;
+        st->sigma_calls += m - nsig;
+        if (split) st->sigma_calls_split += m - nsig;
       }
     }
     bsg_check(bsg_synchronize(ctx), "sigma");

@@ -50,15 +50,20 @@ namespace {
 // copied; device memory between the operators is left as it is.  n: doubles
 // spanned by the offsets.
 void upload_ops(GpuContext& g, const std::vector<std::pair<const SparseOp*, int64_t>>& ops, double* dst, size_t n) {
-  // preloaded operators: device copies, in one launch
+  // device-resident and preloaded operators: device copies, in one launch
   std::vector<std::pair<const SparseOp*, int64_t>> rest;
   std::vector<bsg_copy_seg_t> segs;
   for (const auto& kv : ops) {
+    if (kv.first->dev) {
+      segs.push_back({kv.first->dev, dst + kv.second, (int64_t)kv.first->size()});
+      continue;
+    }
+    if (!kv.first->host_valid) throw std::runtime_error("upload_ops: operator in neither host nor device memory");
     bool done = false;
     for (auto& p : g.pre) {
       auto f = p.at.find(kv.first);
       if (f == p.at.end()) continue;
-      segs.push_back({p.buf + f->second, dst + kv.second, (int64_t)kv.first->data.size()});
+      segs.push_back({p.buf + f->second, dst + kv.second, (int64_t)kv.first->size()});
       p.at.erase(f);
       done = true;
       break;
@@ -117,7 +122,7 @@ WorkList compile_worklist(const SuperBlock& sb) {
       const SparseOp* o = side ? t.b.op : t.a.op;
       if (!wl.arena_off.count(o)) {
         wl.arena_off[o] = wl.arena_size;
-        wl.arena_size += (int64_t)o->data.size();
+        wl.arena_size += (int64_t)o->size();
       }
     }
 
@@ -232,7 +237,8 @@ GpuSigma::GpuSigma(GpuContext& g, const SuperBlock& sb, int max_vec) : g_(g) {
   upload_ops(g_, ops, d_arena_, (size_t)arena_size_);
   if (!owns_arena_) g_.resident = wl.arena_off;
   double t2 = secs();
-  size_t wbytes = sizeof(double) * std::max<long>(1, wave_size_) * std::max(1, max_vec);
+  max_vec_ = std::max(1, max_vec);
+  size_t wbytes = sizeof(double) * std::max<long>(1, wave_size_) * max_vec_;
   bsg_check(bsg_malloc(g_.ctx, wbytes, (void**)&d_c_), "c buffer");
   bsg_check(bsg_malloc(g_.ctx, wbytes, (void**)&d_v_), "v buffer");
   bsg_chunk_t chunk;
@@ -241,6 +247,7 @@ GpuSigma::GpuSigma(GpuContext& g, const SuperBlock& sb, int max_vec) : g_(g) {
   chunk.pairs = wl.pairs.data();
   chunk.n_pairs = (int32_t)wl.pairs.size();
   bsg_check(bsg_plan_create(g_.ctx, &chunk, 1, wave_size_, &plan_), "plan");
+  if (max_vec_ > 1) set_batch(max_vec_);
   flops_ = bsg_plan_flops(plan_);
   double t3 = secs();
   g_setup.compile += t1 - t0;
@@ -273,11 +280,19 @@ void GpuSigma::apply_device(const double* dc, double* dv, int nvec) {
   bsg_check(bsg_sigma_apply(g_.ctx, plan_, d_arena_, dc, dv, nvec, precision_), "sigma");
 }
 
-void GpuSigma::apply(const double* c_host, double* v_host) {
-  size_t bytes = sizeof(double) * wave_size_;
+void GpuSigma::set_batch(int nroot) {
+  nroot = std::max(1, nroot);
+  if (nroot == batch_) return;
+  bsg_check(bsg_plan_set_batch(g_.ctx, plan_, nroot), "batch");
+  batch_ = nroot;
+}
+
+void GpuSigma::apply(const double* c_host, double* v_host, int nvec) {
+  if (nvec < 1 || nvec > max_vec_) throw std::runtime_error("GpuSigma::apply: more vectors than the buffers hold");
+  size_t bytes = sizeof(double) * wave_size_ * nvec;
   bsg_check(bsg_memcpy_h2d(g_.ctx, d_c_, c_host, bytes), "c upload");
   bsg_check(bsg_memset_zero(g_.ctx, d_v_, bytes), "v clear");
-  apply_device(d_c_, d_v_, 1);
+  apply_device(d_c_, d_v_, nvec);
   bsg_check(bsg_memcpy_d2h(g_.ctx, v_host, d_v_, bytes), "v download");
 }
 
@@ -299,7 +314,7 @@ void preload_ops(GpuContext& g, int slot, const std::vector<OpPtr>& ops) {
   int64_t n = 0;
   for (const OpPtr& o : ops) {
     at.push_back({o.get(), n});
-    n += (int64_t)o->data.size();
+    n += (int64_t)o->size();
   }
   if (p.cap < (size_t)n) {
     if (p.buf) bsg_check(bsg_mfree(g.side->ctx, p.buf), "preload free");
@@ -340,15 +355,17 @@ std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, c
   int64_t soff = 0;
   for (const SparseOp* o : ops) {
     src.push_back({o, soff});
-    soff += (int64_t)o->data.size();
+    soff += (int64_t)o->size();
   }
   std::vector<int64_t> ooff(ops.size());
   int64_t osize = 0;
   for (size_t w = 0; w < ops.size(); ++w) {
     const SparseOp& o = *ops[w];
-    out[w] = std::make_shared<SparseOp>(get_pattern(R.ns, o.dn(), o.k2(), o.dirrep()), o.fermion);
+    out[w] = device_out_ ? std::make_shared<SparseOp>(get_pattern(R.ns, o.dn(), o.k2(), o.dirrep()), o.fermion,
+                                                      SparseOp::DeviceOnly{})
+                         : std::make_shared<SparseOp>(get_pattern(R.ns, o.dn(), o.k2(), o.dirrep()), o.fermion);
     ooff[w] = soff + osize;
-    osize += (int64_t)out[w]->data.size();
+    osize += (int64_t)out[w]->size();
   }
   const int64_t total = soff + osize;
   if (osize == 0) return out;
@@ -386,9 +403,9 @@ std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, c
   for (const auto& x : src) {
     auto f = g_.resident.find(x.first);
     if (f != g_.resident.end() && g_.arena && !g_.arena_leased)
-      segs.push_back({g_.arena + f->second, base + x.second, (int64_t)x.first->data.size()});
+      segs.push_back({g_.arena + f->second, base + x.second, (int64_t)x.first->size()});
     else
-      host.push_back(x);
+      host.push_back(x);  // upload_ops copies device-resident ones on the device
   }
   bsg_check(bsg_copy_segments(g_.ctx, segs.data(), (int32_t)segs.size()), "operator copy");
   upload_ops(g_, host, base, (size_t)soff);
@@ -410,6 +427,18 @@ std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, c
     const int rc = bsg_sigma_apply(g_.ctx, plan, ubuf, base, base, 1, BSG_FP64);
     bsg_plan_destroy(plan);
     bsg_check(rc, "rotation");
+  }
+  if (device_out_) {
+    // the rotated operators stay in HBM: one buffer for the batch
+    double* nb = nullptr;
+    auto owner = device_buffer(g_, (size_t)osize, &nb);
+    const bsg_copy_seg_t seg{base + soff, nb, osize};
+    bsg_check(bsg_copy_segments(g_.ctx, &seg, 1), "rotated operators");
+    for (size_t w = 0; w < ops.size(); ++w) {
+      out[w]->dev = nb + (ooff[w] - soff);
+      out[w]->dev_owner = owner;
+    }
+    return out;
   }
   // rotated operators back through the staging
   double* staging = g_.ensure_staging((size_t)osize);

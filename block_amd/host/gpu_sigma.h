@@ -95,8 +95,11 @@ class GpuSigma {
   // (the BF16 slices of the arena are built on first use)
   void set_precision(int mode);
   int precision() const { return precision_; }
-  // host vectors: v = H c (overwrites v)
-  void apply(const double* c_host, double* v_host);
+  // host vectors: v = H c for nvec <= max_vec vectors (overwrites v)
+  void apply(const double* c_host, double* v_host, int nvec = 1);
+  // roots per launch of apply_device (bsg_plan_set_batch)
+  void set_batch(int nroot);
+  int max_vec() const { return max_vec_; }
   // device vectors: v += H c
   void apply_device(const double* dc, double* dv, int nvec = 1);
   std::vector<double> diagonal();
@@ -116,6 +119,8 @@ class GpuSigma {
   bool owns_arena_ = false;
   double* d_c_ = nullptr;
   double* d_v_ = nullptr;
+  int max_vec_ = 1;
+  int batch_ = 1;
   bsg_plan* plan_ = nullptr;
   int precision_ = BSG_FP64;
   bool split_ready_ = false;
@@ -132,15 +137,56 @@ class GpuSigma {
 // context's preload store, so the copy runs beside the sigma kernels of the
 // solve; rotate() then copies them on the device (the sweep joins the
 // preloading thread before it rotates).
+// device_out: the rotated operators stay in HBM (device-only operators, one
+// buffer per batch) instead of coming back to host memory.
 class GpuRotator : public OpRotator {
  public:
-  explicit GpuRotator(GpuContext& g) : g_(g) {}
+  explicit GpuRotator(GpuContext& g, bool device_out = false) : g_(g), device_out_(device_out) {}
   std::vector<OpPtr> rotate(const std::vector<const SparseOp*>& ops, const Rotation& R) const override;
-  void preload(const std::vector<OpPtr>& ops) const override { preload_ops(g_, 0, ops); }
+  void preload(const std::vector<OpPtr>& ops) const override {
+    if (!device_out_) preload_ops(g_, 0, ops);
+  }
+
+ private:
+  GpuContext& g_;
+  bool device_out_;
+};
+
+// A device allocation of n doubles freed (through g) when the last owner
+// goes; g must outlive it.
+std::shared_ptr<void> device_buffer(GpuContext& g, size_t n, double** ptr);
+
+// Composite blocks built in HBM (the GPU side of BuildSumBlock and of the
+// complementary operators of the renormalisation): the host records the
+// scaled block copies of blk::build_sum_block / build_comp_ops (BuildSession)
+// and the device executes them from the block operators' device copies
+// (bsg_build_apply); the new operators are device-only.  Same operators,
+// bitwise, as the host build.
+class GpuBlockBuilder : public BlockBuilder {
+ public:
+  explicit GpuBlockBuilder(GpuContext& g) : g_(g) {}
+  BlockPtr sum_block(const Integrals& I, const SpinBlock& B1, const SpinBlock& dot, const Quantum& target, int nrem,
+                     const BuildSpec& spec, const Screen& scr) const override;
+  std::shared_ptr<CompOpsAhead> comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1,
+                                         const SpinBlock& dot, const Screen& scr) const override;
 
  private:
   GpuContext& g_;
 };
+// Execute a recorded build: every operator of `targets` and every operator an
+// item writes gets device storage (targets in one buffer they own, the
+// other written ones in scratch), sources are read from their device copies
+// (host-only sources are uploaded first).
+void execute_build(GpuContext& g, const BuildSession& session, const std::vector<SparseOp*>& targets);
+// Make g the context that downloads device-only operators for host readers
+// (materialise_host); null: none.
+void set_materialiser_context(GpuContext* g);
+// Seconds spent recording and executing device builds (accumulated).
+struct BuildTimes {
+  double record = 0, execute = 0;
+  double bytes = 0;  // algorithmic bytes of the executed builds (sources read, destinations written)
+};
+const BuildTimes& gpu_build_times();
 
 // Largest Davidson subspace (deflation_max + 1 vectors) the device solve takes.
 constexpr int kDeviceDavidsonMaxVectors = 128;

@@ -259,6 +259,8 @@ OpPtr reduce4(const SIPtr& si, const T4& t, int dn, int dirrep, bool fermion) {
 }  // namespace
 
 BlockPtr make_dot(const Integrals& I, int d, const Screen& scr) {
+  // dot operators are written element by element on the host
+  if (build_recording()) throw std::runtime_error("make_dot: not inside a build recording");
   const int n = I.norb;
   const int g = I.orbsym[d];
   auto si = std::make_shared<StateInfo>();
@@ -556,8 +558,11 @@ struct Task {
 void run_tasks(const Builder& bd, SpinBlock& b, const std::vector<Task>& tasks,
                const std::function<OpPtr(OpPtr)>& post) {
   std::string err;
+  BuildSession::Impl* const session = current_build_session();  // workers record into the caller's
 #pragma omp parallel for schedule(dynamic)
   for (long t = 0; t < (long)tasks.size(); ++t) {
+    BuildSession::Impl* const outer = current_build_session();
+    enter_build_session(session);
     try {
     const Task& k = tasks[t];
     OpPtr o;
@@ -580,6 +585,7 @@ void run_tasks(const Builder& bd, SpinBlock& b, const std::vector<Task>& tasks,
 #pragma omp critical(blk_task_err)
       if (err.empty()) err = e.what();
     }
+    enter_build_session(outer);
   }
   if (!err.empty()) throw std::runtime_error(err);
 }

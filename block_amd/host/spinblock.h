@@ -163,6 +163,15 @@ void add_rotated_comp_ops(const CompOpsAhead& ahead, const Rotation& R, SpinBloc
                           const OpRotator* rotator = nullptr);
 // the operators built ahead (the ones add_rotated_comp_ops rotates)
 std::vector<OpPtr> comp_ops_list(const CompOpsAhead& ahead);
+// Another place to build composite blocks (the GPU): the same operators as
+// build_sum_block / build_comp_ops, possibly device-only.
+struct BlockBuilder {
+  virtual ~BlockBuilder() = default;
+  virtual BlockPtr sum_block(const Integrals& I, const SpinBlock& B1, const SpinBlock& dot, const Quantum& target,
+                             int nrem, const BuildSpec& spec, const Screen& scr) const = 0;
+  virtual std::shared_ptr<CompOpsAhead> comp_ops(const Integrals& I, const SpinBlock& B, const SpinBlock& B1,
+                                                 const SpinBlock& dot, const Screen& scr) const = 0;
+};
 // seconds spent building complementary operators before a batch rotation (accumulated)
 double comp_build_seconds();
 

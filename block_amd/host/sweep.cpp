@@ -292,9 +292,22 @@ struct RenormTimes {
 };
 RenormTimes g_rt;
 
+// B = B1 (x) dot with the builder of the run (the GPU's) or the host's
+BlockPtr sum_block(const DmrgParams& P, const Integrals& I, const SpinBlock& B1, const SpinBlock& dot,
+                   const Quantum& target, int nrem, const BuildSpec& spec, const Screen& scr) {
+  return P.builder ? P.builder->sum_block(I, B1, dot, target, nrem, spec, scr)
+                   : build_sum_block(I, B1, dot, target, nrem, spec, scr);
+}
+
 BlockPtr truncate_block(const Integrals& I, const SpinBlock& B, const SpinBlock& B1, const SpinBlock& dot,
                         const std::vector<std::vector<double>>& rho, int M, const Screen& scr, double* disc,
-                        const OpRotator* rotator, const CompOpsAhead* ahead = nullptr) {
+                        const OpRotator* rotator, const CompOpsAhead* ahead = nullptr,
+                        const BlockBuilder* builder = nullptr) {
+  std::shared_ptr<CompOpsAhead> built;
+  if (!B.has_comp && !ahead && builder) {
+    built = builder->comp_ops(I, B, B1, dot, scr);
+    ahead = built.get();
+  }
   std::vector<std::vector<double>> rot;
   std::vector<int> mkeep;
   double t0 = now_s();
@@ -394,8 +407,8 @@ SuperBlock infinite_warmup(const Integrals& I, const DmrgParams& P, const SigmaF
     BuildSpec sr;
     sr.normal = true;
     sr.comp = true;
-    BlockPtr L = build_sum_block(I, *fwd[s], *dot1, target, k - (s + 2), sl, scr);
-    BlockPtr R = build_sum_block(I, *bwd[k - 1 - s], *dot2, target, k - (s + 2), sr, scr);
+    BlockPtr L = sum_block(P, I, *fwd[s], *dot1, target, k - (s + 2), sl, scr);
+    BlockPtr R = sum_block(P, I, *bwd[k - 1 - s], *dot2, target, k - (s + 2), sr, scr);
     SuperBlock sb = make_superblock(I, L, R, tp);
     if (sb.wave->size < P.nroots) throw std::runtime_error("infinite_warmup: partial superblock too small");
     Solved sol = solve_superblock(sb, P, tol, make_sigma, rng, nullptr, P.ladder_from_sweep <= 0 ? P.ladder : 0.0);
@@ -405,8 +418,9 @@ SuperBlock infinite_warmup(const Integrals& I, const DmrgParams& P, const SigmaF
     double d1 = 0, d2 = 0;
     auto rhoL = density_matrix(sb, sol, weights, true, noise, rng);
     auto rhoR = density_matrix(sb, sol, weights, false, noise, rng);
-    fwd[s + 1] = truncate_block(I, *L, *fwd[s], *dot1, rhoL, M, scr, &d1, P.rotator.get());
-    BlockPtr Rn = truncate_block(I, *R, *bwd[k - 1 - s], *dot2, rhoR, M, scr, &d2, P.rotator.get());
+    fwd[s + 1] = truncate_block(I, *L, *fwd[s], *dot1, rhoL, M, scr, &d1, P.rotator.get(), nullptr, P.builder.get());
+    BlockPtr Rn =
+        truncate_block(I, *R, *bwd[k - 1 - s], *dot2, rhoR, M, scr, &d2, P.rotator.get(), nullptr, P.builder.get());
     bwd[k - 2 - s] = Rn;
     if (kp == k || 2 * (s + 3) > k) {
       last = sb;
@@ -485,7 +499,8 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
     double t0 = now_s();
     const int first = (sweep == 0) ? std::max(0, l0) : 0;
     const int nsteps = k - 3;
-    const bool use_ahead = P.host_ahead > 0 || (P.host_ahead < 0 && P.rotator);
+    // (not with a builder of its own: the device builds share the solve's context)
+    const bool use_ahead = !P.builder && (P.host_ahead > 0 || (P.host_ahead < 0 && P.rotator));
     BlockPtr next_R;  // the next step's R, built during this step's solve
     const SpinBlock* next_env = nullptr;
     int next_dot = -1;
@@ -507,10 +522,10 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       sr.normal = false;
       sr.comp = true;
       double ts = now_s();
-      BlockPtr L = build_sum_block(I, *sys, *dot1, target, nR, sl, scr);
+      BlockPtr L = sum_block(P, I, *sys, *dot1, target, nR, sl, scr);
       BlockPtr R = (next_R && next_env == env.get() && next_dot == d2)
                        ? next_R
-                       : build_sum_block(I, *env, *dot2, target, nL, sr, scr);
+                       : sum_block(P, I, *env, *dot2, target, nL, sr, scr);
       next_R.reset();
       next_env = nullptr;
       SuperBlock sb = make_superblock(I, L, R, target);
@@ -586,7 +601,8 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       auto rho = density_matrix(sb, sol, weights, true, S->noise, rng);
       trho = now_s() - trho;
       double disc = 0.0;
-      BlockPtr nsys = truncate_block(I, *L, *sys, *dot1, rho, S->M, scr, &disc, P.rotator.get(), compL.get());
+      BlockPtr nsys = truncate_block(I, *L, *sys, *dot1, rho, S->M, scr, &disc, P.rotator.get(), compL.get(),
+                                     P.builder.get());
       compL.reset();
       rec.max_discarded = std::max(rec.max_discarded, disc);
       rec.renorm_seconds += now_s() - ts;

@@ -34,6 +34,9 @@ struct DmrgParams {
   int ladder_from_sweep = 0;  // first sweep that uses the ladder (0: also the warm-up)
   bool transform_guess = true;  // start each step from the previous wavefunction in the new basis
   std::shared_ptr<const OpRotator> rotator;  // operator rotations of the renormalisation (null: host threads)
+  // composite blocks (sys (x) dot, their complementary operators) built
+  // elsewhere, e.g. in HBM from device-resident blocks (null: host threads)
+  std::shared_ptr<const BlockBuilder> builder;
   std::string checkpoint;  // file the sweep state is written to after every sweep (empty: none)
   std::string resume;      // sweep-state file to continue from instead of the warm-up (empty: none)
   // build host-side operators that do not depend on the current solve (the

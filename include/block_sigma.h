@@ -127,7 +127,16 @@ int64_t bsg_plan_arena_extent(const bsg_plan* plan);
  * No counterpart in the reference (its sigma is FP64 only). */
 int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* plan, const double* arena, int64_t n);
 
-/* v += H c for nvec vectors stored back to back (stride wave_size doubles).
+/* Roots per launch (default 1): bsg_sigma_apply runs up to nroot vectors in
+ * one set of kernel launches (every tile once per root, a tile's roots on one
+ * XCD so they share its operator blocks in L2), the block Davidson's new
+ * vectors of one iteration (linear.C:234-258) and state-averaged roots
+ * together.  Reallocates the plan's partial tiles and C slices for nroot
+ * vectors (after synchronising the context stream). */
+int bsg_plan_set_batch(bsg_context* ctx, bsg_plan* plan, int32_t nroot);
+
+/* v += H c for nvec vectors stored back to back (stride wave_size doubles),
+ * in launches of up to the plan's batch size (bsg_plan_set_batch).
  * arena: device operator arena; c, v: device pointers.
  * precision: BSG_FP64, or BSG_SPLIT_BF16 (BF16 slice products on MFMA with
  * FP32 accumulation per 32-deep step and FP64 sums; needs
@@ -135,14 +144,20 @@ int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* plan, const double* arena, 
 int bsg_sigma_apply(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t nvec,
                     int32_t precision);
 
-/* Timing helper: runs `reps` sigma applications on the context stream with
- * HIP events around every kernel launch.  wall_ms: whole loop;
- * kernel_ms[0..2]: summed time of the fused GEMM-pair kernels (grouped
- * kernel on the context stream and small-block kernel beside it, fork to
- * join), of the partial-tile reduction and of the small-block kernel alone;
- * launches[0..2]: number of timed intervals of each. */
-int bsg_sigma_profile(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t reps,
-                      int32_t precision, double* wall_ms, double* kernel_ms, long* launches);
+/* Doubles of partial-tile scratch one application of the plan uses (V blocks
+ * whose pair list is cut into several work units; 0 if none). */
+int64_t bsg_plan_partial_doubles(const bsg_plan* plan);
+
+/* Timing helper: runs `reps` applications of the sigma to nvec vectors
+ * (c, v: nvec vectors back to back, roots batched as bsg_sigma_apply does)
+ * on the context stream with HIP events around every chunk launch.
+ * wall_ms: whole loop; kernel_ms[0..2]: summed time of the fused GEMM-pair
+ * kernels (grouped kernels on the context stream and the small-block and
+ * basic-tile kernels beside them, fork to join), of the partial-tile
+ * reduction and of the side stream's kernels alone; launches[0]: kernel
+ * dispatches of all applications (every launch counted), launches[1..2]: 0. */
+int bsg_sigma_profile(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t nvec,
+                      int32_t reps, int32_t precision, double* wall_ms, double* kernel_ms, long* launches);
 
 /* diag += diagonal terms (device pointers). */
 int bsg_diag_apply(bsg_context* ctx, const bsg_diag_t* terms_host, int32_t n_terms, const double* arena,
@@ -169,6 +184,28 @@ typedef struct {
   int64_t n;
 } bsg_copy_seg_t;
 int bsg_copy_segments(bsg_context* ctx, const bsg_copy_seg_t* segs_host, int32_t n_segs);
+
+/* Composite-operator build in device memory (replaces the host's
+ * TensorProduct / TensorProductElement for a block (x) dot operator,
+ * operatorfunctions.C:146-254, through MatrixTensorProduct's scaled block
+ * copies, MatrixBLAS.C:138-162, and the operator-vector sums of the
+ * complementary operators, Operators.C:1059-1196,1472-1616,1905-1966):
+ * group g adds into the destination rectangle dst[r*ld + c] (r < rows,
+ * c < cols) the sum over its sources j in [first, first+count) of
+ * (alpha_j * op(src_j)[r][c]) * beta_j, op(src)[r][c] = trans ? src[c*ld_j +
+ * r] : src[r*ld_j + c], summed in list order.  Rectangles of different groups
+ * must not overlap.  Device pointers; returns when the build is done. */
+typedef struct {
+  double* dst;
+  int32_t ld, rows, cols, first, count, pad;
+} bsg_build_group_t;
+typedef struct {
+  const double* src;
+  int32_t ld, trans;
+  double alpha, beta;
+} bsg_build_src_t;
+int bsg_build_apply(bsg_context* ctx, const bsg_build_group_t* groups_host, int32_t n_groups,
+                    const bsg_build_src_t* srcs_host, int32_t n_srcs);
 /* Subspace matrix of the block Davidson (linear.C:265-270): out_host[i*m+j]
  * = <X_i, Y_j> for m <= 128 device vectors of length n stored with stride ld
  * (the reference's subspace holds max(20, nroots+20) vectors, input.C:885). */

@@ -126,7 +126,7 @@ void oracle_multiply_h(const SuperBlock& sb, const double* c, double* v, int nth
 #pragma omp parallel num_threads(nthreads)
   {
     int t = omp_get_thread_num();
-#pragma omp for schedule(dynamic)
+#pragma omp for schedule(static, 1)  // fixed pair -> thread map: reproducible sums
     for (long i = 0; i < (long)P.size(); ++i) tensor_multiply(sb, P[i].a, P[i].b, P[i].coef, c, part[t].data());
   }
   for (int t = 0; t < nthreads; ++t)

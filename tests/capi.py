@@ -24,11 +24,14 @@ for name, args in [
     ("bsg_gram", [vp, vp, vp, _i32, _i64, _i64, vp]),
     ("bsg_lincomb", [vp, vp, _i32, _i64, _i64, vp, _i32, vp, _i64]),
     ("bsg_gs_orthonormalise", [vp, vp, vp, _i32, _i64, _i64, _i32]),
+    ("bsg_sigma_apply", [vp, vp, vp, vp, vp, _i32, _i32]),
 ]:
     f = getattr(lib, name)
     f.restype = ctypes.c_int
     f.argtypes = args
 lib.bsg_last_error.restype = ctypes.c_char_p
+lib.bsg_plan_partial_doubles.restype = ctypes.c_int64
+lib.bsg_plan_partial_doubles.argtypes = [vp]
 
 
 def check(rc):

@@ -49,6 +49,22 @@ def test_resume_rejects_a_mismatched_run(tmp_path):
     assert sw.shape[0] == 3
 
 
+def test_superblock_from_saved_state(tmp_path):
+    """The mid-chain superblock rebuilt from a saved state's blocks solves to
+    the energy the sweeps reached (the state's blocks are converged ones)."""
+    ints = config_fcidump("hubbard12")
+    ck = str(tmp_path / "state.bin")
+    e, sw = _oracle.dmrg_cpu(ints, schedule=SCHED, max_sweeps=6, sweep_tol=0.0, checkpoint=ck)
+    sb = block_amd.SuperBlock.from_sweep_state(ints, ck)
+    d = _oracle.diagonal(sb)
+    g = np.zeros((1, sb.wave_size))
+    g[0, np.argmin(d)] = 1.0
+    e_sb, _, _ = _oracle.davidson(sb, g, tol=1e-14)
+    assert abs(e_sb[0] - e[0]) < 1e-7
+    with pytest.raises(block_amd.BlockError):
+        block_amd.SuperBlock.from_sweep_state(ints, ck, step=ints.norb - 2)
+
+
 def test_resume_rejects_garbage(tmp_path):
     ints = config_fcidump("hubbard12")
     bad = tmp_path / "bad.bin"

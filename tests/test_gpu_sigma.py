@@ -132,3 +132,30 @@ def test_copy_segments():
             if p.value:
                 lib.bsg_mfree(ctx, p)
         lib.bsg_free(ctx)
+
+
+BATCH_CASES = ["exact7d2h", "hubbard12_M100", "syn16_M80", "syn24_M500_dmrg"]
+
+
+@pytest.mark.parametrize("name,make", [c for c in _cases() if c[0] in BATCH_CASES], ids=BATCH_CASES)
+def test_batched_roots(name, make):
+    """Several roots in one set of launches (linear.C:234-258): every root's
+    sigma is bitwise the one-vector sigma, for a batch of four, a batch of
+    three over four vectors (3 + 1) and in split precision."""
+    print(f"[gpu] batched roots {name}", flush=True)
+    sb = make()
+    h = block_amd.multiply_h(sb, max_vec=4)
+    c = np.stack([rng_vec(sb.wave_size, 20 + i) for i in range(4)])
+    for prec in (block_amd.FP64, block_amd.SPLIT_BF16):
+        h.precision = prec
+        h.set_batch(1)
+        one = np.stack([h(c[i]) for i in range(4)])
+        for nb in (4, 3):
+            h.set_batch(nb)
+            got = h(c)
+            assert np.array_equal(got, one), (prec, nb)
+    h.precision = block_amd.FP64
+    ref = _oracle.sigma(sb, c[2], nthreads=8)
+    assert np.linalg.norm(got[2] - one[2]) == 0.0
+    h.set_batch(4)
+    assert np.linalg.norm(h(c)[2] - ref) <= TOL * np.linalg.norm(ref)
