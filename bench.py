@@ -1,18 +1,30 @@
 """Benchmark of the Davidson sigma step (H.C) on one MI355X.
 
-A step is one H.C application (multiply_h, davidson.C:17-22) of the FP64 GPU
-path on the mid-chain two-site superblock of the 24-orbital / 24-electron C1
-synthetic Hamiltonian at M = 500, reached by the infinite-system DMRG warm-up
-(both halves truncated by density matrices).  Operator blocks and the
-wavefunction are resident in HBM before the timed region; the timed region is
-exactly --steps sigma applications bracketed by stream synchronisation, timed
-with HIP events on the sigma stream.  The same line reports the
-split-precision sigma on that superblock, one full sweep at M = 500 in FP64
-and with the precision ladder, and the CPU baseline.
+Workload (default --config syn36d2h): the 36-orbital / 36-electron synthetic
+Hamiltonian with D2h irreps round-robin (integrals breaking the symmetry set
+to zero), S = 0, at bond dimension M = 1000.  The run first grows the blocks
+by a noisy two-site sweep ramp (M 250 -> 500 -> 1000, density-matrix noise
+1e-4 -> 1e-5, FP64 sigma, composite operators built in HBM), checkpointing the
+sweep state after every sweep.  Then:
 
-Prints one JSON line (see the fields below); with --gpus > 1 it exits with a
-message, the path runs on one GPU.  --summarize-trace / --summarize-traffic
-turn rocprofv3 CSVs of such a run into the tables kept with the profiles.
+* step: one H.C (multiply_h, davidson.C:17-22) of the FP64 GPU path on the
+  mid-chain two-site superblock rebuilt from the ramp's final blocks; operator
+  blocks and wavefunction resident in HBM; the timed region is exactly --steps
+  applications bracketed by stream synchronisation, HIP events on the sigma
+  stream.  `value` is its FP64-equivalent algorithmic TFLOP/s.
+* mixed_split_bf16: the same superblock in split precision (three BF16
+  slices, FP32 MFMA steps, FP64 sums), its speed-up and errors.
+* batched_roots: four vectors in one launch set against four serial sigmas.
+* sweep: one full two-site sweep (forward and backward) at M from the ramp's
+  checkpoint, once with FP64 sigmas and once with the precision ladder
+  (split-precision sigmas until the Davidson residual norm is below 1e-5,
+  FP64 after), sweep times, phases and |dE|.
+* cpu_baseline: the oracle's restatement of multiplyH with the reference's
+  OpenMP scheme on all host cores, on a bounded sample of the same H.C.
+
+Prints one JSON line; with --gpus > 1 it exits with a message (the path runs
+on one GPU).  --summarize-trace / --summarize-traffic / --summarize-sweep turn
+rocprofv3 CSVs of such a run into the tables kept with the profiles.
 """
 import argparse
 import ctypes
@@ -34,63 +46,119 @@ from block_amd import integrals as gen  # noqa: E402
 
 METRIC = "H·C sigma TFLOP/s + sweep time, mixed vs FP64 GPU at |ΔE|≤1e-8 Eh"
 PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+COLS = block_amd.SWEEP_COLS
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(sb, seconds_budget):
-    """The oracle's direct multiplyH restatement (OpenMP over operator pairs,
-    per-thread sigma copies) on the host cores: one full H.C of the same
-    superblock, repeated while under the time budget."""
+    """The oracle's restatement of multiplyH with the reference's parallel
+    scheme (OpenMP over operator pairs, per-thread sigma copies summed at the
+    end, operatorloops.h:87-97, distribute.h:143-147) and the host's blocked
+    DGEMM for the sector products, on every host core this process may use:
+    full H.C of the same superblock, repeated within the time budget."""
     path = os.path.join(ROOT, "oracle", "liboracle.so")
     lib = ctypes.CDLL(path)
-    f = lib.orc_sigma_timed
+    f = lib.orc_sigma_fast_timed
     dp = ctypes.POINTER(ctypes.c_double)
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_int, dp]
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    threads = max(1, min(threads, 16))
     c = np.random.default_rng(1).standard_normal(sb.wave_size)
     v = np.zeros(sb.wave_size)
     total, n = 0.0, 0
     while n == 0 or total < seconds_budget:
         t = ctypes.c_double()
-        rc = f(sb._h, c.ctypes.data_as(dp), v.ctypes.data_as(dp), threads, ctypes.byref(t))
-        if rc != 0:
+        if f(sb._h, c.ctypes.data_as(dp), v.ctypes.data_as(dp), threads, ctypes.byref(t)) != 0:
             raise RuntimeError("oracle sigma failed")
         total += t.value
         n += 1
-        if total > seconds_budget / 2 and n >= 1:
+        if total > seconds_budget / 2:
             break
     return {"value": sb.flops * n / total / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"{n} full H.C of the same superblock ({sb.flops:.3e} flop each), {total:.1f} s"}
+            "cpu": cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"{n} full H.C of the benchmark superblock ({sb.flops:.3e} flop each), {total:.1f} s"}
 
 
-def sweep_leg(ints, cfg, M):
-    """One full two-site sweep (forward and backward half-sweeps) at M after a
-    ramp from the warm-up (M/2 for one full sweep), in FP64 and with the
-    precision ladder (split-precision sigmas until the Davidson residual norm
-    is below 1e-5) in the timed sweeps; the warm-up and ramp are FP64 in both
-    runs, so both time the same sweeps from the same blocks.  Every step
-    starts from the previous step's transformed wavefunction."""
-    sched = [(0, M // 2, 1e-7, 1e-6), (2, M, 1e-12, 0.0)]
-    cols = block_amd.SWEEP_COLS
-    out = {"schedule": [list(r) for r in sched], "timed_sweeps": [2, 3]}
+def _rows(sw, first, last):
+    return sw[(sw[:, COLS.index("sweep")] >= first) & (sw[:, COLS.index("sweep")] <= last)]
+
+
+def ramp_leg(ints, cfg, M, ck):
+    """FP64 sweeps growing the blocks to M with density-matrix noise, the
+    sweep state saved after every sweep (the timed legs start from it)."""
+    sched = [tuple(r) for r in cfg["ramp"]]
+    n = cfg["ramp_sweeps"]
+    t0 = time.time()
+    e, sw = block_amd.dmrg(ints, nelec=cfg["nelec"], schedule=sched, max_sweeps=n, sweep_tol=0.0, checkpoint=ck,
+                           verbose=1)
+    fwd, bwd, fsec, bsec = block_amd.state_block_dims(ints, ck)
+    k = ints.norb
+    return {"schedule": [list(r) for r in sched], "sweeps": n, "seconds": round(time.time() - t0, 1),
+            "energy": float(e[0]),
+            "sweep_seconds": [round(float(x), 2) for x in sw[:, COLS.index("seconds")]],
+            "kept_states_fwd": fwd, "kept_states_bwd": bwd, "sectors_fwd": fsec, "sectors_bwd": bsec,
+            "kept_states_mid": fwd[k // 2 - 2], "max_kept": max(fwd + bwd)}
+
+
+def sweep_leg(ints, cfg, M, ck):
+    """One full two-site sweep at M from the ramp's saved state, in FP64 and
+    with the precision ladder; every step starts from the previous step's
+    transformed wavefunction."""
+    n = cfg["ramp_sweeps"]
+    sched = [tuple(r) for r in cfg["ramp"]] + [(n, M, cfg["timed_tol"], 0.0)]
+    out = {"from_sweep": n, "davidson_tol": cfg["timed_tol"], "timed_sweeps": [n, n + 1]}
     for name, ladder in (("fp64", 0.0), ("mixed", 1e-5)):
         t0 = time.time()
-        e, sw = block_amd.dmrg(ints, nelec=cfg["nelec"], schedule=sched, max_sweeps=4, sweep_tol=0.0,
-                               ladder=ladder, ladder_from=2)
+        e, sw = block_amd.dmrg(ints, nelec=cfg["nelec"], schedule=sched, max_sweeps=n + 2, sweep_tol=0.0,
+                               ladder=ladder, ladder_from=n, resume=ck, verbose=1)
         wall = time.time() - t0
-        rows = sw[2:4]
-        col = lambda c: float(rows[:, cols.index(c)].sum())
+        rows = _rows(sw, n, n + 1)
+        col = lambda c: float(rows[:, COLS.index(c)].sum())
         out[name] = {"sweep_seconds": round(col("seconds"), 3), "sigma_seconds": round(col("sigma_seconds"), 3),
                      "sigma_calls": int(col("sigma_calls")), "split_calls": int(col("split_calls")),
                      "build_seconds": round(col("build_seconds"), 3),
                      "davidson_other_seconds": round(col("davidson_seconds"), 3),
                      "renorm_seconds": round(col("renorm_seconds"), 3),
                      "sigma_tflops": round(col("sigma_flops") / max(col("sigma_seconds"), 1e-9) / 1e12, 3),
+                     "max_discarded": float(rows[:, COLS.index("discarded")].max()),
                      "energy": float(e[0]), "run_seconds": round(wall, 2)}
+        log(f"sweep {name}: {out[name]['sweep_seconds']} s, E = {out[name]['energy']:.12f}")
     out["sweep_speedup_mixed_vs_fp64"] = round(out["fp64"]["sweep_seconds"] / out["mixed"]["sweep_seconds"], 3)
-    out["sigma_speedup_mixed_vs_fp64"] = round(out["fp64"]["sigma_seconds"] / out["mixed"]["sigma_seconds"], 3)
+    out["sigma_speedup_mixed_vs_fp64"] = round(out["fp64"]["sigma_seconds"] / max(out["mixed"]["sigma_seconds"], 1e-9), 3)
     out["abs_dE_Eh"] = abs(out["fp64"]["energy"] - out["mixed"]["energy"])
+    return out
+
+
+def batched_leg(sb, steps, warmup, nvec=4):
+    """nvec vectors in one launch set (every tile once per root, a tile's
+    roots on one XCD) against nvec serial sigmas, FP64 and split."""
+    h = block_amd.multiply_h(sb, max_vec=nvec)
+    c = np.stack([np.random.default_rng(10 + i).standard_normal(sb.wave_size) for i in range(nvec)])
+    out = {"nvec": nvec}
+    for name, prec in (("fp64", block_amd.FP64), ("split", block_amd.SPLIT_BF16)):
+        h.precision = prec
+        res = {}
+        for label, nb in (("serial", 1), ("batched", nvec)):
+            h.set_batch(nb)
+            wall, _, disp = h.timed(c, warmup, steps)
+            res[label] = {"ms_per_step": round(wall / steps, 4), "dispatches_per_step": disp / steps,
+                          "tflops": round(h.flops * nvec * steps / (wall * 1e-3) / 1e12, 3)}
+        res["speedup_vs_serial"] = round(res["serial"]["ms_per_step"] / res["batched"]["ms_per_step"], 3)
+        out[name] = res
+    h.precision = block_amd.FP64
     return out
 
 
@@ -260,19 +328,21 @@ def fetch_val(d, name):
     return next(iter(d.values()))[name]
 
 
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="syn24")
+    ap.add_argument("--config", default="syn36d2h")
     ap.add_argument("--M", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision timing")
-    ap.add_argument("--no-sweep", action="store_true", help="skip the sweep-time leg")
-    ap.add_argument("--sweep-only", action="store_true", help="only the sweep-time leg (its JSON object)")
-    ap.add_argument("--wilson", action="store_true", help="Wilson-warmed blocks instead of the DMRG warm-up")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the timed sweeps")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-roots leg")
+    ap.add_argument("--state", metavar="FILE", help="start from this saved sweep state instead of the ramp")
+    ap.add_argument("--keep-state", metavar="FILE", help="copy the ramp's saved state here")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run a device copy of N doubles (HBM counter calibration)")
     ap.add_argument("--summarize-trace", metavar="DIR", help="table of a rocprofv3 kernel trace (no GPU run)")
@@ -297,66 +367,65 @@ def main():
 
     cfg = gen.CONFIGS[args.config]
     M = args.M or cfg["M"]
+    if args.M:
+        cfg = dict(cfg, ramp=[(s, min(m, M), t, n) for (s, m, t, n) in cfg["ramp"]])
     norb = cfg["norb"]
-    nsys = norb // 2 - 1
     tmp = tempfile.mkdtemp(prefix="blkbench_")
     fcidump = os.path.join(tmp, f"{args.config}.FCIDUMP")
     gen.make_config_fcidump(args.config, fcidump)
     ints = block_amd.Integrals(fcidump)
-    if args.sweep_only:
-        print(json.dumps({"sweep": sweep_leg(ints, cfg, M)}))
-        return 0
+    ck = args.state or os.path.join(tmp, "ramp.state")
+    ramp = None
+    if not args.state:
+        log(f"ramp to M={M} on {args.config}")
+        ramp = ramp_leg(ints, cfg, M, ck)
+        log(f"ramp done in {ramp['seconds']} s, kept states at mid-chain {ramp['kept_states_mid']}")
+        if args.keep_state:
+            import shutil
+            shutil.copyfile(ck, args.keep_state)
     t0 = time.time()
-    if args.wilson:
-        sb = block_amd.SuperBlock(ints, cfg["nelec"], 0, 0, nsys, M)
-        warm = f"sys={nsys} sites (Wilson-warmed blocks)"
-    else:
-        sb = block_amd.SuperBlock.from_dmrg_warmup(ints, cfg["nelec"], 0, 0, M, nroots=cfg.get("nroots", 1))
-        warm = "halves of norb/2 sites from the infinite-system DMRG warm-up (density-matrix truncation)"
+    sb = block_amd.SuperBlock.from_sweep_state(ints, ck, cfg["nelec"], 0, ints.isym)
     t_build = time.time() - t0
     h = block_amd.multiply_h(sb, device=0)
     c = np.random.default_rng(0).standard_normal(sb.wave_size)
     c /= np.linalg.norm(c)
-    wall_ms, kern_ms, launches = h.timed(c, args.warmup, args.steps)
+    log(f"superblock: wave {sb.wave_size}, {h.pairs} GEMM pairs, {h.flops:.3e} flop per H.C")
+    wall_ms, kern_ms, disp = h.timed(c, args.warmup, args.steps)
     flops = h.flops
     f1, f2 = h.phase_flops()
     ms_per_step = wall_ms / args.steps
     value = flops * args.steps / (wall_ms * 1e-3) / 1e12
-    # dominant kernel: the fused GEMM-pair kernel (k_reduce is the ordered
-    # partial-tile sum); its algorithmic flops are both GEMMs of every pair
-    # (k_sigma, the grouped kernel, with k_small, the small-block kernel,
-    # running beside it on a second stream: timed fork to join)
-    kname = "k_sigma+k_small"
-    kflops = f1 + f2
-    k = 0
-    achieved = kflops * args.steps / (kern_ms[0] * 1e-3) / 1e12
+    # dominant kernels: the fused GEMM-pair kernels of one H.C (grouped tiles
+    # on the sigma stream, small blocks and basic tiles beside them), timed
+    # fork to join with HIP events on the sigma stream
+    achieved = (f1 + f2) * args.steps / (kern_ms[0] * 1e-3) / 1e12
     traffic = traffic_from_profile(args.config, M, sb.wave_size, h.pairs)
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_MFMA, 5), "traffic": traffic, "kernel": kname,
-                "kernel_ms_per_step": round(kern_ms[k] / args.steps, 4),
-                "launches_per_step": launches[k] / args.steps,
-                "kernel_ms_per_step_all": {"k_sigma+k_small": round(kern_ms[0] / args.steps, 4),
+                "frac": round(achieved / PEAK_FP64_MFMA, 5), "traffic": traffic, "kernel": "k_sigma+k_small",
+                "kernel_ms_per_step": round(kern_ms[0] / args.steps, 4),
+                "dispatches_per_step": disp / args.steps,
+                "kernel_ms_per_step_all": {"fused": round(kern_ms[0] / args.steps, 4),
                                            "k_reduce": round(kern_ms[1] / args.steps, 4),
-                                           "k_small": round(kern_ms[2] / args.steps, 4)}}
-    # split precision (BF16 slices, FP32 steps, FP64 sums) on the same superblock:
-    # throughput in FP64-equivalent algorithmic flops and its error against the
-    # FP64 GPU sigma (the FP64 path is parity-tested against the oracle)
+                                           "side_stream": round(kern_ms[2] / args.steps, 4)}}
     mixed = None
     if not args.no_split:
         v64 = h(c)
         h.precision = block_amd.SPLIT_BF16
         vs = h(c)
-        s_wall, s_kern, _ = h.timed(c, args.warmup, args.steps)
+        s_wall, s_kern, s_disp = h.timed(c, args.warmup, args.steps)
         h.precision = block_amd.FP64
         mixed = {"tflops": round(flops * args.steps / (s_wall * 1e-3) / 1e12, 4),
                  "ms_per_step": round(s_wall / args.steps, 4),
-                 "kernel_ms_per_step": {"k_sigma+k_small": round(s_kern[0] / args.steps, 4),
-                                        "k_reduce": round(s_kern[1] / args.steps, 4),
-                                        "k_small": round(s_kern[2] / args.steps, 4)},
+                 "kernel_ms_per_step": round(s_kern[0] / args.steps, 4),
+                 "dispatches_per_step": s_disp / args.steps,
                  "speedup_vs_fp64": round(wall_ms / s_wall, 3),
                  "sigma_rel_err_vs_fp64": float(np.linalg.norm(vs - v64) / np.linalg.norm(v64)),
                  "rayleigh_err_vs_fp64_Eh": float(abs(c @ vs - c @ v64))}
+    pairs, arena_gb = int(h.pairs), round(h.arena_bytes / 1e9, 3)
+    del h
+    batched = None if args.no_batch else batched_leg(sb, max(2, args.steps // 2), args.warmup)
     nl, nr, nb = sb.sectors()
+    shapes = sb.block_shapes()
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -371,19 +440,25 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{args.config}: one H.C on the mid-chain two-site superblock, {norb} orbitals / "
-                               f"{cfg['nelec']} electrons, S=0, {'D2h' if cfg.get('d2h') else 'C1'}, M={M}, "
-                               f"{warm}, FP64 sigma",
+                               f"{cfg['nelec']} electrons, S=0, {'D2h' if cfg.get('d2h') else 'C1'}, M={M}, blocks "
+                               f"from a noisy two-site sweep ramp, FP64 sigma",
                    "wave_size": sb.wave_size, "flop_per_step": flops, "terms": sb.nterms,
-                   "sectors_L_R_blocks": [nl, nr, nb], "gemm_pairs": h.pairs,
-                   "operator_arena_GB": round(h.arena_bytes / 1e9, 3), "superblock_build_s": round(t_build, 2),
+                   "sectors_L_R_blocks": [nl, nr, nb], "largest_V_block": max(shapes, key=lambda s: s[0] * s[1]),
+                   "gemm_pairs": pairs, "operator_arena_GB": arena_gb,
+                   "superblock_build_s": round(t_build, 2),
                    "host_peak_rss_GB": round(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1e6, 2)},
         "roofline": roofline,
     }
+    if ramp:
+        out["config"]["ramp"] = ramp
     if mixed:
         out["mixed_split_bf16"] = mixed
+    if batched:
+        out["batched_roots"] = batched
     if not args.no_sweep:
-        out["sweep"] = sweep_leg(ints, cfg, M)
+        out["sweep"] = sweep_leg(ints, cfg, M, ck)
     if not args.no_cpu:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(sb, args.cpu_seconds)
     print(json.dumps(out))
     return 0

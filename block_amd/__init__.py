@@ -76,6 +76,7 @@ _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
       ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp])
 _sig(_h, "blkh_build_check", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _lp])
+_sig(_h, "blkh_state_block_dims", ctypes.c_int, [_vp, ctypes.c_char_p] + [ctypes.POINTER(ctypes.c_int)] * 4)
 
 _sig(_h, "blkh_davidson", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp,
                                         ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.POINTER(ctypes.c_int)])
@@ -335,6 +336,18 @@ def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=
     if n < 0:
         raise _err()
     return e, sw[: nc * n].reshape(n, nc)
+
+
+def state_block_dims(ints, path):
+    """Kept states and sectors of every block of a saved sweep state:
+    (fwd_states, bwd_states, fwd_sectors, bwd_sectors), lists over the
+    orbitals (fwd[l]: orbitals 0..l, bwd[l]: orbitals l..norb-1; 0 where
+    the state holds no block)."""
+    k = ints.norb
+    arrs = [(ctypes.c_int * k)() for _ in range(4)]
+    if host_lib.blkh_state_block_dims(ints._h, str(path).encode(), *arrs) != 0:
+        raise _err()
+    return tuple(list(a) for a in arrs)
 
 
 def build_check(ints, nsys, M, device=0):

@@ -162,6 +162,23 @@ void* blkh_superblock_from_state(void* ints, const char* path, int nelec, int tw
     return new SuperBlock(make_superblock(I, L, R, target));
   });
 }
+// Kept states (multiplets) and sectors of every block of a saved sweep
+// state: fwd_m[l], bwd_m[l] (0 where absent), fwd_sec[l], bwd_sec[l];
+// arrays of norb entries.
+int blkh_state_block_dims(void* ints, const char* path, int* fwd_m, int* bwd_m, int* fwd_sec, int* bwd_sec) {
+  return guarded([&]() {
+    const int k = static_cast<Integrals*>(ints)->norb;
+    SweepState st = load_sweep_state(path);
+    if ((int)st.fwd.size() != k) throw std::runtime_error("blkh_state_block_dims: another number of orbitals");
+    for (int l = 0; l < k; ++l) {
+      fwd_m[l] = st.fwd[l] ? (int)st.fwd[l]->si->total : 0;
+      bwd_m[l] = st.bwd[l] ? (int)st.bwd[l]->si->total : 0;
+      fwd_sec[l] = st.fwd[l] ? st.fwd[l]->si->nsec() : 0;
+      bwd_sec[l] = st.bwd[l] ? st.bwd[l]->si->nsec() : 0;
+    }
+    return 0;
+  });
+}
 void blkh_free_superblock(void* h) { delete static_cast<SuperBlock*>(h); }
 long blkh_wave_size(void* h) { return static_cast<SuperBlock*>(h)->wave->size; }
 double blkh_flops(void* h) { return sigma_flops(*static_cast<SuperBlock*>(h)); }

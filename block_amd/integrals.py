@@ -70,13 +70,27 @@ def synthetic(n, seed, d2h=False, nchol=72):
     return h1, h2, orbsym
 
 
-# Benchmark configurations (seeds are part of the definition)
+# Benchmark configurations (seeds are part of the definition).  ramp: the
+# sweep schedule (start sweep, M, Davidson tolerance on the squared residual
+# norm, density-matrix noise) that grows the blocks to M, ramp_sweeps sweeps
+# long; timed_tol: the Davidson tolerance of the timed sweeps that follow.
+_RAMP = {
+    500: [(0, 250, 1e-7, 1e-5), (2, 500, 1e-8, 1e-6)],
+    800: [(0, 200, 1e-6, 1e-4), (2, 400, 1e-7, 1e-5), (4, 800, 1e-8, 1e-6)],
+    1000: [(0, 250, 1e-6, 1e-4), (2, 500, 1e-7, 1e-4), (4, 1000, 1e-8, 1e-5)],
+    2000: [(0, 250, 1e-6, 1e-4), (2, 500, 1e-7, 1e-4), (4, 1000, 1e-8, 1e-5), (6, 2000, 1e-8, 1e-6)],
+}
 CONFIGS = {
-    "hubbard12": dict(kind="hubbard", norb=12, nelec=12, M=100),
-    "syn24": dict(kind="synthetic", norb=24, nelec=24, M=500, seed=2024, d2h=False),
-    "syn36d2h": dict(kind="synthetic", norb=36, nelec=36, M=1000, seed=2036, d2h=True),
-    "syn30": dict(kind="synthetic", norb=30, nelec=30, M=800, seed=2030, d2h=False, nroots=4),
-    "syn50": dict(kind="synthetic", norb=50, nelec=50, M=2000, seed=2050, d2h=False),
+    "hubbard12": dict(kind="hubbard", norb=12, nelec=12, M=100, ramp=[(0, 50, 1e-7, 1e-5), (2, 100, 1e-9, 1e-6)],
+                      ramp_sweeps=4, timed_tol=1e-12),
+    "syn24": dict(kind="synthetic", norb=24, nelec=24, M=500, seed=2024, d2h=False, ramp=_RAMP[500], ramp_sweeps=4,
+                  timed_tol=1e-12),
+    "syn36d2h": dict(kind="synthetic", norb=36, nelec=36, M=1000, seed=2036, d2h=True, ramp=_RAMP[1000],
+                     ramp_sweeps=6, timed_tol=1e-12),
+    "syn30": dict(kind="synthetic", norb=30, nelec=30, M=800, seed=2030, d2h=False, nroots=4, ramp=_RAMP[800],
+                  ramp_sweeps=6, timed_tol=1e-12),
+    "syn50": dict(kind="synthetic", norb=50, nelec=50, M=2000, seed=2050, d2h=False, ramp=_RAMP[2000],
+                  ramp_sweeps=8, timed_tol=1e-12),
 }
 
 

@@ -22,6 +22,9 @@
 namespace oracle {
 
 void oracle_multiply_h(const blk::SuperBlock& sb, const double* c, double* v, int nthreads = 1);
+// Sector products through the host library's blocked DGEMM instead of the
+// oracle's own loops (the timed CPU baseline; the parity tests keep the loops).
+void oracle_set_blocked_gemm(bool on);
 std::vector<double> oracle_diagonal(const blk::SuperBlock& sb);
 
 struct FciResult {

@@ -52,6 +52,14 @@ int orc_sigma_timed(void* sb, const double* c, double* v, int nthreads, double* 
   *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return rc;
 }
+// the same with the blocked host DGEMM for the sector products (the CPU
+// baseline of the benchmark)
+int orc_sigma_fast_timed(void* sb, const double* c, double* v, int nthreads, double* secs) {
+  oracle::oracle_set_blocked_gemm(true);
+  const int rc = orc_sigma_timed(sb, c, v, nthreads, secs);
+  oracle::oracle_set_blocked_gemm(false);
+  return rc;
+}
 int orc_diag(void* sb, double* out) {
   return guarded([&]() {
     auto d = oracle::oracle_diagonal(*static_cast<SuperBlock*>(sb));

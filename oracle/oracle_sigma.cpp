@@ -31,6 +31,17 @@ void mm_acc(int m, int n, int k, double alpha, const double* A, int lda, const d
   }
 }
 
+// the sector products: the oracle's own loops, or (fast, the timed CPU
+// baseline only) the host library's blocked DGEMM
+bool g_fast = false;
+inline void gemm_acc(int m, int n, int k, double alpha, const double* A, int lda, const double* B, int ldb, double* C,
+                     int ldc) {
+  if (g_fast)
+    dgemm(false, false, m, n, k, alpha, A, lda, B, ldb, 1.0, C, ldc);
+  else
+    mm_acc(m, n, k, alpha, A, lda, B, ldb, C, ldc);
+}
+
 // v += scale * [a(L) (x) b(R)]^0 c   -- operatorfunctions.C:485-537
 void tensor_multiply(const SuperBlock& sb, const OpView& a, const OpView& b, double scale, const double* c,
                      double* v) {
@@ -50,7 +61,7 @@ void tensor_multiply(const SuperBlock& sb, const OpView& a, const OpView& b, dou
         for (int i = 0; i < dl; ++i)
           for (int j = 0; j < dlp; ++j) abuf[(size_t)i * dlp + j] = vel(a, ia, i, j, dl, dlp);
         m.assign((size_t)dl * drp, 0.0);
-        mm_acc(dl, drp, dlp, a.scale(lQ, lQp), abuf.data(), dlp, c + W.blocks[ic].off, drp, m.data(), drp);
+        gemm_acc(dl, drp, dlp, a.scale(lQ, lQp), abuf.data(), dlp, c + W.blocks[ic].off, drp, m.data(), drp);
         for (int rQ = 0; rQ < rs.nsec(); ++rQ) {
           int iv = W.find(lQ, rQ);
           int ib = b.find(rQ, rQp);
@@ -64,7 +75,7 @@ void tensor_multiply(const SuperBlock& sb, const OpView& a, const OpView& b, dou
           bt.resize((size_t)drp * dr);
           for (int j = 0; j < drp; ++j)
             for (int r = 0; r < dr; ++r) bt[(size_t)j * dr + r] = vel(b, ib, r, j, dr, drp);
-          mm_acc(dl, dr, drp, f, m.data(), drp, bt.data(), dr, v + W.blocks[iv].off, dr);
+          gemm_acc(dl, dr, drp, f, m.data(), drp, bt.data(), dr, v + W.blocks[iv].off, dr);
         }
       }
 }
@@ -113,6 +124,8 @@ std::vector<Pair> enumerate(const SuperBlock& sb) {
   return P;
 }
 }  // namespace
+
+void oracle_set_blocked_gemm(bool on) { g_fast = on; }
 
 void oracle_multiply_h(const SuperBlock& sb, const double* c, double* v, int nthreads) {
   auto P = enumerate(sb);
