@@ -76,6 +76,7 @@ _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
       ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp])
 _sig(_h, "blkh_build_check", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _lp])
+_sig(_h, "blkh_build_check_cpu", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _lp])
 _sig(_h, "blkh_state_block_dims", ctypes.c_int, [_vp, ctypes.c_char_p] + [ctypes.POINTER(ctypes.c_int)] * 4)
 
 _sig(_h, "blkh_davidson", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp,
@@ -348,6 +349,17 @@ def state_block_dims(ints, path):
     if host_lib.blkh_state_block_dims(ints._h, str(path).encode(), *arrs) != 0:
         raise _err()
     return tuple(list(a) for a in arrs)
+
+
+def build_check_cpu(ints, nsys, M):
+    """The recorded composite-operator build (the items and groups the device
+    executes) run on the host against the host build: (largest absolute
+    element difference, operators compared)."""
+    worst = ctypes.c_double()
+    n = ctypes.c_long()
+    if host_lib.blkh_build_check_cpu(ints._h, nsys, M, ctypes.byref(worst), ctypes.byref(n)) != 0:
+        raise _err()
+    return worst.value, n.value
 
 
 def build_check(ints, nsys, M, device=0):

@@ -36,6 +36,101 @@ void download_ops(const std::vector<SparseOp*>& ops) {
 
 const BuildTimes& gpu_build_times() { return g_build; }
 
+BuildPlan plan_build(const std::vector<BuildItem>& items, const std::function<const double*(const SparseOp*)>& addr) {
+  // groups: items with the same level and destination rectangle, in the
+  // order recorded (one recording thread per destination operator)
+  std::vector<int> order(items.size());
+  for (size_t i = 0; i < items.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const BuildItem &x = items[a], &y = items[b];
+    if (x.level != y.level) return x.level < y.level;
+    if (x.dst != y.dst) return std::less<const SparseOp*>()(x.dst, y.dst);
+    return x.dst_off < y.dst_off;
+  });
+  BuildPlan bp;
+  for (size_t k = 0; k < order.size();) {
+    const BuildItem& h = items[order[k]];
+    if (h.level < 0 || h.level > 1) throw std::runtime_error("plan_build: unknown item level");
+    auto& groups = bp.groups[h.level];
+    auto& srcs = bp.srcs[h.level];
+    bsg_build_group_t gr;
+    gr.dst = const_cast<double*>(addr(h.dst)) + h.dst_off;
+    gr.ld = h.dst_ld;
+    gr.rows = h.rows;
+    gr.cols = h.cols;
+    gr.first = (int32_t)srcs.size();
+    gr.pad = 0;
+    size_t e = k;
+    for (; e < order.size(); ++e) {
+      const BuildItem& it = items[order[e]];
+      if (it.level != h.level || it.dst != h.dst || it.dst_off != h.dst_off) break;
+      if (it.rows != h.rows || it.cols != h.cols || it.dst_ld != h.dst_ld)
+        throw std::runtime_error("plan_build: overlapping destination rectangles");
+      bsg_build_src_t s;
+      s.src = addr(it.src) + it.src_off;
+      s.ld = it.src_ld;
+      s.trans = it.trans;
+      s.alpha = it.alpha;
+      s.beta = it.beta;
+      srcs.push_back(s);
+    }
+    gr.count = (int32_t)(e - k);
+    bp.bytes += 8.0 * h.rows * h.cols * (gr.count + 2);
+    groups.push_back(gr);
+    k = e;
+  }
+  for (int l = 0; l < 2; ++l)
+    if (bp.groups[l].size() > 0x7fffffff || bp.srcs[l].size() > 0x7fffffff)
+      throw std::runtime_error("plan_build: too many items");
+  return bp;
+}
+
+void run_build_host(const BuildPlan& bp) {
+  for (int level = 0; level < 2; ++level) {
+    const auto& groups = bp.groups[level];
+    const auto& srcs = bp.srcs[level];
+#pragma omp parallel for schedule(dynamic)
+    for (long g = 0; g < (long)groups.size(); ++g) {
+      const bsg_build_group_t gr = groups[g];
+      for (int r = 0; r < gr.rows; ++r)
+        for (int c = 0; c < gr.cols; ++c) {
+          double s = 0.0;
+          for (int j = 0; j < gr.count; ++j) {
+            const bsg_build_src_t& sj = srcs[gr.first + j];
+            const double a = sj.trans ? sj.src[(int64_t)c * sj.ld + r] : sj.src[(int64_t)r * sj.ld + c];
+            s += (sj.alpha * a) * sj.beta;
+          }
+          gr.dst[(int64_t)r * gr.ld + c] += s;
+        }
+    }
+  }
+}
+
+void execute_build_host(const BuildSession& session, const std::vector<SparseOp*>& targets) {
+  const std::vector<BuildItem> items = session.items();
+  const std::vector<OpPtr> written = session.written();
+  std::unordered_map<const SparseOp*, double*> where;
+  std::vector<std::vector<double>> scratch;
+  for (SparseOp* o : targets) {
+    if (!o || where.count(o)) continue;
+    o->data.assign(o->size(), 0.0);
+    o->host_valid = true;
+    where[o] = o->data.data();
+  }
+  for (const OpPtr& o : written) {
+    if (where.count(o.get())) continue;
+    scratch.emplace_back(o->size(), 0.0);
+    where[o.get()] = scratch.back().data();
+  }
+  auto addr = [&](const SparseOp* o) -> const double* {
+    auto f = where.find(o);
+    if (f != where.end()) return f->second;
+    if (!o->host_valid) throw std::runtime_error("execute_build_host: source operator not in host memory");
+    return o->data.data();
+  };
+  run_build_host(plan_build(items, addr));
+}
+
 void set_materialiser_context(GpuContext* g) {
   std::lock_guard<std::mutex> lk(g_mat_mu);
   g_mat_ctx = g;
@@ -105,56 +200,12 @@ void execute_build(GpuContext& g, const BuildSession& session, const std::vector
     if (o->dev) return o->dev;
     throw std::runtime_error("execute_build: operator without an address");
   };
-  // groups: items with the same level and destination rectangle, in the
-  // order recorded (one recording thread per destination operator)
-  std::vector<int> order(items.size());
-  for (size_t i = 0; i < items.size(); ++i) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    const BuildItem &x = items[a], &y = items[b];
-    if (x.level != y.level) return x.level < y.level;
-    if (x.dst != y.dst) return std::less<const SparseOp*>()(x.dst, y.dst);
-    return x.dst_off < y.dst_off;
-  });
-  double bytes = 0.0;
-  for (int level = 0; level < 2; ++level) {
-    std::vector<bsg_build_group_t> groups;
-    std::vector<bsg_build_src_t> srcs;
-    for (size_t k = 0; k < order.size();) {
-      const BuildItem& h = items[order[k]];
-      if (h.level != level) {
-        ++k;
-        continue;
-      }
-      bsg_build_group_t gr;
-      gr.dst = const_cast<double*>(addr(h.dst)) + h.dst_off;
-      gr.ld = h.dst_ld;
-      gr.rows = h.rows;
-      gr.cols = h.cols;
-      gr.first = (int32_t)srcs.size();
-      gr.pad = 0;
-      size_t e = k;
-      for (; e < order.size(); ++e) {
-        const BuildItem& it = items[order[e]];
-        if (it.level != level || it.dst != h.dst || it.dst_off != h.dst_off) break;
-        if (it.rows != h.rows || it.cols != h.cols || it.dst_ld != h.dst_ld)
-          throw std::runtime_error("execute_build: overlapping destination rectangles");
-        bsg_build_src_t s;
-        s.src = addr(it.src) + it.src_off;
-        s.ld = it.src_ld;
-        s.trans = it.trans;
-        s.alpha = it.alpha;
-        s.beta = it.beta;
-        srcs.push_back(s);
-      }
-      gr.count = (int32_t)(e - k);
-      bytes += 8.0 * h.rows * h.cols * (gr.count + 2);
-      groups.push_back(gr);
-      k = e;
-    }
-    if (groups.size() > 0x7fffffff || srcs.size() > 0x7fffffff) throw std::runtime_error("execute_build: too many items");
-    bsg_check(bsg_build_apply(g.ctx, groups.data(), (int32_t)groups.size(), srcs.data(), (int32_t)srcs.size()),
+  const BuildPlan bp = plan_build(items, addr);
+  for (int level = 0; level < 2; ++level)
+    bsg_check(bsg_build_apply(g.ctx, bp.groups[level].data(), (int32_t)bp.groups[level].size(), bp.srcs[level].data(),
+                              (int32_t)bp.srcs[level].size()),
               "composite build");
-  }
+  const double bytes = bp.bytes;
   for (auto& x : tlist) {
     x.first->data = std::vector<double>();
     x.first->host_valid = false;

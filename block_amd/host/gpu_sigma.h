@@ -178,6 +178,19 @@ class GpuBlockBuilder : public BlockBuilder {
 // other written ones in scratch), sources are read from their device copies
 // (host-only sources are uploaded first).
 void execute_build(GpuContext& g, const BuildSession& session, const std::vector<SparseOp*>& targets);
+// The grouped form bsg_build_apply takes (level 0 then level 1), with the
+// operators' addresses from addr; run_build_host is the same arithmetic on
+// the host (k_build's loop), execute_build_host the whole recorded build in
+// host memory (the targets become host operators).  They check the
+// recording and the grouping on the CPU.
+struct BuildPlan {
+  std::vector<bsg_build_group_t> groups[2];
+  std::vector<bsg_build_src_t> srcs[2];
+  double bytes = 0.0;  // algorithmic bytes (sources read, destinations read and written)
+};
+BuildPlan plan_build(const std::vector<BuildItem>& items, const std::function<const double*(const SparseOp*)>& addr);
+void run_build_host(const BuildPlan& plan);
+void execute_build_host(const BuildSession& session, const std::vector<SparseOp*>& targets);
 // Make g the context that downloads device-only operators for host readers
 // (materialise_host); null: none.
 void set_materialiser_context(GpuContext* g);

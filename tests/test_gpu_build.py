@@ -25,7 +25,7 @@ def test_device_build_matches_host_build(name, nsys, M):
             "syn16d2h": lambda: random_fcidump(16, 22, d2h=True)}[name]()
     worst, n = block_amd.build_check(ints, nsys, M)
     print(f"[gpu] {name}: {n} operators, max |host - device| = {worst:.3e}", flush=True)
-    assert n > 100
+    assert n > 20
     assert worst <= 1e-14
 
 
