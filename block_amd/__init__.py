@@ -68,6 +68,7 @@ _sig(_h, "blkh_gpu_sigma_phase_flops", ctypes.c_int, [_vp, _dp, _dp])
 _sig(_h, "blkh_gpu_sigma_arena_bytes", ctypes.c_long, [_vp])
 _sig(_h, "blkh_gpu_sigma_timed", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _lp])
 _sig(_h, "blkh_gpu_sigma_set_batch", ctypes.c_int, [_vp, ctypes.c_int])
+_sig(_h, "blkh_gpu_sigma_set_graph", ctypes.c_int, [_vp, ctypes.c_int])
 _sig(_h, "blkh_gpu_sigma_apply_n", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int])
 _sig(_h, "blkh_gpu_sigma_set_precision", ctypes.c_int, [_vp, ctypes.c_int])
 _sig(_h, "blkh_gpu_sigma_raw", ctypes.c_int, [_vp] + [ctypes.POINTER(_vp)] * 5)
@@ -261,6 +262,12 @@ class multiply_h:
     def set_batch(self, nroot):
         """Roots per launch (bsg_plan_set_batch)."""
         if host_lib.blkh_gpu_sigma_set_batch(self._h, int(nroot)) != 0:
+            raise _err()
+
+    def set_graph(self, on):
+        """Applications as one captured HIP graph launch each (bsg_plan_capture
+        / bsg_plan_replay) instead of their kernel launches."""
+        if host_lib.blkh_gpu_sigma_set_graph(self._h, int(bool(on))) != 0:
             raise _err()
 
     def raw(self):

@@ -1235,6 +1235,13 @@ __global__ void k_scal_rsqrt_dev(const double* __restrict__ s, double floor, dou
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     x[i] *= a;
 }
+// y -= (num[0] / den[0]) x (Olsen's step with both dot products on the device)
+__global__ void k_axpy_ratio_dev(const double* __restrict__ num, const double* __restrict__ den,
+                                 const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+  const double a = -(num[0] / den[0]);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
 __global__ void k_axpy_neg_dev(const double* __restrict__ s, const double* __restrict__ x, double* __restrict__ y,
                                int64_t n) {
   const double a = -s[0];
@@ -1268,6 +1275,8 @@ __global__ __launch_bounds__(256) void k_copy_chunks(const bsg_copy_seg_t* __res
 __global__ __launch_bounds__(256) void k_build(const bsg_build_group_t* __restrict__ groups,
                                                const int64_t* __restrict__ estart, int ngroups,
                                                const bsg_build_src_t* __restrict__ srcs, int64_t nelem) {
+  // no fused multiply-adds: the host build rounds every product and sum
+#pragma clang fp contract(off)
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (e >= nelem) return;
   // last group whose start is <= e
@@ -1402,10 +1411,12 @@ struct bsg_context {
   // beside the grouped kernel; fork/join events tie it to `stream`
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
-  double* red = nullptr;  // RMAXB partials + 1 result
+  double* red = nullptr;  // RMAXB partials + 4 result slots
   double* gpart = nullptr;  // Gram partials (GBLK GMAX^2) + result (GRAM_MAX^2) + coefficients (GRAM_MAX^2)
   bsg_copy_seg_t* copy_list = nullptr;  // chunk list of bsg_copy_segments (grown on demand)
   size_t copy_cap = 0;
+  double* coef_pin = nullptr;    // page-locked staging of bsg_lincomb's coefficients (GRAM_MAX^2)
+  hipEvent_t coef_done = nullptr;  // its last copy to the device
 };
 
 struct PlanChunk {
@@ -1441,6 +1452,12 @@ struct bsg_plan {
   int64_t ab_elems = 0, c_elems = 0;
   SliceItem *d_ab_items = nullptr, *d_c_items = nullptr;
   __bf16 *d_abs = nullptr, *d_cs = nullptr;
+  // one captured application (bsg_plan_capture): its executable graph and
+  // what it was captured for
+  hipGraphExec_t graph = nullptr;
+  const double *g_arena = nullptr, *g_c = nullptr;
+  double* g_v = nullptr;
+  int32_t g_nvec = 0, g_precision = -1;
 };
 
 extern "C" {
@@ -1460,7 +1477,7 @@ int bsg_init(bsg_context** ctx, int device) {
     delete c;
     return fail(BSG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
-  e = hipMalloc(&c->red, sizeof(double) * (RMAXB + 1));
+  e = hipMalloc(&c->red, sizeof(double) * (RMAXB + 4));
   if (e != hipSuccess) {
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1473,10 +1490,14 @@ int bsg_init(bsg_context** ctx, int device) {
     return fail(BSG_ERR_NOMEM, "bsg_init: Gram scratch");
   }
   if (hipStreamCreate(&c->side) != hipSuccess || hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->coef_done, hipEventDisableTiming) != hipSuccess ||
+      hipHostMalloc((void**)&c->coef_pin, sizeof(double) * GRAM_MAX * GRAM_MAX, hipHostMallocDefault) != hipSuccess) {
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
+    if (c->coef_done) (void)hipEventDestroy(c->coef_done);
+    if (c->coef_pin) (void)hipHostFree(c->coef_pin);
     (void)hipFree(c->gpart);
     (void)hipFree(c->red);
     (void)hipStreamDestroy(c->stream);
@@ -1495,6 +1516,8 @@ int bsg_free(bsg_context* ctx) {
   if (ctx->copy_list) HIPCHK(hipFree(ctx->copy_list));
   HIPCHK(hipStreamSynchronize(ctx->side));
   HIPCHK(hipFree(ctx->gpart));
+  HIPCHK(hipHostFree(ctx->coef_pin));
+  HIPCHK(hipEventDestroy(ctx->coef_done));
   HIPCHK(hipEventDestroy(ctx->fork));
   HIPCHK(hipEventDestroy(ctx->join));
   HIPCHK(hipStreamDestroy(ctx->side));
@@ -1826,6 +1849,7 @@ int bsg_plan_destroy(bsg_plan* p) {
   (void)hipFree(p->d_c_items);
   (void)hipFree(p->d_abs);
   (void)hipFree(p->d_cs);
+  if (p->graph) (void)hipGraphExecDestroy(p->graph);
   delete p;
   return BSG_OK;
 }
@@ -1862,6 +1886,10 @@ int bsg_plan_set_batch(bsg_context* ctx, bsg_plan* p, int32_t nroot) {
   if (nroot == p->batch) return BSG_OK;
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipStreamSynchronize(ctx->stream));  // no queued work reads the old buffers
+  if (p->graph) {  // it reads the buffers replaced below
+    HIPCHK(hipGraphExecDestroy(p->graph));
+    p->graph = nullptr;
+  }
   if (p->part_doubles > 0) {
     double* np = nullptr;
     if (hipMalloc(&np, sizeof(double) * p->part_doubles * nroot) != hipSuccess)
@@ -2022,6 +2050,54 @@ int bsg_sigma_apply(bsg_context* ctx, bsg_plan* p, const double* arena, const do
   return sigma_apply_with(ctx, p, arena, c, v, nvec, precision, p->part, p->batch);
 }
 
+int bsg_plan_capture(bsg_context* ctx, bsg_plan* p, const double* arena, const double* c, double* v, int32_t nvec,
+                     int32_t precision) {
+  if (!ctx || !p || !arena || !c || !v || nvec <= 0) return fail(BSG_ERR_ARG, "bsg_plan_capture: bad args");
+  if (p->ctx != ctx) return fail(BSG_ERR_ARG, "bsg_plan_capture: plan belongs to another context");
+  int rc = check_precision(p, arena, precision);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (p->graph) {
+    HIPCHK(hipGraphExecDestroy(p->graph));
+    p->graph = nullptr;
+    p->g_nvec = 0;
+  }
+  // the launches of one application on the context stream (the side stream
+  // joins through the fork/join events and is captured with it); no memset
+  // or allocation is recorded
+  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  rc = sigma_apply_with(ctx, p, arena, c, v, nvec, precision, p->part, p->batch);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(BSG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  hipGraphExec_t x = nullptr;
+  const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e2 != hipSuccess) return fail(BSG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+  p->graph = x;
+  p->g_arena = arena;
+  p->g_c = c;
+  p->g_v = v;
+  p->g_nvec = nvec;
+  p->g_precision = precision;
+  return BSG_OK;
+}
+
+int bsg_plan_replay(bsg_context* ctx, bsg_plan* p) {
+  if (!ctx || !p || !p->graph) return fail(BSG_ERR_ARG, "bsg_plan_replay: no captured application");
+  if (p->ctx != ctx) return fail(BSG_ERR_ARG, "bsg_plan_replay: plan belongs to another context");
+  // the slices the split kernels read must still be those of the captured arena
+  int rc = check_precision(p, p->g_arena, p->g_precision);
+  if (rc) return rc;
+  HIPCHK(hipGraphLaunch(p->graph, ctx->stream));
+  return BSG_OK;
+}
+
 int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const double* c, double* v, int32_t nvec,
                       int32_t reps, int32_t precision, double* wall_ms, double* kernel_ms, long* launches) {
   if (!ctx || !p || !arena || !c || !v || nvec <= 0 || reps <= 0 || !wall_ms || !kernel_ms || !launches)
@@ -2177,7 +2253,12 @@ int bsg_lincomb(bsg_context* ctx, const double* X, int32_t m, int64_t n, int64_t
   if (!ctx || !X || !coef || !Y || m <= 0 || m > GRAM_MAX || mo <= 0 || mo > GRAM_MAX || n < 0 || ld < n || ldy < n)
     return fail(BSG_ERR_ARG, "bsg_lincomb: bad args");
   double* cd = ctx->gpart + (size_t)GBLK * GMAX * GMAX + (size_t)GRAM_MAX * GRAM_MAX;
-  HIPCHK(hipMemcpyAsync(cd, coef, sizeof(double) * m * mo, hipMemcpyHostToDevice, ctx->stream));
+  // coefficients through page-locked staging, so the call returns without
+  // waiting for the kernels (the staging is reused once its last copy is done)
+  HIPCHK(hipEventSynchronize(ctx->coef_done));
+  std::memcpy(ctx->coef_pin, coef, sizeof(double) * m * mo);
+  HIPCHK(hipMemcpyAsync(cd, ctx->coef_pin, sizeof(double) * m * mo, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipEventRecord(ctx->coef_done, ctx->stream));
   // GMAX x GMAX coefficient tiles; the first row tile of each output tile
   // overwrites Y, the others add in order
   for (int j0 = 0; j0 < mo; j0 += GMAX)
@@ -2186,7 +2267,6 @@ int bsg_lincomb(bsg_context* ctx, const double* X, int32_t m, int64_t n, int64_t
                          std::min(GMAX, m - i0), n, ld, cd + (int64_t)i0 * mo + j0, mo, std::min(GMAX, mo - j0),
                          Y + (int64_t)j0 * ldy, ldy, i0 > 0 ? 1 : 0);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(ctx->stream));  // coef is a host buffer the caller may reuse
   return BSG_OK;
 }
 int bsg_copy(bsg_context* ctx, const double* x, double* y, int64_t n) {
@@ -2281,10 +2361,15 @@ int bsg_olsen_precondition(bsg_context* ctx, double* r, const double* c, const d
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, ctx->stream, c, work, n);
   int rc;
   if ((rc = bsg_precondition(ctx, work, d, e, shift, n))) return rc;
-  double num = 0.0, den = 0.0;
-  if ((rc = bsg_dot(ctx, work, r, n, &num))) return rc;
-  if ((rc = bsg_dot(ctx, c, work, n, &den))) return rc;
-  if ((rc = bsg_axpy(ctx, -num / den, c, r, n))) return rc;
+  // the two dot products stay in device memory: no host round trip
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(RMAXB, (n + RB - 1) / RB));
+  double* s = ctx->red + RMAXB;
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(RB), 0, ctx->stream, work, r, n, ctx->red);
+  hipLaunchKernelGGL(k_sum_partial, dim3(1), dim3(RB), 0, ctx->stream, ctx->red, nb, s + 1);
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(RB), 0, ctx->stream, c, work, n, ctx->red);
+  hipLaunchKernelGGL(k_sum_partial, dim3(1), dim3(RB), 0, ctx->stream, ctx->red, nb, s + 2);
+  hipLaunchKernelGGL(k_axpy_ratio_dev, dim3(grid_for(n)), dim3(256), 0, ctx->stream, s + 1, s + 2, c, r, n);
+  HIPCHK(hipGetLastError());
   return bsg_precondition(ctx, r, d, e, shift, n);
 }
 

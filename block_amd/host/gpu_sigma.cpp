@@ -277,7 +277,32 @@ void GpuSigma::set_precision(int mode) {
 }
 
 void GpuSigma::apply_device(const double* dc, double* dv, int nvec) {
-  bsg_check(bsg_sigma_apply(g_.ctx, plan_, d_arena_, dc, dv, nvec, precision_), "sigma");
+  if (!use_graph_ || nvec > max_vec_) {
+    bsg_check(bsg_sigma_apply(g_.ctx, plan_, d_arena_, dc, dv, nvec, precision_), "sigma");
+    return;
+  }
+  // one graph launch on the handle's own c and v buffers (captured once per
+  // precision and vector count): c copied in, v cleared outside the graph,
+  // the result added to dv
+  const int64_t n = (int64_t)wave_size_ * nvec;
+  if (graph_precision_ != precision_ || graph_nvec_ != nvec) {
+    bsg_check(bsg_plan_capture(g_.ctx, plan_, d_arena_, d_c_, d_v_, nvec, precision_), "sigma capture");
+    graph_precision_ = precision_;
+    graph_nvec_ = nvec;
+  }
+  const bool own = dc == d_c_ && dv == d_v_;  // already the captured buffers (v += H c in place)
+  if (!own) {
+    bsg_check(bsg_copy(g_.ctx, dc, d_c_, n), "sigma input");
+    bsg_check(bsg_memset_zero(g_.ctx, d_v_, sizeof(double) * n), "sigma clear");
+  }
+  bsg_check(bsg_plan_replay(g_.ctx, plan_), "sigma replay");
+  if (!own) bsg_check(bsg_axpy(g_.ctx, 1.0, d_v_, dv, n), "sigma output");
+}
+
+void GpuSigma::set_graph(bool on) {
+  use_graph_ = on;
+  graph_precision_ = -1;
+  graph_nvec_ = 0;
 }
 
 void GpuSigma::set_batch(int nroot) {
@@ -285,6 +310,8 @@ void GpuSigma::set_batch(int nroot) {
   if (nroot == batch_) return;
   bsg_check(bsg_plan_set_batch(g_.ctx, plan_, nroot), "batch");
   batch_ = nroot;
+  graph_precision_ = -1;  // the plan dropped its capture
+  graph_nvec_ = 0;
 }
 
 void GpuSigma::apply(const double* c_host, double* v_host, int nvec) {

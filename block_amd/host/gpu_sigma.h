@@ -102,6 +102,10 @@ class GpuSigma {
   int max_vec() const { return max_vec_; }
   // device vectors: v += H c
   void apply_device(const double* dc, double* dv, int nvec = 1);
+  // apply_device through a captured HIP graph (one launch per application;
+  // the input and output pass through the handle's own buffers)
+  void set_graph(bool on);
+  bool graph() const { return use_graph_; }
   std::vector<double> diagonal();
   bsg_context* ctx() const { return g_.ctx; }
   double* dev_c() const { return d_c_; }
@@ -124,6 +128,8 @@ class GpuSigma {
   bsg_plan* plan_ = nullptr;
   int precision_ = BSG_FP64;
   bool split_ready_ = false;
+  bool use_graph_ = false;
+  int graph_precision_ = -1, graph_nvec_ = 0;
   std::vector<bsg_diag_t> diag_terms_;
 };
 

@@ -144,6 +144,18 @@ int bsg_plan_set_batch(bsg_context* ctx, bsg_plan* plan, int32_t nroot);
 int bsg_sigma_apply(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t nvec,
                     int32_t precision);
 
+/* One application captured as a HIP graph (no counterpart in the reference):
+ * bsg_plan_capture records the launches of bsg_sigma_apply for these exact
+ * arena, c, v, nvec and precision (both streams, no memset, no allocation)
+ * into the plan, replacing any earlier capture; bsg_plan_replay launches it
+ * on the context stream (v += H c again; the caller clears v beforehand if it
+ * wants v = H c).  The caller keeps the buffers in place between replays; a
+ * capture is dropped by bsg_plan_set_batch.  One graph launch instead of the
+ * application's kernel launches and stream events. */
+int bsg_plan_capture(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t nvec,
+                     int32_t precision);
+int bsg_plan_replay(bsg_context* ctx, bsg_plan* plan);
+
 /* Doubles of partial-tile scratch one application of the plan uses (V blocks
  * whose pair list is cut into several work units; 0 if none). */
 int64_t bsg_plan_partial_doubles(const bsg_plan* plan);
@@ -212,13 +224,16 @@ int bsg_build_apply(bsg_context* ctx, const bsg_build_group_t* groups_host, int3
 int bsg_gram(bsg_context* ctx, const double* X, const double* Y, int32_t m, int64_t n, int64_t ld, double* out_host);
 /* Ritz rotation (linear.C:279-294): Y_j = sum_i coef_host[i*mo+j] X_i for
  * j < mo, m and mo <= 128; X, Y device vector sets (strides ld, ldy; must not
- * overlap). */
+ * overlap).  The coefficients are copied to page-locked staging at once, so
+ * coef_host may be reused on return; the kernels finish in stream order. */
 int bsg_lincomb(bsg_context* ctx, const double* X, int32_t m, int64_t n, int64_t ld, const double* coef_host,
                 int32_t mo, double* Y, int64_t ldy);
 /* x[i] /= (e - diag[i] + shift) where |e - diag[i]| > 1e-12 (linear.C:27-47) */
 int bsg_precondition(bsg_context* ctx, double* x, const double* diag, double e, double shift, int64_t n);
 /* Olsen step (linear.C:50-60): r -= (<C~,r>/<C,C~>) C with C~ = precond(C),
- * then r = precond(r).  work: device scratch of n doubles. */
+ * then r = precond(r).  work: device scratch of n doubles.  The two dot
+ * products stay in device memory (no host round trip); returns without
+ * waiting for the kernels. */
 int bsg_olsen_precondition(bsg_context* ctx, double* r, const double* c, const double* diag, double e, double shift,
                            double* work, int64_t n);
 /* Gram-Schmidt of the new Davidson vector (linear.C:358-366 and the

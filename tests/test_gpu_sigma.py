@@ -159,3 +159,28 @@ def test_batched_roots(name, make):
     assert np.linalg.norm(got[2] - one[2]) == 0.0
     h.set_batch(4)
     assert np.linalg.norm(h(c)[2] - ref) <= TOL * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("name,make", [c for c in _cases() if c[0] in ("exact7d2h", "syn16_M80", "syn24_M500_dmrg")],
+                         ids=["exact7d2h", "syn16_M80", "syn24_M500_dmrg"])
+def test_graph_replays_match_eager(name, make):
+    """Applications through a captured HIP graph: every replay, with eager
+    applications and host round trips between them, FP64 and split
+    precision, is bitwise the eagerly launched sigma."""
+    print(f"[gpu] graph replays {name}", flush=True)
+    sb = make()
+    h = block_amd.multiply_h(sb)
+    c = rng_vec(sb.wave_size, 31)
+    x = rng_vec(sb.wave_size, 32)
+    for prec in (block_amd.FP64, block_amd.SPLIT_BF16):
+        h.precision = prec
+        h.set_graph(False)
+        want, want_x = h(c), h(x)
+        h.set_graph(True)
+        for rep in range(4):
+            assert np.array_equal(h(c), want), (prec, rep)
+            h.set_graph(False)
+            assert np.array_equal(h(x), want_x)  # eager work between replays
+            h.set_graph(True)
+            assert np.array_equal(h(x), want_x)  # a replay after a new capture
+    h.precision = block_amd.FP64
