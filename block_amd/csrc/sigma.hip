@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <cstdio>
 #include <cstring>
@@ -1403,6 +1404,37 @@ int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(204
 
 }  // namespace
 
+// ---------------------------------------------------------------- static scratch
+// Partial tiles of bsg_sigma_apply_noalloc: a file-scope device array, one
+// copy per GPU the library runs on, held until the process ends.  Host code
+// never passes the array's name to a kernel (on the host it is a stand-in,
+// not a device address): the address comes from hipGetSymbolAddress, once
+// per device with that device current, and is kept per device.
+__device__ __align__(256) double scratch[BSG_NOALLOC_SCRATCH_DOUBLES];
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+// this device's scratch address (device current); nullptr with g_err set on failure
+double* scratch_address(int device) {
+  if (device < 0 || device >= MAX_DEVICES) {
+    g_err = "static scratch: device index out of range";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      g_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
 // ---------------------------------------------------------------- context and plan
 struct bsg_context {
   int device = 0;
@@ -1417,6 +1449,7 @@ struct bsg_context {
   size_t copy_cap = 0;
   double* coef_pin = nullptr;    // page-locked staging of bsg_lincomb's coefficients (GRAM_MAX^2)
   hipEvent_t coef_done = nullptr;  // its last copy to the device
+  double* scratch = nullptr;  // this device's copy of the static scratch (bsg_sigma_apply_noalloc)
 };
 
 struct PlanChunk {
@@ -1488,6 +1521,15 @@ int bsg_init(bsg_context** ctx, int device) {
     (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(BSG_ERR_NOMEM, "bsg_init: Gram scratch");
+  }
+  c->scratch = scratch_address(device);
+  if (!c->scratch) {
+    const std::string m = g_err;
+    (void)hipFree(c->gpart);
+    (void)hipFree(c->red);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(BSG_ERR_HIP, "bsg_init: " + m);
   }
   if (hipStreamCreate(&c->side) != hipSuccess || hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
@@ -2096,6 +2138,21 @@ int bsg_plan_replay(bsg_context* ctx, bsg_plan* p) {
   if (rc) return rc;
   HIPCHK(hipGraphLaunch(p->graph, ctx->stream));
   return BSG_OK;
+}
+
+int bsg_sigma_apply_noalloc(bsg_context* ctx, bsg_plan* p, const double* arena, const double* c, double* v,
+                            int32_t nvec, int32_t precision) {
+  if (!ctx || !p || !arena || !c || !v || nvec <= 0) return fail(BSG_ERR_ARG, "bsg_sigma_apply_noalloc: bad args");
+  if (p->ctx != ctx) return fail(BSG_ERR_ARG, "bsg_sigma_apply_noalloc: plan belongs to another context");
+  if (!ctx->scratch) return fail(BSG_ERR_ARG, "bsg_sigma_apply_noalloc: no scratch address for this device");
+  if (p->part_doubles > BSG_NOALLOC_SCRATCH_DOUBLES)
+    return fail(BSG_ERR_ARG, "bsg_sigma_apply_noalloc: the plan needs " + std::to_string(p->part_doubles) +
+                                 " doubles of partial tiles, more than the " +
+                                 std::to_string((long long)BSG_NOALLOC_SCRATCH_DOUBLES) + " of the static scratch");
+  // as many roots per launch as the scratch holds partial tiles for
+  const int group = p->part_doubles > 0 ? (int)std::min<int64_t>(p->batch, BSG_NOALLOC_SCRATCH_DOUBLES / p->part_doubles)
+                                        : p->batch;
+  return sigma_apply_with(ctx, p, arena, c, v, nvec, precision, ctx->scratch, group);
 }
 
 int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const double* c, double* v, int32_t nvec,

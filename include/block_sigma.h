@@ -160,6 +160,21 @@ int bsg_plan_replay(bsg_context* ctx, bsg_plan* plan);
  * whose pair list is cut into several work units; 0 if none). */
 int64_t bsg_plan_partial_doubles(const bsg_plan* plan);
 
+/* bsg_sigma_apply without any allocation, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid it).  No counterpart in the reference.  The partial tiles live
+ * in a static device array of BSG_NOALLOC_SCRATCH_DOUBLES doubles (64 MiB),
+ * one per GPU, held until the process ends; a plan whose
+ * bsg_plan_partial_doubles exceeds it is rejected with BSG_ERR_ARG.  Results
+ * are bitwise those of bsg_sigma_apply.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other contexts' streams, other
+ * threads) race and must be serialised by the caller.  The plan itself (and,
+ * for BSG_SPLIT_BF16, bsg_plan_split_arena's slice buffers) is set up
+ * beforehand by the allocating calls. */
+#define BSG_NOALLOC_SCRATCH_DOUBLES (8LL * 1024 * 1024)
+int bsg_sigma_apply_noalloc(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v,
+                            int32_t nvec, int32_t precision);
+
 /* Timing helper: runs `reps` applications of the sigma to nvec vectors
  * (c, v: nvec vectors back to back, roots batched as bsg_sigma_apply does)
  * on the context stream with HIP events around every chunk launch.

@@ -25,6 +25,7 @@ for name, args in [
     ("bsg_lincomb", [vp, vp, _i32, _i64, _i64, vp, _i32, vp, _i64]),
     ("bsg_gs_orthonormalise", [vp, vp, vp, _i32, _i64, _i64, _i32]),
     ("bsg_sigma_apply", [vp, vp, vp, vp, vp, _i32, _i32]),
+    ("bsg_sigma_apply_noalloc", [vp, vp, vp, vp, vp, _i32, _i32]),
 ]:
     f = getattr(lib, name)
     f.restype = ctypes.c_int
