@@ -1,6 +1,7 @@
 // State bookkeeping, block-sparse operator patterns, the composite-operator
 // tensor product and small dense kernels of the host library.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -122,32 +123,32 @@ struct Recorder {
 };
 }  // namespace
 struct BuildSession::Impl {
+  uint64_t id = 0;  // unique over the process (a later session may reuse this address)
   std::mutex mu;
   std::vector<std::unique_ptr<Recorder>> recs;
 };
 namespace {
+std::atomic<uint64_t> g_session_ids{0};
 // the session the calling thread records into, and its recorder there
+// (identified by the session's id, never by its address)
 thread_local BuildSession::Impl* tl_session = nullptr;
-thread_local BuildSession::Impl* tl_rec_session = nullptr;
+thread_local uint64_t tl_rec_id = 0;
 thread_local Recorder* tl_rec = nullptr;
 Recorder* recorder() {
   BuildSession::Impl* s = tl_session;
   if (!s) return nullptr;
-  if (tl_rec_session != s) {
+  if (tl_rec_id != s->id) {
     auto r = std::make_unique<Recorder>();
-    tl_rec = r.get();
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->recs.push_back(std::move(r));
-    tl_rec_session = s;
+    Recorder* raw = r.get();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->recs.push_back(std::move(r));
+    }
+    tl_rec = raw;
+    tl_rec_id = s->id;
   }
   return tl_rec;
 }
-// an OpenMP worker records into its parent's session while in scope
-struct SessionScope {
-  BuildSession::Impl* prev;
-  explicit SessionScope(BuildSession::Impl* s) : prev(tl_session) { tl_session = s; }
-  ~SessionScope() { tl_session = prev; }
-};
 void keep_written(Recorder* r, const SparseOp& o) {
   // operators are owned by shared pointers; keep the written ones alive so
   // no later allocation can reuse an address an item refers to
@@ -156,14 +157,11 @@ void keep_written(Recorder* r, const SparseOp& o) {
 }  // namespace
 BuildSession::BuildSession() : impl(new Impl) {
   if (tl_session) throw std::runtime_error("BuildSession: a session is already open on this thread");
+  impl->id = ++g_session_ids;
   tl_session = impl.get();
 }
 BuildSession::~BuildSession() {
   if (tl_session == impl.get()) tl_session = nullptr;
-  if (tl_rec_session == impl.get()) {
-    tl_rec_session = nullptr;
-    tl_rec = nullptr;
-  }
 }
 std::vector<BuildItem> BuildSession::items() const {
   std::vector<BuildItem> out;
