@@ -180,6 +180,68 @@ int blkh_state_block_dims(void* ints, const char* path, int* fwd_m, int* bwd_m, 
     return 0;
   });
 }
+// The compiled work list of a superblock with its provenance, for checking
+// the scalar factors against other conventions.  Returns the pair count
+// (call with max 0 to size the arrays).  pair_int[6 p ..]: term, l, l', r,
+// r', identity flags (bit 0: A, bit 1: B); pair_dbl[2 p ..]: alpha_a, alpha.
+// term_int[7 t ..]: rank of a, rank of b (2k), a adjoint, b adjoint, b
+// fermionic, delta N of a, delta N of b; term_dbl[t]: coefficient.
+// lq[3 i ..], rq[3 i ..]: (N, 2S, irrep) of the left and right sectors;
+// meta[0..4]: number of terms, of left and right sectors, 2S of the target.
+long blkh_worklist_provenance(void* sbp, long max, int* pair_int, double* pair_dbl, int* term_int, double* term_dbl,
+                              int* lq, int* rq, int* meta) {
+  try {
+    const SuperBlock& sb = *static_cast<SuperBlock*>(sbp);
+    WorkList wl = compile_worklist(sb);
+    const long n = (long)wl.pairs.size();
+    if (max <= 0) return n;
+    if (max < n) throw std::runtime_error("blkh_worklist_provenance: arrays too small");
+    for (long p = 0; p < n; ++p) {
+      const auto& pv = wl.prov[p];
+      const auto& pr = wl.pairs[p];
+      int* pi = pair_int + 6 * p;
+      pi[0] = pv.term;
+      pi[1] = pv.l;
+      pi[2] = pv.lp;
+      pi[3] = pv.r;
+      pi[4] = pv.rp;
+      pi[5] = (pr.a_off < 0 ? 1 : 0) | (pr.b_off < 0 ? 2 : 0);
+      pair_dbl[2 * p] = pr.alpha_a;
+      pair_dbl[2 * p + 1] = pr.alpha;
+    }
+    for (size_t t = 0; t < sb.terms.size(); ++t) {
+      const auto& tm = sb.terms[t];
+      int* ti = term_int + 7 * t;
+      ti[0] = tm.a.k2();
+      ti[1] = tm.b.k2();
+      ti[2] = tm.a.dag;
+      ti[3] = tm.b.dag;
+      ti[4] = tm.b.fermion();
+      ti[5] = tm.a.dn();
+      ti[6] = tm.b.dn();
+      term_dbl[t] = tm.coef;
+    }
+    const StateInfo &ls = *sb.wave->L, &rs = *sb.wave->R;
+    for (int i = 0; i < ls.nsec(); ++i) {
+      lq[3 * i] = ls.q[i].n;
+      lq[3 * i + 1] = ls.q[i].s2;
+      lq[3 * i + 2] = ls.q[i].irrep;
+    }
+    for (int i = 0; i < rs.nsec(); ++i) {
+      rq[3 * i] = rs.q[i].n;
+      rq[3 * i + 1] = rs.q[i].s2;
+      rq[3 * i + 2] = rs.q[i].irrep;
+    }
+    meta[0] = (int)sb.terms.size();
+    meta[1] = ls.nsec();
+    meta[2] = rs.nsec();
+    meta[3] = sb.wave->target.s2;
+    return n;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
 void blkh_free_superblock(void* h) { delete static_cast<SuperBlock*>(h); }
 long blkh_wave_size(void* h) { return static_cast<SuperBlock*>(h)->wave->size; }
 double blkh_flops(void* h) { return sigma_flops(*static_cast<SuperBlock*>(h)); }

@@ -136,6 +136,7 @@ WorkList compile_worklist(const SuperBlock& sb) {
     toff_b[it] = wl.arena_off.at(sb.terms[it].b.op);
   }
   std::vector<std::vector<bsg_pair_t>> per_v(W.blocks.size());
+  std::vector<std::vector<WorkList::Prov>> per_p(W.blocks.size());
 #pragma omp parallel for schedule(dynamic)
   for (long iv = 0; iv < (long)W.blocks.size(); ++iv) {
     const int l = W.blocks[iv].l, r = W.blocks[iv].r;
@@ -165,6 +166,7 @@ WorkList compile_worklist(const SuperBlock& sb) {
           p.alpha_a = aid ? 1.0 : t.a.scale(l, lp);
           p.alpha = f;
           per_v[iv].push_back(p);
+          per_p[iv].push_back({(int)it, l, lp, r, rp});
         }
       }
     }
@@ -181,6 +183,7 @@ WorkList compile_worklist(const SuperBlock& sb) {
     const auto& b = W.blocks[iv];
     wl.vb.push_back({b.off, b.rows, b.cols, (int32_t)wl.pairs.size(), (int32_t)per_v[iv].size()});
     wl.pairs.insert(wl.pairs.end(), per_v[iv].begin(), per_v[iv].end());
+    wl.prov.insert(wl.prov.end(), per_p[iv].begin(), per_p[iv].end());
   }
 
   // diagonal terms (SpinBlock::diagonalH)

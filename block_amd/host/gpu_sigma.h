@@ -70,6 +70,12 @@ struct WorkList {
   std::vector<bsg_diag_t> diag;
   std::unordered_map<const SparseOp*, int64_t> arena_off;
   int64_t arena_size = 0;
+  // per pair (same order): term index and the sectors l, l', r, r' it joins
+  // (V block (l, r), operator blocks a(l, l') and b(r, r'), C block (l', r'))
+  struct Prov {
+    int term, l, lp, r, rp;
+  };
+  std::vector<Prov> prov;
 };
 // Flatten the term list into GEMM pairs grouped by the V block they update.
 WorkList compile_worklist(const SuperBlock& sb);
