@@ -407,11 +407,25 @@ def main():
                 "kernel_ms_per_step_all": {"fused": round(kern_ms[0] / args.steps, 4),
                                            "k_reduce": round(kern_ms[1] / args.steps, 4),
                                            "side_stream": round(kern_ms[2] / args.steps, 4)}}
+    # the same applications as one captured HIP graph launch each
+    h.set_graph(True)
+    g_wall, _, _ = h.timed(c, args.warmup, args.steps)
+    h.set_graph(False)
+    graph = {"ms_per_step": round(g_wall / args.steps, 4), "speedup_vs_launches": round(wall_ms / g_wall, 3)}
     mixed = None
     if not args.no_split:
         v64 = h(c)
+        # errors of the split sigma at the FP64 ground state of this
+        # superblock (device Davidson, FP64): Ritz energy and residual norm
+        d = sb.diagonal_cpu()
+        g0 = np.zeros((1, sb.wave_size))
+        g0[0, int(np.argmin(d))] = 1.0
+        e_ref, x_ref, _, _ = block_amd.davidson(sb, g0, tol=1e-14)
+        x = x_ref[0] / np.linalg.norm(x_ref[0])
+        hx64 = h(x)
         h.precision = block_amd.SPLIT_BF16
         vs = h(c)
+        hxs = h(x)
         s_wall, s_kern, s_disp = h.timed(c, args.warmup, args.steps)
         h.precision = block_amd.FP64
         mixed = {"tflops": round(flops * args.steps / (s_wall * 1e-3) / 1e12, 4),
@@ -420,7 +434,12 @@ def main():
                  "dispatches_per_step": s_disp / args.steps,
                  "speedup_vs_fp64": round(wall_ms / s_wall, 3),
                  "sigma_rel_err_vs_fp64": float(np.linalg.norm(vs - v64) / np.linalg.norm(v64)),
-                 "rayleigh_err_vs_fp64_Eh": float(abs(c @ vs - c @ v64))}
+                 "rayleigh_err_vs_fp64_Eh": float(abs(c @ vs - c @ v64)),
+                 "at_fp64_ground_state": {
+                     "energy_Eh": float(e_ref[0]),
+                     "ritz_energy_err_Eh": float(abs(x @ hxs - x @ hx64)),
+                     "residual_norm_fp64": float(np.linalg.norm(hx64 - (x @ hx64) * x)),
+                     "residual_norm_split": float(np.linalg.norm(hxs - (x @ hxs) * x))}}
     pairs, arena_gb = int(h.pairs), round(h.arena_bytes / 1e9, 3)
     del h
     batched = None if args.no_batch else batched_leg(sb, max(2, args.steps // 2), args.warmup)
@@ -448,6 +467,7 @@ def main():
                    "superblock_build_s": round(t_build, 2),
                    "host_peak_rss_GB": round(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1e6, 2)},
         "roofline": roofline,
+        "graph": graph,
     }
     if ramp:
         out["config"]["ramp"] = ramp
