@@ -69,6 +69,11 @@ _sig(_h, "blkh_gpu_sigma_arena_bytes", ctypes.c_long, [_vp])
 _sig(_h, "blkh_gpu_sigma_timed", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _lp])
 _sig(_h, "blkh_gpu_sigma_set_batch", ctypes.c_int, [_vp, ctypes.c_int])
 _sig(_h, "blkh_gpu_sigma_set_graph", ctypes.c_int, [_vp, ctypes.c_int])
+_sig(_h, "blkh_gpu_sigma_replan", ctypes.c_int, [_vp])
+_sig(sigma_lib, "bsg_set_tile_table", ctypes.c_int, [ctypes.POINTER(ctypes.c_int32), ctypes.c_int32])
+_sig(sigma_lib, "bsg_tile_candidates", ctypes.c_int, [ctypes.c_int32])
+_sig(sigma_lib, "bsg_last_error", ctypes.c_char_p, [])
+
 _sig(_h, "blkh_gpu_sigma_apply_n", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int])
 _sig(_h, "blkh_gpu_sigma_set_precision", ctypes.c_int, [_vp, ctypes.c_int])
 _sig(_h, "blkh_gpu_sigma_raw", ctypes.c_int, [_vp] + [ctypes.POINTER(_vp)] * 5)
@@ -99,6 +104,35 @@ class BlockError(RuntimeError):
 
 def _err():
     return BlockError(host_lib.blkh_last_error().decode())
+
+
+TILE_TABLE = os.path.join(_HERE, "tile_table.txt")
+
+
+def set_tile_table(rules):
+    """Measured tile choices for the plans created from now on: rules of
+    (m_lo, m_hi, n_lo, n_hi, fp64 geometry, split geometry), geometry numbers
+    into the kernels' candidate lists, -1 for the model's choice
+    (bsg_set_tile_table)."""
+    flat = [int(x) for r in rules for x in r]
+    arr = (ctypes.c_int32 * max(1, len(flat)))(*flat)
+    if sigma_lib.bsg_set_tile_table(arr, len(rules)) != 0:
+        raise BlockError(sigma_lib.bsg_last_error().decode())
+
+
+def read_tile_table(path=TILE_TABLE):
+    """Rules of a tile table file: one rule of six integers per line, '#'
+    comments."""
+    rules = []
+    for line in open(path):
+        line = line.split("#", 1)[0].split()
+        if line:
+            rules.append(tuple(int(x) for x in line))
+    return rules
+
+
+def tile_candidates(precision):
+    return sigma_lib.bsg_tile_candidates(int(precision))
 
 
 def _ptr(a):
@@ -264,6 +298,12 @@ class multiply_h:
         if host_lib.blkh_gpu_sigma_set_batch(self._h, int(nroot)) != 0:
             raise _err()
 
+    def replan(self):
+        """Build the plan again from the work list (tile choices set since
+        with set_tile_table apply)."""
+        if host_lib.blkh_gpu_sigma_replan(self._h) != 0:
+            raise _err()
+
     def set_graph(self, on):
         """Applications as one captured HIP graph launch each (bsg_plan_capture
         / bsg_plan_replay) instead of their kernel launches."""
@@ -423,3 +463,9 @@ def fiedler_order_matrix(m):
     if host_lib.blkh_fiedler_order_matrix(n, _ptr(m), out) != 0:
         raise _err()
     return list(out)
+
+
+# the committed tile table (bench.py --tune) applies to every plan this
+# process creates
+if os.path.exists(TILE_TABLE):
+    set_tile_table(read_tile_table())

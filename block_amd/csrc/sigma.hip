@@ -1117,14 +1117,32 @@ inline Geom make_geom_w8(int nv) {
   g.cls = nv == 6 ? 3 : 2;
   return g;
 }
-inline Geom pick_geom(int m, int n, int max_nv) {
-  // typical contraction depth: the block's own dimensions
-  const long k = std::max(m, n);
+// Measured tile choices (bsg_set_tile_table): for V blocks with m in
+// [m_lo, m_hi] and n in [n_lo, n_hi], the index of the geometry to use in
+// pick_geom's candidate list (FP64 and split-precision lists; -1: the
+// model's choice).  Set before plans are created; read when they are.
+struct TileRule {
+  int32_t m_lo, m_hi, n_lo, n_hi, fp64, split;
+};
+std::vector<TileRule> g_tile_rules;
+std::vector<Geom> geom_candidates(int max_nv) {
   std::vector<Geom> cand;
   for (int lwr = 0; lwr <= 2; ++lwr)
     for (int nr = 1; nr <= 2; ++nr) cand.push_back(make_geom(lwr, nr, PB));
   for (int nv = 4; nv <= max_nv; nv += 2) cand.push_back(make_geom(0, 1, nv));
   cand.push_back(make_geom_w8(max_nv >= 6 ? 6 : 4));
+  return cand;
+}
+inline Geom pick_geom(int m, int n, int max_nv) {
+  // typical contraction depth: the block's own dimensions
+  const long k = std::max(m, n);
+  const std::vector<Geom> cand = geom_candidates(max_nv);
+  for (const TileRule& r : g_tile_rules)
+    if (m >= r.m_lo && m <= r.m_hi && n >= r.n_lo && n <= r.n_hi) {
+      const int idx = max_nv >= 6 ? r.fp64 : r.split;
+      if (idx >= 0 && idx < (int)cand.size() && cand[idx].tm <= GTM) return cand[idx];
+      break;
+    }
   Geom best = cand[0];
   long bc = -1;
   for (const Geom& g : cand) {
@@ -1920,6 +1938,24 @@ int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* p, const double* arena, int
   HIPCHK(hipGetLastError());
   p->split_src = arena;
   return BSG_OK;
+}
+
+int bsg_set_tile_table(const int32_t* rules, int32_t n) {
+  if (n < 0 || (n > 0 && !rules)) return fail(BSG_ERR_ARG, "bsg_set_tile_table: bad args");
+  std::vector<TileRule> t;
+  const int nfp = (int)geom_candidates(6).size(), nsp = (int)geom_candidates(4).size();
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t* e = rules + 6 * i;
+    TileRule r{e[0], e[1], e[2], e[3], e[4], e[5]};
+    if (r.m_lo > r.m_hi || r.n_lo > r.n_hi || r.fp64 < -1 || r.fp64 >= nfp || r.split < -1 || r.split >= nsp)
+      return fail(BSG_ERR_ARG, "bsg_set_tile_table: bad rule");
+    t.push_back(r);
+  }
+  g_tile_rules = t;
+  return BSG_OK;
+}
+int bsg_tile_candidates(int32_t precision) {
+  return (int)geom_candidates(precision == BSG_SPLIT_BF16 ? 4 : 6).size();
 }
 
 int bsg_plan_set_batch(bsg_context* ctx, bsg_plan* p, int32_t nroot) {

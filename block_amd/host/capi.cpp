@@ -364,6 +364,13 @@ int blkh_gpu_sigma_timed(void* h, const double* c, int nvec, int warmup, int rep
     return 0;
   });
 }
+// the handle's plan built again (tile choices set since with bsg_set_tile_table)
+int blkh_gpu_sigma_replan(void* h) {
+  return guarded([&]() {
+    static_cast<GpuSigmaHandle*>(h)->sig->replan();
+    return 0;
+  });
+}
 // sigma applications through a captured HIP graph (1) or launched one by one (0)
 int blkh_gpu_sigma_set_graph(void* h, int on) {
   return guarded([&]() {

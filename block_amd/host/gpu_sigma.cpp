@@ -244,13 +244,9 @@ GpuSigma::GpuSigma(GpuContext& g, const SuperBlock& sb, int max_vec) : g_(g) {
   size_t wbytes = sizeof(double) * std::max<long>(1, wave_size_) * max_vec_;
   bsg_check(bsg_malloc(g_.ctx, wbytes, (void**)&d_c_), "c buffer");
   bsg_check(bsg_malloc(g_.ctx, wbytes, (void**)&d_v_), "v buffer");
-  bsg_chunk_t chunk;
-  chunk.vblocks = wl.vb.data();
-  chunk.n_vblocks = (int32_t)wl.vb.size();
-  chunk.pairs = wl.pairs.data();
-  chunk.n_pairs = (int32_t)wl.pairs.size();
-  bsg_check(bsg_plan_create(g_.ctx, &chunk, 1, wave_size_, &plan_), "plan");
-  if (max_vec_ > 1) set_batch(max_vec_);
+  vb_ = std::move(wl.vb);
+  pairs_ = std::move(wl.pairs);
+  replan();
   flops_ = bsg_plan_flops(plan_);
   double t3 = secs();
   g_setup.compile += t1 - t0;
@@ -306,6 +302,23 @@ void GpuSigma::set_graph(bool on) {
   use_graph_ = on;
   graph_precision_ = -1;
   graph_nvec_ = 0;
+}
+
+void GpuSigma::replan() {
+  if (plan_) bsg_check(bsg_plan_destroy(plan_), "plan destroy");
+  plan_ = nullptr;
+  bsg_chunk_t chunk;
+  chunk.vblocks = vb_.data();
+  chunk.n_vblocks = (int32_t)vb_.size();
+  chunk.pairs = pairs_.data();
+  chunk.n_pairs = (int32_t)pairs_.size();
+  bsg_check(bsg_plan_create(g_.ctx, &chunk, 1, wave_size_, &plan_), "plan");
+  batch_ = 1;
+  split_ready_ = false;
+  graph_precision_ = -1;
+  graph_nvec_ = 0;
+  if (max_vec_ > 1) set_batch(max_vec_);
+  if (precision_ == BSG_SPLIT_BF16) set_precision(BSG_SPLIT_BF16);
 }
 
 void GpuSigma::set_batch(int nroot) {

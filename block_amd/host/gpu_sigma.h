@@ -105,6 +105,8 @@ class GpuSigma {
   void apply(const double* c_host, double* v_host, int nvec = 1);
   // roots per launch of apply_device (bsg_plan_set_batch)
   void set_batch(int nroot);
+  // the plan built again from the work list (after bsg_set_tile_table)
+  void replan();
   int max_vec() const { return max_vec_; }
   // device vectors: v += H c
   void apply_device(const double* dc, double* dv, int nvec = 1);
@@ -137,6 +139,8 @@ class GpuSigma {
   bool use_graph_ = false;
   int graph_precision_ = -1, graph_nvec_ = 0;
   std::vector<bsg_diag_t> diag_terms_;
+  std::vector<bsg_vblock_t> vb_;  // the work list the plan is built from
+  std::vector<bsg_pair_t> pairs_;
 };
 
 // Operator rotations O' = R_r^T O R_c on the GPU (transform_operators,

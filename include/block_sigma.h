@@ -127,6 +127,17 @@ int64_t bsg_plan_arena_extent(const bsg_plan* plan);
  * No counterpart in the reference (its sigma is FP64 only). */
 int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* plan, const double* arena, int64_t n);
 
+/* Measured tile choices (no counterpart in the reference): n rules of six
+ * int32 (m_lo, m_hi, n_lo, n_hi, fp64, split); a V block of m x n inside a
+ * rule's ranges gets tile geometry number fp64 (FP64 kernels) or split
+ * (split-precision kernels) of the candidate list, -1 leaving the MFMA-count
+ * model's choice; the first matching rule applies.  Process-wide, read when
+ * plans are created (n = 0 clears).  bsg_tile_candidates gives the list's
+ * length for a precision.  The tuner that measures them is bench.py --tune;
+ * block_amd/tile_table.txt holds the committed result. */
+int bsg_set_tile_table(const int32_t* rules, int32_t n);
+int bsg_tile_candidates(int32_t precision);
+
 /* Roots per launch (default 1): bsg_sigma_apply runs up to nroot vectors in
  * one set of kernel launches (every tile once per root, a tile's roots on one
  * XCD so they share its operator blocks in L2), the block Davidson's new
