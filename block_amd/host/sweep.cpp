@@ -275,11 +275,23 @@ std::vector<std::vector<double>> density_matrix(const SuperBlock& sb, const Solv
         dgemm(true, false, bk.cols, bk.cols, bk.rows, weights[r], C, bk.cols, C, bk.cols, 1.0, rho[bk.r].data(), bk.cols);
     }
   if (noise > 0.0) {
+    // one random state over every sector, normalised, added with total
+    // weight `noise` (add_twodot_noise, density.C:92-156: random
+    // wavefunctions scaled to unit norm, MultiplyProduct with weight noise):
+    // a sector's share grows with its size, so noise does not spread the
+    // kept states evenly over the (many, mostly unphysical) small sectors
+    int nz = 0;
+    for (int q = 0; q < si.nsec(); ++q) nz += si.dim[q] > 0;
     for (int q = 0; q < si.nsec(); ++q) {
       int d = si.dim[q];
+      if (d == 0) continue;
       std::vector<double> G((size_t)d * d);
-      for (auto& x : G) x = gauss(rng);
-      dgemm(false, true, d, d, d, noise / ((double)d * d), G.data(), d, G.data(), d, 1.0, rho[q].data(), d);
+      double z = 0.0;
+      for (auto& x : G) {
+        x = gauss(rng);
+        z += x * x;
+      }
+      dgemm(false, true, d, d, d, noise / (nz * z), G.data(), d, G.data(), d, 1.0, rho[q].data(), d);
     }
   }
   return rho;
