@@ -96,14 +96,14 @@ def _rows(sw, first, last):
     return sw[(sw[:, COLS.index("sweep")] >= first) & (sw[:, COLS.index("sweep")] <= last)]
 
 
-def ramp_leg(ints, cfg, M, ck):
+def ramp_leg(ints, cfg, M, ck, verbose=1):
     """FP64 sweeps growing the blocks to M with density-matrix noise, the
     sweep state saved after every sweep (the timed legs start from it)."""
     sched = [tuple(r) for r in cfg["ramp"]]
     n = cfg["ramp_sweeps"]
     t0 = time.time()
     e, sw = block_amd.dmrg(ints, nelec=cfg["nelec"], schedule=sched, max_sweeps=n, sweep_tol=0.0, checkpoint=ck,
-                           verbose=1)
+                           verbose=verbose)
     fwd, bwd, fsec, bsec = block_amd.state_block_dims(ints, ck)
     k = ints.norb
     return {"schedule": [list(r) for r in sched], "sweeps": n, "seconds": round(time.time() - t0, 1),
@@ -113,19 +113,22 @@ def ramp_leg(ints, cfg, M, ck):
             "kept_states_mid": fwd[k // 2 - 2], "max_kept": max(fwd + bwd)}
 
 
-def sweep_leg(ints, cfg, M, ck):
+def sweep_leg(ints, cfg, M, ck, tmp, verbose=1):
     """One full two-site sweep at M from the ramp's saved state, in FP64 and
     with the precision ladder; every step starts from the previous step's
-    transformed wavefunction."""
+    transformed wavefunction.  The FP64 leg saves its final state, whose
+    blocks give the kept states of the timed sweep."""
     n = cfg["ramp_sweeps"]
     sched = [tuple(r) for r in cfg["ramp"]] + [(n, M, cfg["timed_tol"], 0.0)]
-    out = {"from_sweep": n, "davidson_tol": cfg["timed_tol"], "timed_sweeps": [n, n + 1]}
+    out = {"from_sweep": n, "davidson_tol": cfg["timed_tol"], "timed_sweeps": [n]}
+    after = os.path.join(tmp, "timed.state")
     for name, ladder in (("fp64", 0.0), ("mixed", 1e-5)):
         t0 = time.time()
-        e, sw = block_amd.dmrg(ints, nelec=cfg["nelec"], schedule=sched, max_sweeps=n + 2, sweep_tol=0.0,
-                               ladder=ladder, ladder_from=n, resume=ck, verbose=1)
+        e, sw = block_amd.dmrg(ints, nelec=cfg["nelec"], schedule=sched, max_sweeps=n + 1, sweep_tol=0.0,
+                               ladder=ladder, ladder_from=n, resume=ck, verbose=verbose,
+                               checkpoint=after if name == "fp64" else None)
         wall = time.time() - t0
-        rows = _rows(sw, n, n + 1)
+        rows = _rows(sw, n, n)
         col = lambda c: float(rows[:, COLS.index(c)].sum())
         out[name] = {"sweep_seconds": round(col("seconds"), 3), "sigma_seconds": round(col("sigma_seconds"), 3),
                      "sigma_calls": int(col("sigma_calls")), "split_calls": int(col("split_calls")),
@@ -136,6 +139,8 @@ def sweep_leg(ints, cfg, M, ck):
                      "max_discarded": float(rows[:, COLS.index("discarded")].max()),
                      "energy": float(e[0]), "run_seconds": round(wall, 2)}
         log(f"sweep {name}: {out[name]['sweep_seconds']} s, E = {out[name]['energy']:.12f}")
+    fwd, bwd, fsec, bsec = block_amd.state_block_dims(ints, after)
+    out["kept_states_after"] = {"fwd": fwd, "bwd": bwd, "fwd_sectors": fsec, "bwd_sectors": bsec}
     out["sweep_speedup_mixed_vs_fp64"] = round(out["fp64"]["sweep_seconds"] / out["mixed"]["sweep_seconds"], 3)
     out["sigma_speedup_mixed_vs_fp64"] = round(out["fp64"]["sigma_seconds"] / max(out["mixed"]["sigma_seconds"], 1e-9), 3)
     out["abs_dE_Eh"] = abs(out["fp64"]["energy"] - out["mixed"]["energy"])
@@ -350,6 +355,7 @@ def main():
                     help="GPU time by kernel and copy direction of a profiled run (no GPU run)")
     ap.add_argument("--summarize-traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
                     help="HBM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (no GPU run)")
+    ap.add_argument("--verbose", type=int, default=1, help="sweep log level (2: per step)")
     ap.add_argument("--last", type=int, default=6, help="dispatches per kernel to summarise (the last ones)")
     ap.add_argument("--kernels", default="k_sigma|k_small|k_reduce", help="regex of the kernel names to summarise")
     args = ap.parse_args()
@@ -378,7 +384,7 @@ def main():
     ramp = None
     if not args.state:
         log(f"ramp to M={M} on {args.config}")
-        ramp = ramp_leg(ints, cfg, M, ck)
+        ramp = ramp_leg(ints, cfg, M, ck, args.verbose)
         log(f"ramp done in {ramp['seconds']} s, kept states at mid-chain {ramp['kept_states_mid']}")
         if args.keep_state:
             import shutil
@@ -476,7 +482,7 @@ def main():
     if batched:
         out["batched_roots"] = batched
     if not args.no_sweep:
-        out["sweep"] = sweep_leg(ints, cfg, M, ck)
+        out["sweep"] = sweep_leg(ints, cfg, M, ck, tmp, args.verbose)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(sb, args.cpu_seconds)

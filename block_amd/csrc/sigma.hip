@@ -149,55 +149,6 @@ __device__ inline void split_f32(double x, __bf16& h, __bf16& m, __bf16& l) {
   m = (__bf16)r;
   l = (__bf16)(r - (float)m);
 }
-// acc[r][t] += A(r) B(t) over one 32-deep K group for t < nv, r < nr.  Lane
-// (li, lk) supplies 8 values per operand, its K slots 8 lk .. 8 lk + 7; the
-// caller maps slots to K indices the same way for A and B (a bijection onto
-// the group's 32 K), so the sum is over all 32 K.  Result register i of a
-// block holds (4 lk + i, li) (v_mfma_f32_16x16x32_bf16 layout, checked on
-// gfx950).
-template <int NR, int NB>
-__device__ inline void mma_split32(int nv, int nr, dbl4 (&acc)[NR][NB], const double (&a)[NR][8],
-                                   const double (&b)[NB][8]) {
-  // B slices once, A slices one row block at a time (register budget)
-  bf16x8 bh[NB], bm[NB], bl[NB];
-#pragma unroll
-  for (int t = 0; t < NB; ++t)
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      __bf16 h, m, l;
-      split_f32(b[t][e], h, m, l);
-      bh[t][e] = h;
-      bm[t][e] = m;
-      bl[t][e] = l;
-    }
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    if (r >= nr) continue;
-    bf16x8 ah, am, al;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      __bf16 h, m, l;
-      split_f32(a[r][e], h, m, l);
-      ah[e] = h;
-      am[e] = m;
-      al[e] = l;
-    }
-#pragma unroll
-    for (int t = 0; t < NB; ++t)
-      if (t < nv) {
-        f32x4 x = {0.f, 0.f, 0.f, 0.f};
-        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], x, 0, 0, 0);
-        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[t], x, 0, 0, 0);
-        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], x, 0, 0, 0);
-        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[t], x, 0, 0, 0);
-        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[t], x, 0, 0, 0);
-        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], x, 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[r][t][i] += (double)x[i];
-      }
-  }
-}
-
 // acc[r][t] += A(row blocks of this wave, 4-deep K step at k) * b[t] for the
 // first nv column blocks and the first nr row blocks; A k-major in LDS.
 template <int NRM, int NB>
@@ -870,9 +821,12 @@ __global__ __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(2))) v
 // the element W(lane%16, lane/16 + 4 i), which is exactly the A operand of
 // the second product's K step i.  Identity-B pairs use the direct W = op(A) C
 // orientation, whose result layout matches the V accumulators.
+// Small blocks run in FP64 in both precision modes: their MFMAs are not
+// what bounds them (operand latency is), so BF16 slices would only add the
+// slicing work, and FP64 is exact.
 // SB: 16-blocks per dimension of the V blocks an instantiation takes (the
 // plan uses SB = 2, blocks up to 32 x 32, two waves per SIMD).
-template <int SB, bool SPLIT>
+template <int SB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_small(
     const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
     const int4* __restrict__ units, int nunits, const double* __restrict__ arena, const double* __restrict__ c,
@@ -892,8 +846,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const Seg sg = segs[si];
   const bsg_vblock_t vb = vbs[sg.vb];
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-  // row of accumulator register i inside its block (FP64 / split MFMA layout)
-  const int rq = SPLIT ? 4 * lk : lk, rs = SPLIT ? 1 : 4;
+  // row of accumulator register i inside its block: lk + 4 i
   const int nrb = (vb.m + 15) / 16, nnb = (vb.n + 15) / 16;
   dbl4 accV[SB][SB];
 #pragma unroll
@@ -913,7 +866,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int t = 0; t < SB; ++t)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const int row = r * 16 + rq + rs * i, col = t * 16 + li;
+            const int row = r * 16 + lk + 4 * i, col = t * 16 + li;
             const bool ok = row < vb.m && col < vb.n;
             accV[r][t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
           }
@@ -951,19 +904,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         dbl4 accW[SB][1];
 #pragma unroll
         for (int r = 0; r < SB; ++r) accW[r][0] = dbl4{0.0, 0.0, 0.0, 0.0};
-        if constexpr (!SPLIT) {
 #pragma unroll
-          for (int s4 = 0; s4 < KF; ++s4)
-            if (s4 < nk1)
+        for (int s4 = 0; s4 < KF; ++s4)
+          if (s4 < nk1)
 #pragma unroll
-              for (int r = 0; r < SB; ++r)
-                if (r < nrb) accW[r][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r][s4], ct[t][s4], accW[r][0], 0, 0, 0);
-        } else {
-          double bt[1][8];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) bt[0][e] = ct[t][e];
-          mma_split32<SB, 1>(1, nrb, accW, a, bt);
-        }
+            for (int r = 0; r < SB; ++r)
+              if (r < nrb) accW[r][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r][s4], ct[t][s4], accW[r][0], 0, 0, 0);
 #pragma unroll
         for (int r = 0; r < SB; ++r)
 #pragma unroll
@@ -973,75 +919,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     const __amdgpu_buffer_rsrc_t rB = block_rsrc(arena + pr.b_off, (int64_t)vb.n * pr.k2);
     const int ks = pr.b_trans ? 1 : vb.n, cs = pr.b_trans ? pr.k2 : 1;
-    if constexpr (!SPLIT) {
-      double b[SB][4][SB];  // b[jb][i][t]: B'(jb*16 + 4 i + lk, t*16 + li)
+    double b[SB][4][SB];  // b[jb][i][t]: B'(jb*16 + 4 i + lk, t*16 + li)
 #pragma unroll
-      for (int t = 0; t < SB; ++t) {
-        const uint32_t boff = (t < nnb && t * 16 + li < vb.n) ? (uint32_t)(lk * ks + (t * 16 + li) * cs) * 8u : OOB;
+    for (int t = 0; t < SB; ++t) {
+      const uint32_t boff = (t < nnb && t * 16 + li < vb.n) ? (uint32_t)(lk * ks + (t * 16 + li) * cs) * 8u : OOB;
 #pragma unroll
-        for (int jb = 0; jb < SB; ++jb)
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-            b[jb][i][t] = jb < njb ? bload2(rB, boff, (uint32_t)((jb * 4 + i) * ks) * 32u) : 0.0;
-      }
-#pragma unroll
-      for (int jb = 0; jb < SB; ++jb) {
-        if (jb >= njb) continue;
-        // W^T block row jb: register i of lane (li, lk) is W(li, jb*16 + lk + 4 i)
-        dbl4 accT[SB];
-#pragma unroll
-        for (int r = 0; r < SB; ++r) accT[r] = dbl4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int s4 = 0; s4 < KF; ++s4)
-          if (s4 < nk1)
-#pragma unroll
-            for (int r = 0; r < SB; ++r)
-              if (r < nrb) accT[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ct[jb][s4], a[r][s4], accT[r], 0, 0, 0);
-        // GEMM 2 over K = jb*16 + 4 i + lk, A operand = register i of accT
+      for (int jb = 0; jb < SB; ++jb)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
+          b[jb][i][t] = jb < njb ? bload2(rB, boff, (uint32_t)((jb * 4 + i) * ks) * 32u) : 0.0;
+    }
+#pragma unroll
+    for (int jb = 0; jb < SB; ++jb) {
+      if (jb >= njb) continue;
+      // W^T block row jb: register i of lane (li, lk) is W(li, jb*16 + lk + 4 i)
+      dbl4 accT[SB];
+#pragma unroll
+      for (int r = 0; r < SB; ++r) accT[r] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s4 = 0; s4 < KF; ++s4)
+        if (s4 < nk1)
 #pragma unroll
           for (int r = 0; r < SB; ++r)
+            if (r < nrb) accT[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ct[jb][s4], a[r][s4], accT[r], 0, 0, 0);
+      // GEMM 2 over K = jb*16 + 4 i + lk, A operand = register i of accT
 #pragma unroll
-            for (int t = 0; t < SB; ++t)
-              if (r < nrb && t < nnb)
-                accV[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aw * accT[r][i], b[jb][i][t], accV[r][t], 0, 0, 0);
-      }
-    } else {
-      // W^T blocks (rows: W columns of block jb, columns: W rows of block r);
-      // with the split MFMA layout register i of lane (li, lk) is
-      // W(r*16 + li, jb*16 + 4 lk + i)
-      dbl4 accT[SB][SB];
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int jb = 0; jb < SB; ++jb) {
+        for (int r = 0; r < SB; ++r)
 #pragma unroll
-        for (int r = 0; r < SB; ++r) accT[jb][r] = dbl4{0.0, 0.0, 0.0, 0.0};
-        if (jb < njb) {
-          double cj[1][8];
-          dbl4 at[1][SB];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) cj[0][e] = ct[jb][e];
-#pragma unroll
-          for (int r = 0; r < SB; ++r) at[0][r] = dbl4{0.0, 0.0, 0.0, 0.0};
-          mma_split32<1, SB>(nrb, 1, at, cj, a);
-#pragma unroll
-          for (int r = 0; r < SB; ++r) accT[jb][r] = at[0][r];
-        }
-      }
-      // GEMM 2 over the 32 W columns: slot e is column (e < 4 ? 4 lk + e : 16 + 4 lk + e - 4)
-      double aa[SB][8], bb[SB][8];
-#pragma unroll
-      for (int r = 0; r < SB; ++r)
-#pragma unroll
-        for (int e = 0; e < 8; ++e) aa[r][e] = aw * accT[e >> 2][r][e & 3];
-#pragma unroll
-      for (int t = 0; t < SB; ++t) {
-        const uint32_t boff =
-            (t < nnb && t * 16 + li < vb.n) ? (uint32_t)(4 * lk * ks + (t * 16 + li) * cs) * 8u : OOB;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) bb[t][e] = bload2(rB, boff, (uint32_t)(((e >> 2) * 16 + (e & 3)) * ks) * 8u);
-      }
-      mma_split32<SB, SB>(nnb, nrb, accV, aa, bb);
+          for (int t = 0; t < SB; ++t)
+            if (r < nrb && t < nnb)
+              accV[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aw * accT[r][i], b[jb][i][t], accV[r][t], 0, 0, 0);
     }
   }
   const bool direct = sg.p_off < 0;
@@ -1053,7 +962,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       const int col = t * 16 + li;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int row = r * 16 + rq + rs * i;
+        const int row = r * 16 + lk + 4 * i;
         if (row < vb.m && col < vb.n) {
           if (direct)
             out[(size_t)row * vb.n + col] += accV[r][t][i];
@@ -2018,12 +1927,8 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
     HIPCHK(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
     if (ev) HIPCHK(hipEventRecord(ev[1], ctx->side));
     if (ch.n_s[0] > 0) {
-      if (split)
-        hipLaunchKernelGGL((k_small<2, true>), grid(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
-                           ch.d_s[0], ch.n_s[0], arena, c, v, part, rs);
-      else
-        hipLaunchKernelGGL((k_small<2, false>), grid(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg,
-                           ch.d_s[0], ch.n_s[0], arena, c, v, part, rs);
+      hipLaunchKernelGGL((k_small<2>), grid(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_s[0],
+                         ch.n_s[0], arena, c, v, part, rs);
       ++nl;
     }
     if (nbasic > 0) {

@@ -645,10 +645,12 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
     res.sweeps.push_back(rec);
     if (P.verbose > 1) {
       const SetupTimes& st = gpu_setup_times();
+      const BuildTimes& bt = gpu_build_times();
       std::fprintf(stderr,
                    "  sigma set-up so far: compile %.2f upload %.2f plan %.2f; renorm so far: eig %.2f rotate %.2f comp "
-                   "%.2f (build %.2f)\n",
-                   st.compile, st.upload, st.plan, g_rt.eig, g_rt.rotate, g_rt.comp, comp_build_seconds());
+                   "%.2f (build %.2f); device builds so far: record %.2f execute %.2f\n",
+                   st.compile, st.upload, st.plan, g_rt.eig, g_rt.rotate, g_rt.comp, comp_build_seconds(), bt.record,
+                   bt.execute);
     }
     if (P.verbose)
       std::fprintf(stderr,
