@@ -334,6 +334,76 @@ def fetch_val(d, name):
 
 
 
+# V block dimension ranges of the tile tuner's buckets (blocks up to 32 x 32
+# run in the small-block kernel, which has no tile choice)
+TUNE_EDGES = [(1, 32), (33, 64), (65, 128), (129, 256), (257, 512), (513, 1 << 30)]
+
+
+def tune_tiles(sb, out_path, steps, warmup, top=6, min_gain=0.02):
+    """Measured tile table: the V blocks are bucketed by (rows, cols) ranges;
+    for the buckets with most of the flops (about m n (m + n) per block),
+    every candidate geometry of the FP64 and of the split-precision kernels
+    is timed on this superblock's whole sigma (the other buckets on their
+    current choice), and a bucket keeps the fastest candidate when it beats
+    the MFMA-count model's choice by min_gain.  Writes the rules to out_path
+    and checks that the tuned plan gives the same sigma."""
+    shapes = sb.block_shapes()
+    weight = {}
+    for m, n in shapes:
+        if max(m, n) <= 32:
+            continue
+        bm = next(i for i, (lo, hi) in enumerate(TUNE_EDGES) if lo <= m <= hi)
+        bn = next(i for i, (lo, hi) in enumerate(TUNE_EDGES) if lo <= n <= hi)
+        weight[(bm, bn)] = weight.get((bm, bn), 0.0) + float(m) * n * (m + n)
+    buckets = sorted(weight, key=lambda b: -weight[b])[:top]
+    total = sum(weight.values()) or 1.0
+    h = block_amd.multiply_h(sb)
+    c = np.random.default_rng(1).standard_normal(sb.wave_size)
+    block_amd.set_tile_table([])
+    h.replan()
+    v0 = h(c)
+    choice = {b: [-1, -1] for b in buckets}
+
+    def rules():
+        return [(TUNE_EDGES[bm][0], TUNE_EDGES[bm][1], TUNE_EDGES[bn][0], TUNE_EDGES[bn][1], f, sp)
+                for (bm, bn), (f, sp) in choice.items() if f >= 0 or sp >= 0]
+
+    def time_with():
+        block_amd.set_tile_table(rules())
+        h.replan()
+        return min(h.timed(c, warmup, steps)[0] for _ in range(2)) / steps
+
+    log_rows = []
+    for slot, prec in ((0, block_amd.FP64), (1, block_amd.SPLIT_BF16)):
+        h.precision = prec
+        for b in buckets:
+            choice[b][slot] = -1
+            base = time_with()
+            best, best_ms = -1, base
+            for cand in range(block_amd.tile_candidates(prec)):
+                choice[b][slot] = cand
+                ms = time_with()
+                if ms < best_ms:
+                    best, best_ms = cand, ms
+            choice[b][slot] = best if best_ms < base * (1.0 - min_gain) else -1
+            log_rows.append((slot, b, round(weight[b] / total, 3), round(base, 4), best, round(best_ms, 4)))
+            log(f"tune {'fp64' if slot == 0 else 'split'} bucket m {TUNE_EDGES[b[0]]} n {TUNE_EDGES[b[1]]} "
+                f"({weight[b] / total:.1%} of flops): model {base:.4f} ms, best candidate {best} {best_ms:.4f} ms")
+    h.precision = block_amd.FP64
+    block_amd.set_tile_table(rules())
+    h.replan()
+    v1 = h(c)
+    err = float(np.linalg.norm(v1 - v0) / np.linalg.norm(v0))
+    if err > 1e-12:
+        raise RuntimeError(f"tuned plan changed sigma: relative difference {err:.2e}")
+    with open(out_path, "w") as f:
+        f.write("# m_lo m_hi n_lo n_hi fp64 split: tile geometry per V block shape range (-1: MFMA-count model)\n")
+        f.write(f"# measured by bench.py --tune on {sb.wave_size}-element superblock, sigma ms per bucket trial\n")
+        for r in rules():
+            f.write(" ".join(str(x) for x in r) + "\n")
+    return {"rules": rules(), "trials": log_rows, "sigma_rel_diff_tuned_vs_model": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -355,6 +425,7 @@ def main():
                     help="GPU time by kernel and copy direction of a profiled run (no GPU run)")
     ap.add_argument("--summarize-traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
                     help="HBM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (no GPU run)")
+    ap.add_argument("--tune", metavar="OUT", help="measure the tile table on this superblock and write it to OUT")
     ap.add_argument("--verbose", type=int, default=1, help="sweep log level (2: per step)")
     ap.add_argument("--last", type=int, default=6, help="dispatches per kernel to summarise (the last ones)")
     ap.add_argument("--kernels", default="k_sigma|k_small|k_reduce", help="regex of the kernel names to summarise")
@@ -392,6 +463,9 @@ def main():
     t0 = time.time()
     sb = block_amd.SuperBlock.from_sweep_state(ints, ck, cfg["nelec"], 0, ints.isym)
     t_build = time.time() - t0
+    if args.tune:
+        print(json.dumps({"tune": tune_tiles(sb, args.tune, args.steps, args.warmup)}))
+        return 0
     h = block_amd.multiply_h(sb, device=0)
     c = np.random.default_rng(0).standard_normal(sb.wave_size)
     c /= np.linalg.norm(c)

@@ -83,6 +83,7 @@ _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
       ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp])
 _sig(_h, "blkh_build_check", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _lp])
 _sig(_h, "blkh_build_check_cpu", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _lp])
+_sig(_h, "blkh_build_times", ctypes.c_int, [_dp])
 _sig(_h, "blkh_state_block_dims", ctypes.c_int, [_vp, ctypes.c_char_p] + [ctypes.POINTER(ctypes.c_int)] * 4)
 
 _sig(_h, "blkh_davidson", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp,
@@ -396,6 +397,14 @@ def state_block_dims(ints, path):
     if host_lib.blkh_state_block_dims(ints._h, str(path).encode(), *arrs) != 0:
         raise _err()
     return tuple(list(a) for a in arrs)
+
+
+def build_times():
+    """Accumulated recorded-build phases of this process: dict of seconds
+    (record, execute, grouping, device), algorithmic bytes and items."""
+    out = np.zeros(6)
+    host_lib.blkh_build_times(_ptr(out))
+    return dict(zip(("record", "execute", "grouping", "device", "bytes", "items"), out.tolist()))
 
 
 def build_check_cpu(ints, nsys, M):

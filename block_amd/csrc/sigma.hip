@@ -824,15 +824,16 @@ __global__ __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(2))) v
 // Small blocks run in FP64 in both precision modes: their MFMAs are not
 // what bounds them (operand latency is), so BF16 slices would only add the
 // slicing work, and FP64 is exact.
-// SB: 16-blocks per dimension of the V blocks an instantiation takes (the
-// plan uses SB = 2, blocks up to 32 x 32, two waves per SIMD).
+// SB: 16-blocks per dimension of the V blocks an instantiation takes: SB = 1
+// for blocks (and contraction depths) up to 16, whose few registers leave
+// room for four waves per SIMD to hide the operand latency, SB = 2 for the
+// rest up to 32 x 32 (two waves per SIMD).
 template <int SB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_small(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SB == 1 ? 4 : 2))) void k_small(
     const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
     const int4* __restrict__ units, int nunits, const double* __restrict__ arena, const double* __restrict__ c,
     double* __restrict__ v, double* __restrict__ part, RootSet roots) {
   constexpr int KF = 4 * SB;  // 4-deep K steps of a dimension <= 16 SB
-  static_assert(KF == 8, "one 32-deep K group per product");
   int u;
   const int root = root_of(blockIdx.x, roots.n, u);
   if (u >= nunits) return;
@@ -1388,7 +1389,7 @@ struct PlanChunk {
   int n_g[NCLS] = {};
   TileRef* d_gbf[3] = {};  // split-precision tiles (basic, 256 wide, 512 wide)
   int n_gbf[3] = {};
-  int4* d_s[2] = {nullptr, nullptr};  // FP64 small-block units, 4 per workgroup (SB = 2, 3)
+  int4* d_s[2] = {nullptr, nullptr};  // FP64 small-block units, 4 per workgroup (SB = 2, SB = 1)
   int n_s[2] = {0, 0};
   RedItem* d_red[2] = {nullptr, nullptr};  // after the FP64 / the split-precision kernels
   int n_red[2] = {0, 0};
@@ -1614,6 +1615,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       // whose C block is larger than that (or whose A is the identity and B
       // is not): those run in the grouped kernel with its K pipeline
       const bool small_vb = std::max(vb.m, vb.n) <= 32;
+      // tiny: the block and every small-kernel pair's C block within 16 x 16
+      bool tiny = std::max(vb.m, vb.n) <= 16;
       std::vector<int> fast, slow;  // pair indices (absolute) in term order
       for (int q = 0; q < vb.count; ++q) {
         const auto& pr = ch.pairs[vb.first + q];
@@ -1632,6 +1635,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         }
         const bool f = small_vb && ((pr.a_off >= 0 && pr.k1 <= 32 && pr.k2 <= 32) || (pr.a_off < 0 && pr.b_off < 0));
         (f ? fast : slow).push_back(vb.first + q);
+        if (f && (pr.k1 > 16 || pr.k2 > 16)) tiny = false;
       }
       // the block's pairs in device order: the small-kernel ones, then the rest
       const int base = (int)perm.size();
@@ -1692,7 +1696,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           if (usmall[u]) {
             long sc = 0;
             for (int q = 0; q < sg.count; ++q) sc += small_cost(vb.m, vb.n, perm[sg.first + q]);
-            sunits[0].push_back({sc, si});
+            sunits[tiny ? 1 : 0].push_back({sc, si});
             continue;
           }
           long ucost = 0;
@@ -1921,7 +1925,7 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
   // the wide tiles on the context stream; they write disjoint V blocks and
   // partial tiles, and the reduction waits for both
   const int nbasic = split ? ch.n_gbf[0] : ch.n_g[0];
-  const bool side = ch.n_s[0] > 0 || nbasic > 0;
+  const bool side = ch.n_s[0] > 0 || ch.n_s[1] > 0 || nbasic > 0;
   if (side) {
     HIPCHK(hipEventRecord(ctx->fork, st));
     HIPCHK(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
@@ -1929,6 +1933,11 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
     if (ch.n_s[0] > 0) {
       hipLaunchKernelGGL((k_small<2>), grid(ch.n_s[0]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_s[0],
                          ch.n_s[0], arena, c, v, part, rs);
+      ++nl;
+    }
+    if (ch.n_s[1] > 0) {
+      hipLaunchKernelGGL((k_small<1>), grid(ch.n_s[1]), dim3(256), 0, ctx->side, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_s[1],
+                         ch.n_s[1], arena, c, v, part, rs);
       ++nl;
     }
     if (nbasic > 0) {
@@ -2127,7 +2136,7 @@ int bsg_sigma_profile(bsg_context* ctx, bsg_plan* p, const double* arena, const 
   size_t k = 1;
   for (int r = 0; r < reps * ngroups; ++r)
     for (auto& ch : p->chunks) {
-      const bool sidek = ch.n_s[0] > 0 || (split ? ch.n_gbf[0] : ch.n_g[0]) > 0;
+      const bool sidek = ch.n_s[0] > 0 || ch.n_s[1] > 0 || (split ? ch.n_gbf[0] : ch.n_g[0]) > 0;
       // [0] fork to join of all fused kernels, [1] reduction, [2] the side
       // stream's kernels alone
       const int from[3] = {0, 3, 1}, to[3] = {3, 4, 2};

@@ -157,8 +157,9 @@ struct BuildSession {
   ~BuildSession();
   BuildSession(const BuildSession&) = delete;
   BuildSession& operator=(const BuildSession&) = delete;
-  // all threads' items (call after the recorded work returned)
-  std::vector<BuildItem> items() const;
+  // all threads' items in place (call after the recorded work returned;
+  // valid while the session lives)
+  std::vector<const BuildItem*> item_ptrs() const;
   // every operator an item writes, kept alive until the session ends
   std::vector<OpPtr> written() const;
   struct Impl;

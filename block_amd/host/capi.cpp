@@ -482,6 +482,15 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
   });
 }
 
+// Accumulated recorded-build times of this process: out[6] = record,
+// execute, grouping, device, algorithmic bytes, items (BuildTimes).
+int blkh_build_times(double* out) {
+  const BuildTimes& t = gpu_build_times();
+  const double v[6] = {t.record, t.execute, t.plan, t.apply, t.bytes, t.items};
+  for (int i = 0; i < 6; ++i) out[i] = v[i];
+  return 0;
+}
+
 // The recorded build executed on the host (plan_build's groups, k_build's
 // arithmetic) against the host build, for a Wilson-grown block sys of nsys
 // sites and the dot nsys (normal and complementary sets of sys (x) dot and

@@ -163,10 +163,14 @@ BuildSession::BuildSession() : impl(new Impl) {
 BuildSession::~BuildSession() {
   if (tl_session == impl.get()) tl_session = nullptr;
 }
-std::vector<BuildItem> BuildSession::items() const {
-  std::vector<BuildItem> out;
+std::vector<const BuildItem*> BuildSession::item_ptrs() const {
+  std::vector<const BuildItem*> out;
   std::lock_guard<std::mutex> lk(impl->mu);
-  for (const auto& r : impl->recs) out.insert(out.end(), r->items.begin(), r->items.end());
+  size_t n = 0;
+  for (const auto& r : impl->recs) n += r->items.size();
+  out.reserve(n);
+  for (const auto& r : impl->recs)
+    for (const BuildItem& it : r->items) out.push_back(&it);
   return out;
 }
 std::vector<OpPtr> BuildSession::written() const {

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <unordered_map>
 
 #include "gpu_sigma.h"
@@ -24,64 +25,133 @@ GpuContext* g_mat_ctx = nullptr;
 void download_ops(const std::vector<SparseOp*>& ops) {
   std::lock_guard<std::mutex> lk(g_mat_mu);
   if (!g_mat_ctx) throw std::runtime_error("materialise_host: no GPU context to download operators");
+  const double t0 = now_s();
   for (SparseOp* o : ops) {
     if (o->host_valid) continue;
     if (!o->dev) throw std::runtime_error("materialise_host: operator in neither host nor device memory");
     o->data.assign(o->size(), 0.0);
     bsg_check(bsg_memcpy_d2h(g_mat_ctx->ctx, o->data.data(), o->dev, sizeof(double) * o->size()), "operator download");
     o->host_valid = true;
+    g_build.download_bytes += 8.0 * o->size();
   }
+  g_build.download += now_s() - t0;
 }
 }  // namespace
 
 const BuildTimes& gpu_build_times() { return g_build; }
 
-BuildPlan plan_build(const std::vector<BuildItem>& items, const std::function<const double*(const SparseOp*)>& addr) {
-  // groups: items with the same level and destination rectangle, in the
-  // order recorded (one recording thread per destination operator)
-  std::vector<int> order(items.size());
-  for (size_t i = 0; i < items.size(); ++i) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    const BuildItem &x = items[a], &y = items[b];
-    if (x.level != y.level) return x.level < y.level;
-    if (x.dst != y.dst) return std::less<const SparseOp*>()(x.dst, y.dst);
-    return x.dst_off < y.dst_off;
-  });
-  BuildPlan bp;
-  for (size_t k = 0; k < order.size();) {
-    const BuildItem& h = items[order[k]];
-    if (h.level < 0 || h.level > 1) throw std::runtime_error("plan_build: unknown item level");
-    auto& groups = bp.groups[h.level];
-    auto& srcs = bp.srcs[h.level];
-    bsg_build_group_t gr;
-    gr.dst = const_cast<double*>(addr(h.dst)) + h.dst_off;
-    gr.ld = h.dst_ld;
-    gr.rows = h.rows;
-    gr.cols = h.cols;
-    gr.first = (int32_t)srcs.size();
-    gr.pad = 0;
-    size_t e = k;
-    for (; e < order.size(); ++e) {
-      const BuildItem& it = items[order[e]];
-      if (it.level != h.level || it.dst != h.dst || it.dst_off != h.dst_off) break;
-      if (it.rows != h.rows || it.cols != h.cols || it.dst_ld != h.dst_ld)
-        throw std::runtime_error("plan_build: overlapping destination rectangles");
-      bsg_build_src_t s;
-      s.src = addr(it.src) + it.src_off;
-      s.ld = it.src_ld;
-      s.trans = it.trans;
-      s.alpha = it.alpha;
-      s.beta = it.beta;
-      srcs.push_back(s);
+BuildPlan plan_build(const std::vector<const BuildItem*>& items,
+                     const std::function<const double*(const SparseOp*)>& addr) {
+  // groups: items with the same level and destination rectangle, sources in
+  // the order recorded (one recording thread per destination operator).
+  // Items are bucketed by destination operator (first appearance, stable),
+  // each bucket is ordered by (level, offset) on its own, and the buckets
+  // fill their slices of the group and source lists in parallel.  Groups
+  // write disjoint rectangles, so their order does not change the result.
+  const size_t n = items.size();
+  if (n > 0x7fffffff) throw std::runtime_error("plan_build: too many items");
+  std::unordered_map<const SparseOp*, int> dix;
+  std::vector<int> di(n);
+  int nd = 0;
+  const SparseOp* last = nullptr;
+  int lastix = -1;
+  for (size_t i = 0; i < n; ++i) {
+    const BuildItem& it = *items[i];
+    if (it.level < 0 || it.level > 1) throw std::runtime_error("plan_build: unknown item level");
+    if (it.dst != last) {
+      auto f = dix.emplace(it.dst, nd);
+      if (f.second) ++nd;
+      last = it.dst;
+      lastix = f.first->second;
     }
-    gr.count = (int32_t)(e - k);
-    bp.bytes += 8.0 * h.rows * h.cols * (gr.count + 2);
-    groups.push_back(gr);
-    k = e;
+    di[i] = lastix;
   }
-  for (int l = 0; l < 2; ++l)
-    if (bp.groups[l].size() > 0x7fffffff || bp.srcs[l].size() > 0x7fffffff)
-      throw std::runtime_error("plan_build: too many items");
+  std::vector<int64_t> start(nd + 1, 0);
+  for (size_t i = 0; i < n; ++i) ++start[di[i] + 1];
+  for (int d = 0; d < nd; ++d) start[d + 1] += start[d];
+  std::vector<int> order(n);
+  {
+    std::vector<int64_t> pos(start.begin(), start.end() - 1);
+    for (size_t i = 0; i < n; ++i) order[pos[di[i]]++] = (int)i;
+  }
+  std::vector<int64_t> ng[2], ns[2];
+  for (int l = 0; l < 2; ++l) {
+    ng[l].assign(nd + 1, 0);
+    ns[l].assign(nd + 1, 0);
+  }
+  auto same = [&](const BuildItem& x, const BuildItem& y) { return x.level == y.level && x.dst_off == y.dst_off; };
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int d = 0; d < nd; ++d) {
+    std::stable_sort(order.begin() + start[d], order.begin() + start[d + 1], [&](int x, int y) {
+      const BuildItem &p = *items[x], &q = *items[y];
+      if (p.level != q.level) return p.level < q.level;
+      return p.dst_off < q.dst_off;
+    });
+    for (int64_t k = start[d]; k < start[d + 1];) {
+      const BuildItem& h = *items[order[k]];
+      int64_t e = k + 1;
+      while (e < start[d + 1] && same(*items[order[e]], h)) ++e;
+      ++ng[h.level][d + 1];
+      ns[h.level][d + 1] += e - k;
+      k = e;
+    }
+  }
+  BuildPlan bp;
+  for (int l = 0; l < 2; ++l) {
+    for (int d = 0; d < nd; ++d) {
+      ng[l][d + 1] += ng[l][d];
+      ns[l][d + 1] += ns[l][d];
+    }
+    if (ng[l][nd] > 0x7fffffff || ns[l][nd] > 0x7fffffff) throw std::runtime_error("plan_build: too many items");
+    bp.groups[l].resize(ng[l][nd]);
+    bp.srcs[l].resize(ns[l][nd]);
+  }
+  std::string err;
+  double bytes = 0.0;
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : bytes)
+  for (int d = 0; d < nd; ++d) {
+    int64_t g[2] = {ng[0][d], ng[1][d]}, sp[2] = {ns[0][d], ns[1][d]};
+    const SparseOp* ls = nullptr;
+    const double* la = nullptr;
+    try {
+      for (int64_t k = start[d]; k < start[d + 1];) {
+        const BuildItem& h = *items[order[k]];
+        const int l = h.level;
+        bsg_build_group_t& gr = bp.groups[l][g[l]++];
+        gr.dst = const_cast<double*>(addr(h.dst)) + h.dst_off;
+        gr.ld = h.dst_ld;
+        gr.rows = h.rows;
+        gr.cols = h.cols;
+        gr.first = (int32_t)sp[l];
+        gr.pad = 0;
+        int64_t e = k;
+        for (; e < start[d + 1]; ++e) {
+          const BuildItem& it = *items[order[e]];
+          if (!same(it, h)) break;
+          if (it.rows != h.rows || it.cols != h.cols || it.dst_ld != h.dst_ld)
+            throw std::runtime_error("plan_build: overlapping destination rectangles");
+          if (it.src != ls) {
+            ls = it.src;
+            la = addr(it.src);
+          }
+          bsg_build_src_t& s = bp.srcs[l][sp[l]++];
+          s.src = la + it.src_off;
+          s.ld = it.src_ld;
+          s.trans = it.trans;
+          s.alpha = it.alpha;
+          s.beta = it.beta;
+        }
+        gr.count = (int32_t)(e - k);
+        bytes += 8.0 * h.rows * h.cols * (gr.count + 2);
+        k = e;
+      }
+    } catch (const std::exception& ex) {
+#pragma omp critical(plan_build_err)
+      if (err.empty()) err = ex.what();
+    }
+  }
+  if (!err.empty()) throw std::runtime_error(err);
+  bp.bytes = bytes;
   return bp;
 }
 
@@ -107,7 +177,7 @@ void run_build_host(const BuildPlan& bp) {
 }
 
 void execute_build_host(const BuildSession& session, const std::vector<SparseOp*>& targets) {
-  const std::vector<BuildItem> items = session.items();
+  const std::vector<const BuildItem*> items = session.item_ptrs();
   const std::vector<OpPtr> written = session.written();
   std::unordered_map<const SparseOp*, double*> where;
   std::vector<std::vector<double>> scratch;
@@ -128,7 +198,11 @@ void execute_build_host(const BuildSession& session, const std::vector<SparseOp*
     if (!o->host_valid) throw std::runtime_error("execute_build_host: source operator not in host memory");
     return o->data.data();
   };
-  run_build_host(plan_build(items, addr));
+  const double tp = now_s();
+  const BuildPlan bp = plan_build(items, addr);
+  g_build.plan += now_s() - tp;
+  g_build.items += (double)items.size();
+  run_build_host(bp);
 }
 
 void set_materialiser_context(GpuContext* g) {
@@ -147,7 +221,7 @@ std::shared_ptr<void> device_buffer(GpuContext& g, size_t n, double** ptr) {
 
 void execute_build(GpuContext& g, const BuildSession& session, const std::vector<SparseOp*>& targets) {
   const double t0 = now_s();
-  const std::vector<BuildItem> items = session.items();
+  const std::vector<const BuildItem*> items = session.item_ptrs();
   const std::vector<OpPtr> written = session.written();
   // destinations: the targets in one buffer they keep, other written
   // operators (linear-combination intermediates) in scratch
@@ -170,8 +244,11 @@ void execute_build(GpuContext& g, const BuildSession& session, const std::vector
   }
   const int64_t inter = ssize;
   std::vector<std::pair<const SparseOp*, int64_t>> up;  // host-only sources
-  for (const BuildItem& it : items) {
-    const SparseOp* s = it.src;
+  const SparseOp* seen = nullptr;
+  for (const BuildItem* it : items) {
+    const SparseOp* s = it->src;
+    if (s == seen) continue;
+    seen = s;
     if (where.count(s) || s->dev) continue;
     if (!s->host_valid) throw std::runtime_error("execute_build: source operator in neither host nor device memory");
     where[s] = nullptr;
@@ -200,11 +277,16 @@ void execute_build(GpuContext& g, const BuildSession& session, const std::vector
     if (o->dev) return o->dev;
     throw std::runtime_error("execute_build: operator without an address");
   };
+  const double tp = now_s();
   const BuildPlan bp = plan_build(items, addr);
+  const double ta = now_s();
   for (int level = 0; level < 2; ++level)
     bsg_check(bsg_build_apply(g.ctx, bp.groups[level].data(), (int32_t)bp.groups[level].size(), bp.srcs[level].data(),
                               (int32_t)bp.srcs[level].size()),
               "composite build");
+  g_build.plan += ta - tp;
+  g_build.apply += now_s() - ta;
+  g_build.items += (double)items.size();
   const double bytes = bp.bytes;
   for (auto& x : tlist) {
     x.first->data = std::vector<double>();
@@ -212,6 +294,7 @@ void execute_build(GpuContext& g, const BuildSession& session, const std::vector
     x.first->dev = tbuf + x.second;
     x.first->dev_owner = owner;
   }
+  sowner.reset();  // the scratch (intermediates, uploaded sources) goes now
   g_build.execute += now_s() - t0;
   g_build.bytes += bytes;
 }

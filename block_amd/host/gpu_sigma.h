@@ -204,7 +204,8 @@ struct BuildPlan {
   std::vector<bsg_build_src_t> srcs[2];
   double bytes = 0.0;  // algorithmic bytes (sources read, destinations read and written)
 };
-BuildPlan plan_build(const std::vector<BuildItem>& items, const std::function<const double*(const SparseOp*)>& addr);
+BuildPlan plan_build(const std::vector<const BuildItem*>& items,
+                     const std::function<const double*(const SparseOp*)>& addr);
 void run_build_host(const BuildPlan& plan);
 void execute_build_host(const BuildSession& session, const std::vector<SparseOp*>& targets);
 // Make g the context that downloads device-only operators for host readers
@@ -213,7 +214,10 @@ void set_materialiser_context(GpuContext* g);
 // Seconds spent recording and executing device builds (accumulated).
 struct BuildTimes {
   double record = 0, execute = 0;
+  double plan = 0, apply = 0;  // of execute: host grouping (plan_build), device copies and bsg_build_apply
   double bytes = 0;  // algorithmic bytes of the executed builds (sources read, destinations written)
+  double items = 0;  // recorded block copies executed
+  double download = 0, download_bytes = 0;  // device-only operators copied to the host for host readers
 };
 const BuildTimes& gpu_build_times();
 

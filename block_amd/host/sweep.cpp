@@ -300,7 +300,7 @@ std::vector<std::vector<double>> density_matrix(const SuperBlock& sb, const Solv
 // seconds spent in the phases of truncate_block (eigen-decomposition,
 // rotation of the stored operators, complementary operators built rotated)
 struct RenormTimes {
-  double eig = 0, rotate = 0, comp = 0;
+  double eig = 0, rotate = 0, comp = 0, build = 0;  // build: complementary operators built by the builder
 };
 RenormTimes g_rt;
 
@@ -317,8 +317,10 @@ BlockPtr truncate_block(const Integrals& I, const SpinBlock& B, const SpinBlock&
                         const BlockBuilder* builder = nullptr) {
   std::shared_ptr<CompOpsAhead> built;
   if (!B.has_comp && !ahead && builder) {
+    const double tb = now_s();
     built = builder->comp_ops(I, B, B1, dot, scr);
     ahead = built.get();
+    g_rt.build += now_s() - tb;
   }
   std::vector<std::vector<double>> rot;
   std::vector<int> mkeep;
@@ -542,7 +544,8 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       next_env = nullptr;
       SuperBlock sb = make_superblock(I, L, R, target);
       const long N = sb.wave->size;
-      rec.build_seconds += now_s() - ts;
+      const double tbuild = now_s() - ts;
+      rec.build_seconds += tbuild;
       // while the solve runs: L's complementary operators in its composite
       // basis (renormalisation input, handed to the rotator, which may upload
       // them beside the sigma kernels) and the next step's R = env' (x) dot
@@ -631,10 +634,10 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       if (P.verbose > 1)
         std::fprintf(stderr,
                      "  sweep %d step %d  N=%ld secL=%d secR=%d E0=%.12f  iters=%d%s  disc=%.2e  renorm: wait %.3f "
-                     "rho %.3f eig %.3f rotate %.3f comp %.3f\n",
+                     "rho %.3f eig %.3f rotate %.3f comp %.3f (built %.3f); build %.3f\n",
                      sweep, step, N, L->si->nsec(), R->si->nsec(), sol.e[0], sol.st.iterations,
                      sol.guessed ? " (guess)" : "", disc, twait, trho, g_rt.eig - rt0.eig, g_rt.rotate - rt0.rotate,
-                     g_rt.comp - rt0.comp);
+                     g_rt.comp - rt0.comp, g_rt.build - rt0.build, tbuild);
     }
     // the sweep ends at the far edge: the last block is the single-site dot
     if (forward)
@@ -648,9 +651,10 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       const BuildTimes& bt = gpu_build_times();
       std::fprintf(stderr,
                    "  sigma set-up so far: compile %.2f upload %.2f plan %.2f; renorm so far: eig %.2f rotate %.2f comp "
-                   "%.2f (build %.2f); device builds so far: record %.2f execute %.2f\n",
+                   "%.2f (build %.2f); device builds so far: record %.2f execute %.2f [grouping %.2f, device %.2f, %.3g "
+                   "items], host copies %.2f s (%.3g B)\n",
                    st.compile, st.upload, st.plan, g_rt.eig, g_rt.rotate, g_rt.comp, comp_build_seconds(), bt.record,
-                   bt.execute);
+                   bt.execute, bt.plan, bt.apply, bt.items, bt.download, bt.download_bytes);
     }
     if (P.verbose)
       std::fprintf(stderr,
