@@ -856,8 +856,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SB == 1 ? 4
     for (int t = 0; t < SB; ++t) accV[r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
   // The plan sends here only pairs with k1, k2 <= 16 SB and a non-identity A,
   // or identity A and B' (the rest of a small block's pairs run in k_sigma).
+  // The next pair's descriptor is requested before this pair's operands, so
+  // its latency hides behind this pair's loads and MFMAs (units hold at
+  // least one pair).
+  bsg_pair_t nxt = prs[sg.first];
   for (int q = 0; q < sg.count; ++q) {
-    const bsg_pair_t pr = prs[sg.first + q];
+    const bsg_pair_t pr = nxt;
+    if (q + 1 < sg.count) nxt = prs[sg.first + q + 1];
     const __amdgpu_buffer_rsrc_t rC = block_rsrc(c + pr.c_off, (int64_t)pr.k1 * pr.k2);
     const uint32_t crow = (uint32_t)pr.k2 * 8u;
     if (pr.a_off < 0) {  // identity A and B': V += alpha C

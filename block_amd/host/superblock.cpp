@@ -94,26 +94,38 @@ std::vector<double> diagonal_h(const SuperBlock& sb) {
 }
 
 double sigma_flops(const SuperBlock& sb) {
+  // walks the operators' allowed blocks instead of all sector pairs: the
+  // work is proportional to the GEMM pairs, not to terms x sectors^2
   const WaveLayout& W = *sb.wave;
   const StateInfo &ls = *W.L, &rs = *W.R;
   double fl = 0.0;
+  std::vector<std::vector<int>> acol(ls.nsec()), brow(rs.nsec());
   for (const auto& t : sb.terms) {
     bool aid = (t.a.op == sb.L->ident.get()), bid = (t.b.op == sb.R->ident.get());
     if (aid && bid) {
       fl += 2.0 * W.size;
       continue;
     }
+    // view blocks (l, l') of a by l', (r, r') of b by r'
+    for (auto& v : acol) v.clear();
+    for (auto& v : brow) v.clear();
+    for (const auto& bk : t.a.op->pat->blocks) {
+      const int l = t.a.dag ? bk.c : bk.r, lp = t.a.dag ? bk.r : bk.c;
+      acol[lp].push_back(l);
+    }
+    for (const auto& bk : t.b.op->pat->blocks) {
+      const int r = t.b.dag ? bk.c : bk.r, rp = t.b.dag ? bk.r : bk.c;
+      brow[rp].push_back(r);
+    }
     for (const auto& cb : W.blocks) {  // c block (lQ', rQ')
-      int lp = cb.l, rp = cb.r;
-      for (int l = 0; l < ls.nsec(); ++l) {
-        if (t.a.find(l, lp) < 0) continue;
-        double dl = ls.dim[l], dlp = ls.dim[lp], drp = rs.dim[rp];
+      const int lp = cb.l, rp = cb.r;
+      const double dlp = ls.dim[lp], drp = rs.dim[rp];
+      for (int l : acol[lp]) {
+        const double dl = ls.dim[l];
         if (!aid) fl += 2.0 * dl * dlp * drp;
-        for (int r = 0; r < rs.nsec(); ++r) {
-          if (W.find(l, r) < 0 || t.b.find(r, rp) < 0) continue;
-          double dr = rs.dim[r];
-          if (!bid) fl += 2.0 * dl * drp * dr;
-        }
+        if (bid) continue;
+        for (int r : brow[rp])
+          if (W.find(l, r) >= 0) fl += 2.0 * dl * drp * rs.dim[r];
       }
     }
   }

@@ -26,6 +26,10 @@ void oracle_multiply_h(const blk::SuperBlock& sb, const double* c, double* v, in
 // oracle's own loops (the timed CPU baseline; the parity tests keep the loops).
 void oracle_set_blocked_gemm(bool on);
 std::vector<double> oracle_diagonal(const blk::SuperBlock& sb);
+// Algorithmic flops of one H.C (2 dl dl' dr' per A product, 2 dl dr' dr per
+// B product, identity partners not multiplied, 2 per element for the
+// identity-identity term), by brute force over all sector quadruples.
+double oracle_sigma_flops(const blk::SuperBlock& sb);
 
 struct FciResult {
   std::vector<double> energies;

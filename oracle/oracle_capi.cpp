@@ -60,6 +60,7 @@ int orc_sigma_fast_timed(void* sb, const double* c, double* v, int nthreads, dou
   oracle::oracle_set_blocked_gemm(false);
   return rc;
 }
+double orc_sigma_flops(void* sb) { return oracle::oracle_sigma_flops(*static_cast<SuperBlock*>(sb)); }
 int orc_diag(void* sb, double* out) {
   return guarded([&]() {
     auto d = oracle::oracle_diagonal(*static_cast<SuperBlock*>(sb));

@@ -146,6 +146,30 @@ void oracle_multiply_h(const SuperBlock& sb, const double* c, double* v, int nth
     for (long i = 0; i < N; ++i) v[i] += part[t][i];
 }
 
+double oracle_sigma_flops(const SuperBlock& sb) {
+  // every (term, l, l', r, r') quadruple checked against the operators'
+  // allowed blocks and the wavefunction's: the definition, by brute force
+  const WaveLayout& W = *sb.wave;
+  const StateInfo &ls = *W.L, &rs = *W.R;
+  double fl = 0.0;
+  for (const auto& t : sb.terms) {
+    const bool aid = t.a.op == sb.L->ident.get(), bid = t.b.op == sb.R->ident.get();
+    if (aid && bid) {
+      fl += 2.0 * W.size;
+      continue;
+    }
+    for (const auto& cb : W.blocks)
+      for (int l = 0; l < ls.nsec(); ++l) {
+        if (t.a.find(l, cb.l) < 0) continue;
+        if (!aid) fl += 2.0 * ls.dim[l] * ls.dim[cb.l] * rs.dim[cb.r];
+        if (bid) continue;
+        for (int r = 0; r < rs.nsec(); ++r)
+          if (W.find(l, r) >= 0 && t.b.find(r, cb.r) >= 0) fl += 2.0 * ls.dim[l] * rs.dim[cb.r] * rs.dim[r];
+      }
+  }
+  return fl;
+}
+
 std::vector<double> oracle_diagonal(const SuperBlock& sb) {
   auto P = enumerate(sb);
   const WaveLayout& W = *sb.wave;

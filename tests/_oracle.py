@@ -19,6 +19,7 @@ for name, res, args in [
     ("orc_sigma", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int]),
     ("orc_sigma_timed", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, _dp]),
     ("orc_diag", ctypes.c_int, [_vp, _dp]),
+    ("orc_sigma_flops", ctypes.c_double, [_vp]),
     ("orc_exec_worklist", ctypes.c_int, [_vp, _dp, _dp]),
     ("orc_fci", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp]),
     ("orc_exact_superblock", _vp, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
@@ -55,6 +56,10 @@ def sigma_timed(sb, c, nthreads):
     t = ctypes.c_double()
     _chk(lib.orc_sigma_timed(sb._h, _p(c), _p(v), nthreads, ctypes.byref(t)))
     return v, t.value
+
+
+def sigma_flops(sb):
+    return lib.orc_sigma_flops(sb._h)
 
 
 def diagonal(sb):

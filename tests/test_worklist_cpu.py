@@ -20,3 +20,17 @@ def test_worklist_equals_direct(case):
     ref = _oracle.sigma(sb, c, nthreads=4)
     got = _oracle.exec_worklist(sb, c)
     assert np.linalg.norm(got - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("case", ["hubbard12", "syn10", "syn10d2h", "syn10_S1"])
+def test_sigma_flops_equals_brute_force(case):
+    """The flop count the sweep records (walks the operators' allowed blocks)
+    equals the brute-force count over every sector quadruple."""
+    twoS = 2 if case.endswith("S1") else 0
+    if case == "hubbard12":
+        sb = block_amd.SuperBlock(config_fcidump("hubbard12"), 12, 0, 0, 5, 40)
+    else:
+        sb = block_amd.SuperBlock(random_fcidump(10, 11, d2h="d2h" in case), 10, twoS, 0, 4, 40)
+    want = _oracle.sigma_flops(sb)
+    assert want > 0
+    assert sb.flops == pytest.approx(want, rel=1e-15)
