@@ -339,7 +339,7 @@ def fetch_val(d, name):
 TUNE_EDGES = [(1, 32), (33, 64), (65, 128), (129, 256), (257, 512), (513, 1 << 30)]
 
 
-def tune_tiles(sb, out_path, steps, warmup, top=6, min_gain=0.02):
+def tune_tiles(sb, out_path, steps, warmup, top=6, min_gain=0.005):
     """Measured tile table: the V blocks are bucketed by (rows, cols) ranges;
     for the buckets with most of the flops (about m n (m + n) per block),
     every candidate geometry of the FP64 and of the split-precision kernels
@@ -371,7 +371,7 @@ def tune_tiles(sb, out_path, steps, warmup, top=6, min_gain=0.02):
     def time_with():
         block_amd.set_tile_table(rules())
         h.replan()
-        return min(h.timed(c, warmup, steps)[0] for _ in range(2)) / steps
+        return min(h.timed(c, warmup, steps)[0] for _ in range(3)) / steps
 
     log_rows = []
     for slot, prec in ((0, block_amd.FP64), (1, block_amd.SPLIT_BF16)):
@@ -425,6 +425,8 @@ def main():
                     help="GPU time by kernel and copy direction of a profiled run (no GPU run)")
     ap.add_argument("--summarize-traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
                     help="HBM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (no GPU run)")
+    ap.add_argument("--settle", type=int, default=None,
+                    help="noise-free sweeps at M after the ramp, before the timed sweep (default: the config's)")
     ap.add_argument("--tune", metavar="OUT", help="measure the tile table on this superblock and write it to OUT")
     ap.add_argument("--verbose", type=int, default=1, help="sweep log level (2: per step)")
     ap.add_argument("--last", type=int, default=6, help="dispatches per kernel to summarise (the last ones)")
@@ -446,6 +448,12 @@ def main():
     M = args.M or cfg["M"]
     if args.M:
         cfg = dict(cfg, ramp=[(s, min(m, M), t, n) for (s, m, t, n) in cfg["ramp"]])
+    settle = cfg.get("settle", 0) if args.settle is None else args.settle
+    if settle > 0:
+        # noise-free sweeps at M that converge the state before the timed sweep
+        n0 = cfg["ramp_sweeps"]
+        cfg = dict(cfg, ramp=[tuple(r) for r in cfg["ramp"]] + [(n0, M, cfg.get("settle_tol", 1e-10), 0.0)],
+                   ramp_sweeps=n0 + settle)
     norb = cfg["norb"]
     tmp = tempfile.mkdtemp(prefix="blkbench_")
     fcidump = os.path.join(tmp, f"{args.config}.FCIDUMP")

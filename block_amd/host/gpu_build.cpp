@@ -50,26 +50,39 @@ BuildPlan plan_build(const std::vector<const BuildItem*>& items,
   // write disjoint rectangles, so their order does not change the result.
   const size_t n = items.size();
   if (n > 0x7fffffff) throw std::runtime_error("plan_build: too many items");
-  std::unordered_map<const SparseOp*, int> dix;
-  std::vector<int> di(n);
-  int nd = 0;
-  const SparseOp* last = nullptr;
-  int lastix = -1;
+  // destination and source operators numbered once (addresses looked up
+  // once per operator, not per item)
+  std::unordered_map<const SparseOp*, int> dix, six;
+  std::vector<const SparseOp*> dops, sops;
+  std::vector<int, UninitAlloc<int>> di(n), si(n);
+  const SparseOp *last = nullptr, *lasts = nullptr;
+  int lastix = -1, lastsx = -1;
   for (size_t i = 0; i < n; ++i) {
     const BuildItem& it = *items[i];
     if (it.level < 0 || it.level > 1) throw std::runtime_error("plan_build: unknown item level");
     if (it.dst != last) {
-      auto f = dix.emplace(it.dst, nd);
-      if (f.second) ++nd;
+      auto f = dix.emplace(it.dst, (int)dops.size());
+      if (f.second) dops.push_back(it.dst);
       last = it.dst;
       lastix = f.first->second;
     }
+    if (it.src != lasts) {
+      auto f = six.emplace(it.src, (int)sops.size());
+      if (f.second) sops.push_back(it.src);
+      lasts = it.src;
+      lastsx = f.first->second;
+    }
     di[i] = lastix;
+    si[i] = lastsx;
   }
+  const int nd = (int)dops.size();
+  std::vector<const double*> daddr(nd), saddr(sops.size());
+  for (int d = 0; d < nd; ++d) daddr[d] = addr(dops[d]);
+  for (size_t k = 0; k < sops.size(); ++k) saddr[k] = addr(sops[k]);
   std::vector<int64_t> start(nd + 1, 0);
   for (size_t i = 0; i < n; ++i) ++start[di[i] + 1];
   for (int d = 0; d < nd; ++d) start[d + 1] += start[d];
-  std::vector<int> order(n);
+  std::vector<int, UninitAlloc<int>> order(n);
   {
     std::vector<int64_t> pos(start.begin(), start.end() - 1);
     for (size_t i = 0; i < n; ++i) order[pos[di[i]]++] = (int)i;
@@ -111,14 +124,12 @@ BuildPlan plan_build(const std::vector<const BuildItem*>& items,
 #pragma omp parallel for schedule(dynamic, 8) reduction(+ : bytes)
   for (int d = 0; d < nd; ++d) {
     int64_t g[2] = {ng[0][d], ng[1][d]}, sp[2] = {ns[0][d], ns[1][d]};
-    const SparseOp* ls = nullptr;
-    const double* la = nullptr;
     try {
       for (int64_t k = start[d]; k < start[d + 1];) {
         const BuildItem& h = *items[order[k]];
         const int l = h.level;
         bsg_build_group_t& gr = bp.groups[l][g[l]++];
-        gr.dst = const_cast<double*>(addr(h.dst)) + h.dst_off;
+        gr.dst = const_cast<double*>(daddr[d]) + h.dst_off;
         gr.ld = h.dst_ld;
         gr.rows = h.rows;
         gr.cols = h.cols;
@@ -126,16 +137,13 @@ BuildPlan plan_build(const std::vector<const BuildItem*>& items,
         gr.pad = 0;
         int64_t e = k;
         for (; e < start[d + 1]; ++e) {
-          const BuildItem& it = *items[order[e]];
+          const int ie = order[e];
+          const BuildItem& it = *items[ie];
           if (!same(it, h)) break;
           if (it.rows != h.rows || it.cols != h.cols || it.dst_ld != h.dst_ld)
             throw std::runtime_error("plan_build: overlapping destination rectangles");
-          if (it.src != ls) {
-            ls = it.src;
-            la = addr(it.src);
-          }
           bsg_build_src_t& s = bp.srcs[l][sp[l]++];
-          s.src = la + it.src_off;
+          s.src = saddr[si[ie]] + it.src_off;
           s.ld = it.src_ld;
           s.trans = it.trans;
           s.alpha = it.alpha;

@@ -199,9 +199,29 @@ void execute_build(GpuContext& g, const BuildSession& session, const std::vector
 // the host (k_build's loop), execute_build_host the whole recorded build in
 // host memory (the targets become host operators).  They check the
 // recording and the grouping on the CPU.
+// allocator whose resize leaves trivial elements uninitialised (the plan
+// writes every element afterwards, in parallel)
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = UninitAlloc<U>;
+  };
+  UninitAlloc() = default;
+  template <class U>
+  UninitAlloc(const UninitAlloc<U>&) {}
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new ((void*)p) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
 struct BuildPlan {
-  std::vector<bsg_build_group_t> groups[2];
-  std::vector<bsg_build_src_t> srcs[2];
+  std::vector<bsg_build_group_t, UninitAlloc<bsg_build_group_t>> groups[2];
+  std::vector<bsg_build_src_t, UninitAlloc<bsg_build_src_t>> srcs[2];
   double bytes = 0.0;  // algorithmic bytes (sources read, destinations read and written)
 };
 BuildPlan plan_build(const std::vector<const BuildItem*>& items,
