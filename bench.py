@@ -339,7 +339,7 @@ def fetch_val(d, name):
 TUNE_EDGES = [(1, 32), (33, 64), (65, 128), (129, 256), (257, 512), (513, 1 << 30)]
 
 
-def tune_tiles(sb, out_path, steps, warmup, top=6, min_gain=0.005):
+def tune_tiles(sb, out_path, steps, warmup, label, top=6, min_gain=0.005):
     """Measured tile table: the V blocks are bucketed by (rows, cols) ranges;
     for the buckets with most of the flops (about m n (m + n) per block),
     every candidate geometry of the FP64 and of the split-precision kernels
@@ -398,7 +398,8 @@ def tune_tiles(sb, out_path, steps, warmup, top=6, min_gain=0.005):
         raise RuntimeError(f"tuned plan changed sigma: relative difference {err:.2e}")
     with open(out_path, "w") as f:
         f.write("# m_lo m_hi n_lo n_hi fp64 split: tile geometry per V block shape range (-1: MFMA-count model)\n")
-        f.write(f"# measured by bench.py --tune on {sb.wave_size}-element superblock, sigma ms per bucket trial\n")
+        f.write(f"# measured by bench.py --tune on {label} ({sb.wave_size}-element superblock): the whole sigma\n"
+                f"# timed per candidate (min of 3 x {steps} applications), kept if >{min_gain:.1%} faster\n")
         for r in rules():
             f.write(" ".join(str(x) for x in r) + "\n")
     return {"rules": rules(), "trials": log_rows, "sigma_rel_diff_tuned_vs_model": err}
@@ -472,7 +473,8 @@ def main():
     sb = block_amd.SuperBlock.from_sweep_state(ints, ck, cfg["nelec"], 0, ints.isym)
     t_build = time.time() - t0
     if args.tune:
-        print(json.dumps({"tune": tune_tiles(sb, args.tune, args.steps, args.warmup)}))
+        label = f"--config {args.config}" + (f" --M {args.M}" if args.M else "")
+        print(json.dumps({"tune": tune_tiles(sb, args.tune, args.steps, args.warmup, label)}))
         return 0
     h = block_amd.multiply_h(sb, device=0)
     c = np.random.default_rng(0).standard_normal(sb.wave_size)
@@ -548,7 +550,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.config}: one H.C on the mid-chain two-site superblock, {norb} orbitals / "
                                f"{cfg['nelec']} electrons, S=0, {'D2h' if cfg.get('d2h') else 'C1'}, M={M}, blocks "
-                               f"from a noisy two-site sweep ramp, FP64 sigma",
+                               f"from a noisy two-site sweep ramp"
+                               f"{f' and {settle} noise-free sweeps' if settle else ''}, FP64 sigma",
                    "wave_size": sb.wave_size, "flop_per_step": flops, "terms": sb.nterms,
                    "sectors_L_R_blocks": [nl, nr, nb], "largest_V_block": max(shapes, key=lambda s: s[0] * s[1]),
                    "gemm_pairs": pairs, "operator_arena_GB": arena_gb,

@@ -73,7 +73,9 @@ def synthetic(n, seed, d2h=False, nchol=72):
 # Benchmark configurations (seeds are part of the definition).  ramp: the
 # sweep schedule (start sweep, M, Davidson tolerance on the squared residual
 # norm, density-matrix noise) that grows the blocks to M, ramp_sweeps sweeps
-# long; timed_tol: the Davidson tolerance of the timed sweeps that follow.
+# long; settle: noise-free sweeps at M (Davidson tolerance settle_tol) that
+# converge the state after the ramp; timed_tol: the Davidson tolerance of the
+# timed sweeps that follow.
 _RAMP = {
     500: [(0, 250, 1e-7, 1e-5), (2, 500, 1e-8, 1e-6)],
     800: [(0, 200, 1e-6, 1e-4), (2, 400, 1e-7, 1e-5), (4, 800, 1e-8, 1e-6)],
@@ -86,7 +88,7 @@ CONFIGS = {
     "syn24": dict(kind="synthetic", norb=24, nelec=24, M=500, seed=2024, d2h=False, ramp=_RAMP[500], ramp_sweeps=4,
                   timed_tol=1e-12),
     "syn36d2h": dict(kind="synthetic", norb=36, nelec=36, M=1000, seed=2036, d2h=True, ramp=_RAMP[1000],
-                     ramp_sweeps=4, timed_tol=1e-12),
+                     ramp_sweeps=4, settle=4, settle_tol=1e-10, timed_tol=1e-12),
     "syn30": dict(kind="synthetic", norb=30, nelec=30, M=800, seed=2030, d2h=False, nroots=4, ramp=_RAMP[800],
                   ramp_sweeps=6, timed_tol=1e-12),
     "syn50": dict(kind="synthetic", norb=50, nelec=50, M=2000, seed=2050, d2h=False, ramp=_RAMP[2000],
