@@ -135,6 +135,27 @@ WorkList compile_worklist(const SuperBlock& sb) {
     toff_a[it] = wl.arena_off.at(sb.terms[it].a.op);
     toff_b[it] = wl.arena_off.at(sb.terms[it].b.op);
   }
+  // each term's a-view blocks by view row l: (l', block id), l' ascending
+  // (pattern blocks are stored in (row, col) order), so a V block visits
+  // only the allowed (l, l') instead of every l'
+  const int nls = ls.nsec();
+  std::vector<std::vector<int>> arow_start(sb.terms.size());
+  std::vector<std::vector<std::pair<int, int>>> arow(sb.terms.size());
+#pragma omp parallel for schedule(dynamic)
+  for (long it = 0; it < (long)sb.terms.size(); ++it) {
+    const OpView& a = sb.terms[it].a;
+    const auto& blks = a.op->pat->blocks;
+    std::vector<int>& st = arow_start[it];
+    st.assign(nls + 1, 0);
+    for (const auto& bk : blks) ++st[(a.dag ? bk.c : bk.r) + 1];
+    for (int q = 0; q < nls; ++q) st[q + 1] += st[q];
+    std::vector<int> pos(st.begin(), st.end() - 1);
+    arow[it].resize(blks.size());
+    for (size_t ib = 0; ib < blks.size(); ++ib) {
+      const int row = a.dag ? blks[ib].c : blks[ib].r, col = a.dag ? blks[ib].r : blks[ib].c;
+      arow[it][pos[row]++] = {col, (int)ib};
+    }
+  }
   std::vector<std::vector<bsg_pair_t>> per_v(W.blocks.size());
   std::vector<std::vector<WorkList::Prov>> per_p(W.blocks.size());
 #pragma omp parallel for schedule(dynamic)
@@ -143,9 +164,8 @@ WorkList compile_worklist(const SuperBlock& sb) {
     for (size_t it = 0; it < sb.terms.size(); ++it) {
       const auto& t = sb.terms[it];
       const bool aid = (t.a.op == idL), bid = (t.b.op == idR);
-      for (int lp = 0; lp < ls.nsec(); ++lp) {
-        const int ia = t.a.find(l, lp);
-        if (ia < 0) continue;
+      for (int e = arow_start[it][l]; e < arow_start[it][l + 1]; ++e) {
+        const int lp = arow[it][e].first, ia = arow[it][e].second;
         for (int ic : cblocks[lp]) {
           const auto& cb = W.blocks[ic];
           const int rp = cb.r;
