@@ -339,67 +339,82 @@ def fetch_val(d, name):
 TUNE_EDGES = [(1, 32), (33, 64), (65, 128), (129, 256), (257, 512), (513, 1 << 30)]
 
 
-def tune_tiles(sb, out_path, steps, warmup, label, top=6, min_gain=0.005):
-    """Measured tile table: the V blocks are bucketed by (rows, cols) ranges;
-    for the buckets with most of the flops (about m n (m + n) per block),
-    every candidate geometry of the FP64 and of the split-precision kernels
-    is timed on this superblock's whole sigma (the other buckets on their
-    current choice), and a bucket keeps the fastest candidate when it beats
-    the MFMA-count model's choice by min_gain.  Writes the rules to out_path
-    and checks that the tuned plan gives the same sigma."""
-    shapes = sb.block_shapes()
+def tune_tiles(sbs, out_path, steps, warmup, label, top=8, min_gain=0.005):
+    """Measured tile table over one or more superblocks: the V blocks are
+    bucketed by (rows, cols) ranges; for the buckets with most of the flops
+    (about m n (m + n) per block, each superblock's share normalised), every
+    candidate geometry of the FP64 and of the split-precision kernels is
+    timed on the whole sigma of every superblock (the other buckets on their
+    current choice), scored by the sum over superblocks of the time relative
+    to the MFMA-count model's, and a bucket keeps the best candidate when it
+    beats the model by min_gain.  Writes the rules to out_path and checks
+    that the tuned plans give the same sigma."""
     weight = {}
-    for m, n in shapes:
-        if max(m, n) <= 32:
-            continue
-        bm = next(i for i, (lo, hi) in enumerate(TUNE_EDGES) if lo <= m <= hi)
-        bn = next(i for i, (lo, hi) in enumerate(TUNE_EDGES) if lo <= n <= hi)
-        weight[(bm, bn)] = weight.get((bm, bn), 0.0) + float(m) * n * (m + n)
+    for sb in sbs:
+        w = {}
+        for m, n in sb.block_shapes():
+            if max(m, n) <= 32:
+                continue
+            bm = next(i for i, (lo, hi) in enumerate(TUNE_EDGES) if lo <= m <= hi)
+            bn = next(i for i, (lo, hi) in enumerate(TUNE_EDGES) if lo <= n <= hi)
+            w[(bm, bn)] = w.get((bm, bn), 0.0) + float(m) * n * (m + n)
+        tot = sum(w.values()) or 1.0
+        for k, x in w.items():
+            weight[k] = weight.get(k, 0.0) + x / tot / len(sbs)
     buckets = sorted(weight, key=lambda b: -weight[b])[:top]
-    total = sum(weight.values()) or 1.0
-    h = block_amd.multiply_h(sb)
-    c = np.random.default_rng(1).standard_normal(sb.wave_size)
+    hs = [block_amd.multiply_h(sb) for sb in sbs]
+    cs = [np.random.default_rng(1).standard_normal(sb.wave_size) for sb in sbs]
     block_amd.set_tile_table([])
-    h.replan()
-    v0 = h(c)
+    for h in hs:
+        h.replan()
+    v0 = [h(c) for h, c in zip(hs, cs)]
     choice = {b: [-1, -1] for b in buckets}
 
     def rules():
         return [(TUNE_EDGES[bm][0], TUNE_EDGES[bm][1], TUNE_EDGES[bn][0], TUNE_EDGES[bn][1], f, sp)
                 for (bm, bn), (f, sp) in choice.items() if f >= 0 or sp >= 0]
 
-    def time_with():
+    def times():
         block_amd.set_tile_table(rules())
-        h.replan()
-        return min(h.timed(c, warmup, steps)[0] for _ in range(3)) / steps
+        out = []
+        for h, c in zip(hs, cs):
+            h.replan()
+            out.append(min(h.timed(c, warmup, steps)[0] for _ in range(3)) / steps)
+        return out
 
     log_rows = []
     for slot, prec in ((0, block_amd.FP64), (1, block_amd.SPLIT_BF16)):
-        h.precision = prec
+        for h in hs:
+            h.precision = prec
         for b in buckets:
             choice[b][slot] = -1
-            base = time_with()
-            best, best_ms = -1, base
+            base = times()
+            score = lambda t: sum(x / y for x, y in zip(t, base)) / len(base)
+            best, best_s, best_t = -1, 1.0, base
             for cand in range(block_amd.tile_candidates(prec)):
                 choice[b][slot] = cand
-                ms = time_with()
-                if ms < best_ms:
-                    best, best_ms = cand, ms
-            choice[b][slot] = best if best_ms < base * (1.0 - min_gain) else -1
-            log_rows.append((slot, b, round(weight[b] / total, 3), round(base, 4), best, round(best_ms, 4)))
+                t = times()
+                if score(t) < best_s:
+                    best, best_s, best_t = cand, score(t), t
+            choice[b][slot] = best if best_s < 1.0 - min_gain else -1
+            log_rows.append((slot, b, round(weight[b], 3), [round(x, 4) for x in base], best,
+                             [round(x, 4) for x in best_t]))
             log(f"tune {'fp64' if slot == 0 else 'split'} bucket m {TUNE_EDGES[b[0]]} n {TUNE_EDGES[b[1]]} "
-                f"({weight[b] / total:.1%} of flops): model {base:.4f} ms, best candidate {best} {best_ms:.4f} ms")
-    h.precision = block_amd.FP64
+                f"({weight[b]:.1%} of flops): model {[round(x, 4) for x in base]} ms, best candidate {best} "
+                f"{[round(x, 4) for x in best_t]} ms")
     block_amd.set_tile_table(rules())
-    h.replan()
-    v1 = h(c)
-    err = float(np.linalg.norm(v1 - v0) / np.linalg.norm(v0))
+    err = 0.0
+    for h, c, v in zip(hs, cs, v0):
+        h.precision = block_amd.FP64
+        h.replan()
+        err = max(err, float(np.linalg.norm(h(c) - v) / np.linalg.norm(v)))
     if err > 1e-12:
         raise RuntimeError(f"tuned plan changed sigma: relative difference {err:.2e}")
     with open(out_path, "w") as f:
         f.write("# m_lo m_hi n_lo n_hi fp64 split: tile geometry per V block shape range (-1: MFMA-count model)\n")
-        f.write(f"# measured by bench.py --tune on {label} ({sb.wave_size}-element superblock): the whole sigma\n"
-                f"# timed per candidate (min of 3 x {steps} applications), kept if >{min_gain:.1%} faster\n")
+        f.write(f"# measured by bench.py --tune on {label} ({', '.join(str(sb.wave_size) for sb in sbs)}-element\n"
+                f"# superblocks): every superblock's whole sigma timed per candidate (min of 3 x {steps}\n"
+                f"# applications), kept if the mean time relative to the model's is >{min_gain:.1%} lower\n")
         for r in rules():
             f.write(" ".join(str(x) for x in r) + "\n")
     return {"rules": rules(), "trials": log_rows, "sigma_rel_diff_tuned_vs_model": err}
@@ -429,6 +444,8 @@ def main():
     ap.add_argument("--settle", type=int, default=None,
                     help="noise-free sweeps at M after the ramp, before the timed sweep (default: the config's)")
     ap.add_argument("--tune", metavar="OUT", help="measure the tile table on this superblock and write it to OUT")
+    ap.add_argument("--tune-with", default="", metavar="CONFIGS",
+                    help="comma-separated further configurations whose superblocks the tuner also times")
     ap.add_argument("--verbose", type=int, default=1, help="sweep log level (2: per step)")
     ap.add_argument("--last", type=int, default=6, help="dispatches per kernel to summarise (the last ones)")
     ap.add_argument("--kernels", default="k_sigma|k_small|k_reduce", help="regex of the kernel names to summarise")
@@ -473,8 +490,19 @@ def main():
     sb = block_amd.SuperBlock.from_sweep_state(ints, ck, cfg["nelec"], 0, ints.isym)
     t_build = time.time() - t0
     if args.tune:
-        label = f"--config {args.config}" + (f" --M {args.M}" if args.M else "")
-        print(json.dumps({"tune": tune_tiles(sb, args.tune, args.steps, args.warmup, label)}))
+        sbs = [sb]
+        for extra in [x for x in args.tune_with.split(",") if x]:
+            ecfg = gen.CONFIGS[extra]
+            epath = os.path.join(tmp, f"{extra}.FCIDUMP")
+            gen.make_config_fcidump(extra, epath)
+            eints = block_amd.Integrals(epath)
+            eck = os.path.join(tmp, f"{extra}.state")
+            log(f"ramp on {extra} for the tuner")
+            ramp_leg(eints, ecfg, ecfg["M"], eck, args.verbose)
+            sbs.append(block_amd.SuperBlock.from_sweep_state(eints, eck, ecfg["nelec"], 0, eints.isym))
+        label = f"--config {args.config}" + (f" --M {args.M}" if args.M else "") + \
+            (f" --tune-with {args.tune_with}" if args.tune_with else "")
+        print(json.dumps({"tune": tune_tiles(sbs, args.tune, args.steps, args.warmup, label)}))
         return 0
     h = block_amd.multiply_h(sb, device=0)
     c = np.random.default_rng(0).standard_normal(sb.wave_size)
