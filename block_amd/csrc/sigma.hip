@@ -22,6 +22,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -1388,7 +1389,13 @@ struct bsg_context {
 struct PlanChunk {
   bsg_vblock_t* d_vb = nullptr;
   bsg_pair_t* d_pr = nullptr;
-  PairS* d_ps = nullptr;  // slice offsets of the grouped kernel's pairs (split precision)
+  void* d_blob = nullptr;  // one device allocation holding the arrays below (not d_ps)
+  PairS* d_ps = nullptr;  // slice offsets of the grouped kernel's pairs (split precision, built lazily)
+  // host copies for the slice recipes: pairs in device order, the V block of
+  // each grouped-kernel pair (-1: small-block kernel), the V blocks
+  std::vector<bsg_pair_t> h_perm;
+  std::vector<int> h_vbi;
+  std::vector<bsg_vblock_t> h_vb;
   Seg* d_seg = nullptr;
   TileRef* d_g[NCLS] = {};  // FP64 grouped-kernel tiles per kernel class
   int n_g[NCLS] = {};
@@ -1407,6 +1414,7 @@ struct bsg_plan {
   int64_t part_doubles = 0;  // partial tiles of split V blocks per root (largest chunk)
   int32_t batch = 1;         // roots per launch (bsg_plan_set_batch)
   bool cs_ready = false;     // d_cs holds C slices for `batch` roots
+  bool slices_built = false; // slice recipes and pair slice offsets (first bsg_plan_split_arena)
   int64_t wave_size = 0;
   int64_t arena_extent = 0;  // doubles of operator arena the pairs reach
   double flops = 0.0, flops1 = 0.0, flops2 = 0.0;
@@ -1573,38 +1581,11 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     bsg_plan_destroy(p);
     return fail(code, m);
   };
-  // BF16 slice recipes, one per distinct (block, orientation)
-  std::map<std::tuple<int64_t, int, int, int>, int64_t> ab_map;
-  std::map<int64_t, int64_t> c_map;
-  auto slice_of = [&](std::vector<SliceItem>& items, int64_t& elems, int64_t src, int rows, int cols, int trans) {
-    SliceItem it;
-    it.src = src;
-    it.dst = elems;
-    it.rows = rows;
-    it.cols = cols;
-    it.pitch = (cols + 7) & ~7;
-    it.trans = trans;
-    items.push_back(it);
-    elems += 3 * (int64_t)rows * it.pitch;
-    return it.dst;
-  };
-  auto ab_slice = [&](int64_t off, int trans, int rows, int cols) {
-    auto key = std::make_tuple(off, trans, rows, cols);
-    auto f = ab_map.find(key);
-    if (f != ab_map.end()) return f->second;
-    return ab_map[key] = slice_of(p->ab_items, p->ab_elems, off, rows, cols, trans);
-  };
-  auto c_slice = [&](int64_t off, int k1, int k2) {
-    auto f = c_map.find(off);
-    if (f != c_map.end()) return f->second;
-    return c_map[off] = slice_of(p->c_items, p->c_elems, off, k2, k1, 1);  // C^T: (j, p) = C[p k2 + j]
-  };
   for (int ci = 0; ci < n_chunks; ++ci) {
     const bsg_chunk_t& ch = chunks[ci];
     PlanChunk pc;
     std::vector<GUnit> gunits[NCLS], gunits_bf[3];  // the tiles of each work unit (FP64, split)
     std::vector<bsg_pair_t> perm;  // pairs in device order (grouped per V block as given)
-    std::vector<PairS> pslices;    // their slice offsets
     std::vector<std::pair<long, int>> sunits[2];  // (cost, segment) of small-block units
     std::vector<Seg> segs;
     std::vector<RedItem> reds[2];  // ordered reductions after the FP64 / the split-precision kernels
@@ -1646,18 +1627,11 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       const int base = (int)perm.size();
       for (int x : fast) {
         perm.push_back(ch.pairs[x]);
-        pslices.push_back({-1, -1, -1});
+        pc.h_vbi.push_back(-1);  // the small-block kernel reads no slices
       }
       for (int x : slow) {
-        const auto& pr = ch.pairs[x];
-        perm.push_back(pr);
-        PairS q{-1, -1, -1};
-        if (pr.a_off >= 0) {
-          q.a = ab_slice(pr.a_off, pr.a_trans, vb.m, pr.k1);  // op(A)(i, p), K = p contiguous
-          q.c = c_slice(pr.c_off, pr.k1, pr.k2);
-        }
-        if (pr.b_off >= 0) q.b = ab_slice(pr.b_off, pr.b_trans ? 0 : 1, vb.n, pr.k2);  // B'(k, col) as [col][k]
-        pslices.push_back(q);
+        perm.push_back(ch.pairs[x]);
+        pc.h_vbi.push_back(i);
       }
       // cut each list into work units of about SEG_COST(_SMALL) MFMAs per
       // wave.  A block on wide tiles is cut twice: for the FP64 wide tiles
@@ -1736,23 +1710,22 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       }
     }
     pmax = std::max(pmax, poff);
-    auto up = [&](void** d, const void* h, size_t bytes) -> int {
-      if (bytes == 0) return BSG_OK;
-      HIPCHK(hipMalloc(d, bytes));
-      HIPCHK(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
-      return BSG_OK;
+    // every array of the chunk goes into one device allocation with one copy
+    // (256-byte aligned parts): one hipMalloc / hipFree per chunk
+    std::vector<std::pair<void**, size_t>> parts;  // (device pointer to set, offset in the blob)
+    std::vector<char> blob;
+    auto up = [&](void** d, const void* h, size_t bytes) {
+      if (bytes == 0) return;
+      const size_t off = (blob.size() + 255) & ~(size_t)255;
+      blob.resize(off + bytes);
+      std::memcpy(blob.data() + off, h, bytes);
+      parts.push_back({d, off});
     };
-    int rc = BSG_OK;
-    if ((rc = up((void**)&pc.d_vb, ch.vblocks, sizeof(bsg_vblock_t) * ch.n_vblocks)) ||
-        (rc = up((void**)&pc.d_pr, perm.data(), sizeof(bsg_pair_t) * perm.size())) ||
-        (rc = up((void**)&pc.d_ps, pslices.data(), sizeof(PairS) * pslices.size())) ||
-        (rc = up((void**)&pc.d_seg, segs.data(), sizeof(Seg) * segs.size())) ||
-        (rc = up((void**)&pc.d_red[0], reds[0].data(), sizeof(RedItem) * reds[0].size())) ||
-        (rc = up((void**)&pc.d_red[1], reds[1].data(), sizeof(RedItem) * reds[1].size()))) {
-      p->chunks.push_back(pc);
-      std::string m = g_err;
-      return cleanup(rc, m);
-    }
+    up((void**)&pc.d_vb, ch.vblocks, sizeof(bsg_vblock_t) * ch.n_vblocks);
+    up((void**)&pc.d_pr, perm.data(), sizeof(bsg_pair_t) * perm.size());
+    up((void**)&pc.d_seg, segs.data(), sizeof(Seg) * segs.size());
+    up((void**)&pc.d_red[0], reds[0].data(), sizeof(RedItem) * reds[0].size());
+    up((void**)&pc.d_red[1], reds[1].data(), sizeof(RedItem) * reds[1].size());
     std::vector<TileRef> gtiles[NCLS];
     for (int k = 0; k < NCLS; ++k) gtiles[k] = xcd_order(gunits[k]);
     std::vector<TileRef> gtiles_bf[3] = {xcd_order(gunits_bf[0]), xcd_order(gunits_bf[1]), xcd_order(gunits_bf[2])};
@@ -1766,43 +1739,39 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         for (int j = 0; j < 4; ++j) e[j] = i + j < sunits[k].size() ? sunits[k][i + j].second : -1;
         sq[k].push_back(make_int4(e[0], e[1], e[2], e[3]));
       }
-      if ((rc = up((void**)&pc.d_s[k], sq[k].data(), sizeof(int4) * sq[k].size()))) break;
+      up((void**)&pc.d_s[k], sq[k].data(), sizeof(int4) * sq[k].size());
       pc.n_s[k] = (int)sq[k].size();
     }
-    if (rc) {
-      p->chunks.push_back(pc);
-      std::string m = g_err;
-      return cleanup(rc, m);
-    }
-    for (int k = 0; k < NCLS && !rc; ++k) {
-      rc = up((void**)&pc.d_g[k], gtiles[k].data(), sizeof(TileRef) * gtiles[k].size());
+    for (int k = 0; k < NCLS; ++k) {
+      up((void**)&pc.d_g[k], gtiles[k].data(), sizeof(TileRef) * gtiles[k].size());
       pc.n_g[k] = (int)gtiles[k].size();
     }
-    for (int k = 0; k < 3 && !rc; ++k) {
-      rc = up((void**)&pc.d_gbf[k], gtiles_bf[k].data(), sizeof(TileRef) * gtiles_bf[k].size());
+    for (int k = 0; k < 3; ++k) {
+      up((void**)&pc.d_gbf[k], gtiles_bf[k].data(), sizeof(TileRef) * gtiles_bf[k].size());
       pc.n_gbf[k] = (int)gtiles_bf[k].size();
     }
-    if (rc) {
-      p->chunks.push_back(pc);
-      std::string m = g_err;
-      return cleanup(rc, m);
+    if (!blob.empty()) {
+      char* d = nullptr;
+      if (hipMalloc((void**)&d, blob.size()) != hipSuccess ||
+          hipMemcpy(d, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        p->chunks.push_back(std::move(pc));
+        return cleanup(BSG_ERR_NOMEM, "bsg_plan_create: plan arrays");
+      }
+      pc.d_blob = d;
+      for (auto& x : parts) *x.first = d + x.second;
     }
     pc.n_red[0] = (int)reds[0].size();
     pc.n_red[1] = (int)reds[1].size();
-    p->chunks.push_back(pc);
+    pc.h_perm = std::move(perm);
+    pc.h_vb.assign(ch.vblocks, ch.vblocks + ch.n_vblocks);
+    p->chunks.push_back(std::move(pc));
   }
   p->part_doubles = pmax;
   if (pmax > 0) {
     hipError_t e = hipMalloc(&p->part, sizeof(double) * pmax * p->batch);
     if (e != hipSuccess) return cleanup(BSG_ERR_NOMEM, "bsg_plan_create: partial-tile scratch");
   }
-  for (auto* v : {&p->ab_items, &p->c_items})
-    if (!v->empty()) {
-      SliceItem** d = v == &p->ab_items ? &p->d_ab_items : &p->d_c_items;
-      if (hipMalloc((void**)d, sizeof(SliceItem) * v->size()) != hipSuccess ||
-          hipMemcpy(*d, v->data(), sizeof(SliceItem) * v->size(), hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(BSG_ERR_NOMEM, "bsg_plan_create: slice recipes");
-    }
   *plan = p;
   return BSG_OK;
 }
@@ -1811,16 +1780,8 @@ int bsg_plan_destroy(bsg_plan* p) {
   if (!p) return BSG_OK;
   if (p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
   for (auto& c : p->chunks) {
-    (void)hipFree(c.d_vb);
-    (void)hipFree(c.d_pr);
+    (void)hipFree(c.d_blob);  // every array but the lazily built slice offsets
     (void)hipFree(c.d_ps);
-    (void)hipFree(c.d_seg);
-    (void)hipFree(c.d_red[0]);
-    (void)hipFree(c.d_red[1]);
-    for (int k = 0; k < NCLS; ++k) (void)hipFree(c.d_g[k]);
-    for (int k = 0; k < 3; ++k) (void)hipFree(c.d_gbf[k]);
-    (void)hipFree(c.d_s[0]);
-    (void)hipFree(c.d_s[1]);
   }
   (void)hipFree(p->part);
   (void)hipFree(p->d_ab_items);
@@ -1835,10 +1796,90 @@ int bsg_plan_destroy(bsg_plan* p) {
 int64_t bsg_plan_arena_extent(const bsg_plan* p) { return p ? p->arena_extent : 0; }
 int64_t bsg_plan_partial_doubles(const bsg_plan* p) { return p ? p->part_doubles : 0; }
 
+}  // extern "C"
+
+namespace {
+// BF16 slice recipes of a plan, one per distinct (operator block,
+// orientation) and per C block its grouped-kernel pairs read, and the pairs'
+// slice offsets; made on the first split-precision set-up, so FP64-only
+// plans (the rotations, FP64 solves) never pay for them.
+int build_slices(bsg_plan* p) {
+  struct Key {
+    int64_t off;
+    int trans, rows, cols;
+    bool operator==(const Key& o) const { return off == o.off && trans == o.trans && rows == o.rows && cols == o.cols; }
+  };
+  struct KeyHash {
+    size_t operator()(const Key& k) const {
+      uint64_t h = (uint64_t)k.off * 0x9E3779B97F4A7C15ull;
+      h ^= ((uint64_t)k.trans << 62) ^ ((uint64_t)k.rows << 31) ^ (uint64_t)k.cols;
+      return (size_t)(h ^ (h >> 29));
+    }
+  };
+  std::unordered_map<Key, int64_t, KeyHash> ab_map;
+  std::unordered_map<int64_t, int64_t> c_map;
+  auto slice_of = [&](std::vector<SliceItem>& items, int64_t& elems, int64_t src, int rows, int cols, int trans) {
+    SliceItem it;
+    it.src = src;
+    it.dst = elems;
+    it.rows = rows;
+    it.cols = cols;
+    it.pitch = (cols + 7) & ~7;
+    it.trans = trans;
+    items.push_back(it);
+    elems += 3 * (int64_t)rows * it.pitch;
+    return it.dst;
+  };
+  auto ab_slice = [&](int64_t off, int trans, int rows, int cols) {
+    auto f = ab_map.emplace(Key{off, trans, rows, cols}, 0);
+    if (f.second) f.first->second = slice_of(p->ab_items, p->ab_elems, off, rows, cols, trans);
+    return f.first->second;
+  };
+  auto c_slice = [&](int64_t off, int k1, int k2) {
+    auto f = c_map.emplace(off, 0);
+    if (f.second) f.first->second = slice_of(p->c_items, p->c_elems, off, k2, k1, 1);  // C^T: (j, p) = C[p k2 + j]
+    return f.first->second;
+  };
+  for (auto& pc : p->chunks) {
+    std::vector<PairS> ps(pc.h_perm.size(), PairS{-1, -1, -1});
+    for (size_t j = 0; j < pc.h_perm.size(); ++j) {
+      if (pc.h_vbi[j] < 0) continue;
+      const bsg_vblock_t& vb = pc.h_vb[pc.h_vbi[j]];
+      const bsg_pair_t& pr = pc.h_perm[j];
+      if (pr.a_off >= 0) {
+        ps[j].a = ab_slice(pr.a_off, pr.a_trans, vb.m, pr.k1);  // op(A)(i, p), K = p contiguous
+        ps[j].c = c_slice(pr.c_off, pr.k1, pr.k2);
+      }
+      if (pr.b_off >= 0) ps[j].b = ab_slice(pr.b_off, pr.b_trans ? 0 : 1, vb.n, pr.k2);  // B'(k, col) as [col][k]
+    }
+    if (!ps.empty()) {
+      HIPCHK(hipMalloc((void**)&pc.d_ps, sizeof(PairS) * ps.size()));
+      HIPCHK(hipMemcpy(pc.d_ps, ps.data(), sizeof(PairS) * ps.size(), hipMemcpyHostToDevice));
+    }
+  }
+  for (auto* v : {&p->ab_items, &p->c_items})
+    if (!v->empty()) {
+      SliceItem** d = v == &p->ab_items ? &p->d_ab_items : &p->d_c_items;
+      if (hipMalloc((void**)d, sizeof(SliceItem) * v->size()) != hipSuccess ||
+          hipMemcpy(*d, v->data(), sizeof(SliceItem) * v->size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: slice recipes");
+    }
+  p->slices_built = true;
+  return BSG_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* p, const double* arena, int64_t n) {
   if (!ctx || !p || !arena || n < 0) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: bad args");
   if (p->ctx != ctx) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: plan belongs to another context");
   if (n < p->arena_extent) return fail(BSG_ERR_ARG, "bsg_plan_split_arena: arena smaller than the plan's extent");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!p->slices_built) {
+    const int rc = build_slices(p);
+    if (rc != BSG_OK) return rc;
+  }
   // operator slices for the grouped kernel (the small-block kernel splits
   // in registers), and room for the C slices of one launch's roots
   if (!p->d_abs && p->ab_elems > 0) {
@@ -2000,7 +2041,7 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
 int check_precision(const bsg_plan* p, const double* arena, int32_t precision) {
   if (precision == BSG_FP64) return BSG_OK;
   if (precision != BSG_SPLIT_BF16) return fail(BSG_ERR_ARG, "unknown precision mode");
-  if (p->split_src != arena || (p->ab_elems > 0 && !p->d_abs) || (p->c_elems > 0 && !p->d_cs))
+  if (p->split_src != arena || !p->slices_built || (p->ab_elems > 0 && !p->d_abs) || (p->c_elems > 0 && !p->d_cs))
     return fail(BSG_ERR_ARG, "split precision needs bsg_plan_split_arena on this arena first");
   return BSG_OK;
 }
