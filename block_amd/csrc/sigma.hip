@@ -1384,6 +1384,11 @@ struct bsg_context {
   double* coef_pin = nullptr;    // page-locked staging of bsg_lincomb's coefficients (GRAM_MAX^2)
   hipEvent_t coef_done = nullptr;  // its last copy to the device
   double* scratch = nullptr;  // this device's copy of the static scratch (bsg_sigma_apply_noalloc)
+  // operator-slice buffer lent to one split-precision plan at a time (grown
+  // on demand), so successive superblocks do not allocate and free it
+  __bf16* abs_buf = nullptr;
+  size_t abs_cap = 0;  // elements
+  bool abs_leased = false;
 };
 
 struct PlanChunk {
@@ -1415,6 +1420,7 @@ struct bsg_plan {
   int32_t batch = 1;         // roots per launch (bsg_plan_set_batch)
   bool cs_ready = false;     // d_cs holds C slices for `batch` roots
   bool slices_built = false; // slice recipes and pair slice offsets (first bsg_plan_split_arena)
+  bool abs_from_ctx = false; // d_abs is the context's lent buffer
   int64_t wave_size = 0;
   int64_t arena_extent = 0;  // doubles of operator arena the pairs reach
   double flops = 0.0, flops1 = 0.0, flops2 = 0.0;
@@ -1499,6 +1505,7 @@ int bsg_free(bsg_context* ctx) {
   if (ctx->copy_list) HIPCHK(hipFree(ctx->copy_list));
   HIPCHK(hipStreamSynchronize(ctx->side));
   HIPCHK(hipFree(ctx->gpart));
+  if (ctx->abs_buf) HIPCHK(hipFree(ctx->abs_buf));
   HIPCHK(hipHostFree(ctx->coef_pin));
   HIPCHK(hipEventDestroy(ctx->coef_done));
   HIPCHK(hipEventDestroy(ctx->fork));
@@ -1786,7 +1793,10 @@ int bsg_plan_destroy(bsg_plan* p) {
   (void)hipFree(p->part);
   (void)hipFree(p->d_ab_items);
   (void)hipFree(p->d_c_items);
-  (void)hipFree(p->d_abs);
+  if (p->abs_from_ctx)
+    p->ctx->abs_leased = false;
+  else
+    (void)hipFree(p->d_abs);
   (void)hipFree(p->d_cs);
   if (p->graph) (void)hipGraphExecDestroy(p->graph);
   delete p;
@@ -1804,20 +1814,24 @@ namespace {
 // slice offsets; made on the first split-precision set-up, so FP64-only
 // plans (the rotations, FP64 solves) never pay for them.
 int build_slices(bsg_plan* p) {
-  struct Key {
-    int64_t off;
-    int trans, rows, cols;
-    bool operator==(const Key& o) const { return off == o.off && trans == o.trans && rows == o.rows && cols == o.cols; }
+  // open-addressing tables (no allocation per entry): keys of the operator
+  // blocks (offset, orientation, rows, cols) and of the C blocks (offset)
+  size_t npairs = 0;
+  for (auto& pc : p->chunks) npairs += pc.h_perm.size();
+  size_t cap = 64;
+  while (cap < 4 * npairs + 64) cap <<= 1;
+  struct Ent {
+    int64_t off = -1;
+    int32_t trans = 0, rows = 0, cols = 0;
+    int64_t val = 0;
   };
-  struct KeyHash {
-    size_t operator()(const Key& k) const {
-      uint64_t h = (uint64_t)k.off * 0x9E3779B97F4A7C15ull;
-      h ^= ((uint64_t)k.trans << 62) ^ ((uint64_t)k.rows << 31) ^ (uint64_t)k.cols;
-      return (size_t)(h ^ (h >> 29));
-    }
+  std::vector<Ent> abt(cap), ct(cap);
+  auto hash = [](int64_t off, int t, int r, int c) {
+    uint64_t h = (uint64_t)off * 0x9E3779B97F4A7C15ull ^ ((uint64_t)t << 61) ^ ((uint64_t)r << 30) ^ (uint64_t)c;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    return (size_t)(h ^ (h >> 29));
   };
-  std::unordered_map<Key, int64_t, KeyHash> ab_map;
-  std::unordered_map<int64_t, int64_t> c_map;
   auto slice_of = [&](std::vector<SliceItem>& items, int64_t& elems, int64_t src, int rows, int cols, int trans) {
     SliceItem it;
     it.src = src;
@@ -1831,14 +1845,29 @@ int build_slices(bsg_plan* p) {
     return it.dst;
   };
   auto ab_slice = [&](int64_t off, int trans, int rows, int cols) {
-    auto f = ab_map.emplace(Key{off, trans, rows, cols}, 0);
-    if (f.second) f.first->second = slice_of(p->ab_items, p->ab_elems, off, rows, cols, trans);
-    return f.first->second;
+    for (size_t h = hash(off, trans, rows, cols) & (cap - 1);; h = (h + 1) & (cap - 1)) {
+      Ent& e = abt[h];
+      if (e.off < 0) {
+        e.off = off;
+        e.trans = trans;
+        e.rows = rows;
+        e.cols = cols;
+        e.val = slice_of(p->ab_items, p->ab_elems, off, rows, cols, trans);
+        return e.val;
+      }
+      if (e.off == off && e.trans == trans && e.rows == rows && e.cols == cols) return e.val;
+    }
   };
   auto c_slice = [&](int64_t off, int k1, int k2) {
-    auto f = c_map.emplace(off, 0);
-    if (f.second) f.first->second = slice_of(p->c_items, p->c_elems, off, k2, k1, 1);  // C^T: (j, p) = C[p k2 + j]
-    return f.first->second;
+    for (size_t h = hash(off, 0, 0, 0) & (cap - 1);; h = (h + 1) & (cap - 1)) {
+      Ent& e = ct[h];
+      if (e.off < 0) {
+        e.off = off;
+        e.val = slice_of(p->c_items, p->c_elems, off, k2, k1, 1);  // C^T: (j, p) = C[p k2 + j]
+        return e.val;
+      }
+      if (e.off == off) return e.val;
+    }
   };
   for (auto& pc : p->chunks) {
     std::vector<PairS> ps(pc.h_perm.size(), PairS{-1, -1, -1});
@@ -1883,8 +1912,23 @@ int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* p, const double* arena, int
   // operator slices for the grouped kernel (the small-block kernel splits
   // in registers), and room for the C slices of one launch's roots
   if (!p->d_abs && p->ab_elems > 0) {
-    hipError_t e = hipMalloc((void**)&p->d_abs, 2 * p->ab_elems);
-    if (e != hipSuccess) return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: operator slices");
+    if (!ctx->abs_leased) {
+      if (ctx->abs_cap < (size_t)p->ab_elems) {
+        (void)hipFree(ctx->abs_buf);
+        ctx->abs_buf = nullptr;
+        ctx->abs_cap = 0;
+        const size_t cap = (size_t)p->ab_elems + (size_t)p->ab_elems / 4;
+        if (hipMalloc((void**)&ctx->abs_buf, 2 * cap) != hipSuccess)
+          return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: operator slices");
+        ctx->abs_cap = cap;
+      }
+      p->d_abs = ctx->abs_buf;
+      ctx->abs_leased = true;
+      p->abs_from_ctx = true;
+    } else {
+      hipError_t e = hipMalloc((void**)&p->d_abs, 2 * p->ab_elems);
+      if (e != hipSuccess) return fail(BSG_ERR_NOMEM, "bsg_plan_split_arena: operator slices");
+    }
   }
   if (!p->cs_ready && p->c_elems > 0) {
     hipError_t e = hipMalloc((void**)&p->d_cs, 2 * p->c_elems * p->batch);
