@@ -1052,7 +1052,9 @@ std::vector<Geom> geom_candidates(int max_nv) {
 inline Geom pick_geom(int m, int n, int max_nv) {
   // typical contraction depth: the block's own dimensions
   const long k = std::max(m, n);
-  const std::vector<Geom> cand = geom_candidates(max_nv);
+  // the two candidate lists, built once (pick_geom runs per V block)
+  static const std::vector<Geom> cand_fp64 = geom_candidates(6), cand_split = geom_candidates(4);
+  const std::vector<Geom>& cand = max_nv >= 6 ? cand_fp64 : cand_split;
   for (const TileRule& r : g_tile_rules)
     if (m >= r.m_lo && m <= r.m_hi && n >= r.n_lo && n <= r.n_hi) {
       const int idx = max_nv >= 6 ? r.fp64 : r.split;
@@ -1597,6 +1599,9 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     std::vector<Seg> segs;
     std::vector<RedItem> reds[2];  // ordered reductions after the FP64 / the split-precision kernels
     int64_t poff = 0;
+    std::vector<int> fast, slow;  // per V block, reused
+    std::vector<std::pair<int, int>> units;
+    std::vector<char> usmall;
     for (int i = 0; i < ch.n_vblocks; ++i) {
       const auto& vb = ch.vblocks[i];
       if (vb.m <= 0 || vb.n <= 0 || vb.v_off < 0 || vb.v_off + (int64_t)vb.m * vb.n > wave_size || vb.first < 0 ||
@@ -1610,7 +1615,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       const bool small_vb = std::max(vb.m, vb.n) <= 32;
       // tiny: the block and every small-kernel pair's C block within 16 x 16
       bool tiny = std::max(vb.m, vb.n) <= 16;
-      std::vector<int> fast, slow;  // pair indices (absolute) in term order
+      fast.clear();  // pair indices (absolute) in term order
+      slow.clear();
       for (int q = 0; q < vb.count; ++q) {
         const auto& pr = ch.pairs[vb.first + q];
         if (pr.k1 <= 0 || pr.k2 <= 0 || pr.c_off < 0 || pr.c_off + (int64_t)pr.k1 * pr.k2 > wave_size ||
@@ -1647,8 +1653,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       // cut (mode 0).  Wide blocks have no small-kernel pairs.
       const int64_t mn = (int64_t)vb.m * vb.n;
       auto emit = [&](const Geom& g, int mode) {
-        std::vector<std::pair<int, int>> units;  // (first, count) in device order
-        std::vector<char> usmall;
+        units.clear();  // (first, count) in device order
+        usmall.clear();
         auto cut = [&](int first, int n, bool sm) {
           int start = 0;
           long cost = 0;
