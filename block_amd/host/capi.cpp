@@ -471,6 +471,7 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
     P.rotator = std::make_shared<GpuRotator>(ctx, device_blocks != 0);
     if (device_blocks) {
       P.builder = std::make_shared<GpuBlockBuilder>(ctx);
+      P.resident_block = [&ctx](const SpinBlock& b) { make_device_resident(ctx, block_ops(b)); };
     } else {
       P.preload_block = [&ctx](const SpinBlock& b) { preload_ops(ctx, 1, block_ops(b)); };
     }

@@ -219,6 +219,28 @@ void set_materialiser_context(GpuContext* g) {
   set_host_materialiser(g ? download_ops : nullptr);
 }
 
+void make_device_resident(GpuContext& g, const std::vector<OpPtr>& ops) {
+  std::vector<std::pair<const SparseOp*, int64_t>> up;
+  int64_t total = 0;
+  for (const OpPtr& o : ops) {
+    if (!o || o->dev || !o->host_valid || o->size() == 0) continue;
+    up.push_back({o.get(), total});
+    total += o->size();
+  }
+  if (up.empty()) return;
+  double* base = nullptr;
+  auto owner = device_buffer(g, (size_t)total, &base);
+  upload_ops(g, up, base, (size_t)total);
+  bsg_check(bsg_synchronize(g.ctx), "resident upload");
+  for (auto& x : up) {
+    SparseOp* o = const_cast<SparseOp*>(x.first);
+    o->dev = base + x.second;
+    o->dev_owner = owner;
+    o->data = std::vector<double>();
+    o->host_valid = false;
+  }
+}
+
 std::shared_ptr<void> device_buffer(GpuContext& g, size_t n, double** ptr) {
   double* p = nullptr;
   bsg_check(bsg_malloc(g.ctx, sizeof(double) * std::max<size_t>(1, n), (void**)&p), "device block storage");

@@ -44,7 +44,6 @@ double* GpuContext::ensure_work(size_t n) {
   return work;
 }
 
-namespace {
 // Upload operators to dst + offset: copied into page-locked staging by all
 // threads in batches of about 64 MB, each batch sent while the next one is
 // copied; device memory between the operators is left as it is.  n: doubles
@@ -101,6 +100,7 @@ void upload_ops(GpuContext& g, const std::vector<std::pair<const SparseOp*, int6
   bsg_check(bsg_synchronize(g.ctx), "operator upload");
 }
 
+namespace {
 SetupTimes g_setup;
 double secs() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 }  // namespace

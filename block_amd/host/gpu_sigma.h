@@ -228,6 +228,13 @@ BuildPlan plan_build(const std::vector<const BuildItem*>& items,
                      const std::function<const double*(const SparseOp*)>& addr);
 void run_build_host(const BuildPlan& plan);
 void execute_build_host(const BuildSession& session, const std::vector<SparseOp*>& targets);
+// Copy operators to dst + offset (doubles): device-resident ones (preloaded
+// or device-only) on the device, host ones through g's page-locked staging.
+// n: doubles spanned by the offsets.
+void upload_ops(GpuContext& g, const std::vector<std::pair<const SparseOp*, int64_t>>& ops, double* dst, size_t n);
+// Move host operators into one device buffer (they become device-only);
+// operators already on the device are left as they are.
+void make_device_resident(GpuContext& g, const std::vector<OpPtr>& ops);
 // Make g the context that downloads device-only operators for host readers
 // (materialise_host); null: none.
 void set_materialiser_context(GpuContext* g);

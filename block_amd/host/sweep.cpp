@@ -480,6 +480,10 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       throw std::runtime_error("run_dmrg: the saved state is for another target (N, 2S, irrep)");
     fwd = std::move(st.fwd);
     bwd = std::move(st.bwd);
+    if (P.resident_block)
+      for (const auto* side : {&fwd, &bwd})
+        for (const BlockPtr& b : *side)
+          if (b) P.resident_block(*b);
     prev.wave = st.wave;
     prev.b = std::move(st.b);
     res = std::move(st.res);

@@ -47,6 +47,9 @@ struct DmrgParams {
   // called on that thread with the next step's environment-side block once
   // it is built (the GPU runs upload its operators ahead of the next solve)
   std::function<void(const SpinBlock&)> preload_block;
+  // called for every block read back by a resume (the GPU runs with device
+  // builds move its operators into HBM, where a continuous run keeps them)
+  std::function<void(const SpinBlock&)> resident_block;
   uint64_t seed = 12345;
   int verbose = 0;
 };
