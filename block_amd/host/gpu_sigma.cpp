@@ -402,6 +402,7 @@ std::vector<OpPtr> block_ops(const SpinBlock& b) {
 }
 
 std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, const Rotation& R) const {
+  const double t0 = secs();
   const StateInfo& os = *R.old;
   const auto& rot = *R.rot;
   // rotation matrices (the plan's operator arena): d_q x m_q row-major per old sector
@@ -457,6 +458,7 @@ std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, c
       pr.push_back(p);
     }
   }
+  const double t1 = secs();
   double* base = g_.ensure_work((size_t)(total + usize));
   double* ubuf = base + total;
   // operators still in the arena are copied on the device, the rest uploaded
@@ -479,6 +481,7 @@ std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, c
     bsg_check(bsg_memcpy_h2d(g_.ctx, ubuf, u.data(), sizeof(double) * usize), "rotation upload");
   }
   bsg_check(bsg_memset_zero(g_.ctx, base + soff, sizeof(double) * osize), "rotation clear");
+  double t2 = secs(), t3 = t2;
   if (!pr.empty()) {
     bsg_chunk_t chunk;
     chunk.vblocks = vb.data();
@@ -487,10 +490,20 @@ std::vector<OpPtr> GpuRotator::rotate(const std::vector<const SparseOp*>& ops, c
     chunk.n_pairs = (int32_t)pr.size();
     bsg_plan* plan = nullptr;
     bsg_check(bsg_plan_create(g_.ctx, &chunk, 1, total, &plan), "rotation plan");
+    t3 = secs();
     const int rc = bsg_sigma_apply(g_.ctx, plan, ubuf, base, base, 1, BSG_FP64);
-    bsg_plan_destroy(plan);
+    bsg_plan_destroy(plan);  // synchronises the stream
     bsg_check(rc, "rotation");
   }
+  const double t4 = secs();
+  g_setup.rot_pairs += t1 - t0;
+  g_setup.rot_in += t2 - t1;
+  g_setup.rot_plan += t3 - t2;
+  g_setup.rot_run += t4 - t3;
+  struct OutTimer {
+    double t;
+    ~OutTimer() { g_setup.rot_out += secs() - t; }
+  } out_timer{t4};
   if (device_out_) {
     // the rotated operators stay in HBM: one buffer for the batch
     double* nb = nullptr;

@@ -83,6 +83,9 @@ WorkList compile_worklist(const SuperBlock& sb);
 // Seconds spent constructing GpuSigma objects, by phase (accumulated).
 struct SetupTimes {
   double compile = 0, upload = 0, plan = 0;
+  // GpuRotator::rotate: host pair lists, operator and matrix copies in, plan
+  // creation, the rotation kernels (synchronised), the results out
+  double rot_pairs = 0, rot_in = 0, rot_plan = 0, rot_run = 0, rot_out = 0;
 };
 const SetupTimes& gpu_setup_times();
 

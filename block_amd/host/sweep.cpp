@@ -656,9 +656,10 @@ DmrgResult run_dmrg(const Integrals& I, const DmrgParams& P, const SigmaFactory&
       std::fprintf(stderr,
                    "  sigma set-up so far: compile %.2f upload %.2f plan %.2f; renorm so far: eig %.2f rotate %.2f comp "
                    "%.2f (build %.2f); device builds so far: record %.2f execute %.2f [grouping %.2f, device %.2f, %.3g "
-                   "items], host copies %.2f s (%.3g B)\n",
+                   "items], host copies %.2f s (%.3g B); rotations: pairs %.2f in %.2f plan %.2f run %.2f out %.2f\n",
                    st.compile, st.upload, st.plan, g_rt.eig, g_rt.rotate, g_rt.comp, comp_build_seconds(), bt.record,
-                   bt.execute, bt.plan, bt.apply, bt.items, bt.download, bt.download_bytes);
+                   bt.execute, bt.plan, bt.apply, bt.items, bt.download, bt.download_bytes, st.rot_pairs, st.rot_in,
+                   st.rot_plan, st.rot_run, st.rot_out);
     }
     if (P.verbose)
       std::fprintf(stderr,
