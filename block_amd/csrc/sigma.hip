@@ -1041,6 +1041,7 @@ struct TileRule {
   int32_t m_lo, m_hi, n_lo, n_hi, fp64, split;
 };
 std::vector<TileRule> g_tile_rules;
+unsigned g_tile_version = 0;  // bumped by bsg_set_tile_table (pick_geom's cache key)
 std::vector<Geom> geom_candidates(int max_nv) {
   std::vector<Geom> cand;
   for (int lwr = 0; lwr <= 2; ++lwr)
@@ -1049,7 +1050,23 @@ std::vector<Geom> geom_candidates(int max_nv) {
   cand.push_back(make_geom_w8(max_nv >= 6 ? 6 : 4));
   return cand;
 }
+Geom pick_geom_uncached(int m, int n, int max_nv);
+// pick_geom per V block shape, remembered per thread (plans repeat shapes)
 inline Geom pick_geom(int m, int n, int max_nv) {
+  thread_local std::unordered_map<uint64_t, Geom> cache;
+  thread_local unsigned version = ~0u;
+  if (version != g_tile_version) {
+    cache.clear();
+    version = g_tile_version;
+  }
+  const uint64_t key = ((uint64_t)(uint32_t)m << 33) | ((uint64_t)(uint32_t)n << 1) | (max_nv >= 6 ? 1u : 0u);
+  auto f = cache.find(key);
+  if (f != cache.end()) return f->second;
+  const Geom g = pick_geom_uncached(m, n, max_nv);
+  cache.emplace(key, g);
+  return g;
+}
+Geom pick_geom_uncached(int m, int n, int max_nv) {
   // typical contraction depth: the block's own dimensions
   const long k = std::max(m, n);
   // the two candidate lists, built once (pick_geom runs per V block)
@@ -1101,22 +1118,26 @@ inline long pair_cost(const Geom& g, int n, const bsg_pair_t& pr) {
 // queue runs costliest units first, and the queues are interleaved
 // (padding tiles with item -1 keep the positions aligned).
 constexpr int NXCD = 8;
+// a work unit's tiles: segment seg, ntm x ntn tiles of geometry code
 struct GUnit {
   long cost = 0;
-  std::vector<TileRef> tiles;
+  int seg = 0, ntm = 0, ntn = 0, code = 0;
+  long ntiles() const { return (long)ntm * ntn; }
 };
 inline std::vector<TileRef> xcd_order(std::vector<GUnit>& units) {
   std::vector<int> ord(units.size());
   for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-    return units[a].cost * (long)units[a].tiles.size() > units[b].cost * (long)units[b].tiles.size();
+    return units[a].cost * units[a].ntiles() > units[b].cost * units[b].ntiles();
   });
   std::vector<std::vector<TileRef>> q(NXCD);
   std::vector<long> load(NXCD, 0);
   for (int u : ord) {
     int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    load[x] += units[u].cost * (long)units[u].tiles.size();
-    q[x].insert(q[x].end(), units[u].tiles.begin(), units[u].tiles.end());
+    const GUnit& g = units[u];
+    load[x] += g.cost * g.ntiles();
+    for (int tm = 0; tm < g.ntm; ++tm)
+      for (int tn = 0; tn < g.ntn; ++tn) q[x].push_back({g.seg, tm, tn, g.code});
   }
   size_t len = 0;
   for (auto& v : q) len = std::max(len, v.size());
@@ -1695,8 +1716,10 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           for (int q = 0; q < sg.count; ++q) ucost += pair_cost(g, vb.n, perm[sg.first + q]);
           GUnit gu;
           gu.cost = ucost;
-          for (int tm = 0; tm < (vb.m + g.tm - 1) / g.tm; ++tm)
-            for (int tn = 0; tn < (vb.n + g.jv - 1) / g.jv; ++tn) gu.tiles.push_back({si, tm, tn, g.code});
+          gu.seg = si;
+          gu.ntm = (vb.m + g.tm - 1) / g.tm;
+          gu.ntn = (vb.n + g.jv - 1) / g.jv;
+          gu.code = g.code;
           if (mode != 2) gunits[g.cls].push_back(gu);
           if (mode != 1 && g.cls < 3) gunits_bf[g.cls].push_back(gu);
         }
@@ -1961,6 +1984,7 @@ int bsg_set_tile_table(const int32_t* rules, int32_t n) {
     t.push_back(r);
   }
   g_tile_rules = t;
+  ++g_tile_version;
   return BSG_OK;
 }
 int bsg_tile_candidates(int32_t precision) {
