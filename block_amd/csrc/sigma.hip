@@ -1845,10 +1845,11 @@ namespace {
 int build_slices(bsg_plan* p) {
   // open-addressing tables (no allocation per entry): keys of the operator
   // blocks (offset, orientation, rows, cols) and of the C blocks (offset)
-  size_t npairs = 0;
-  for (auto& pc : p->chunks) npairs += pc.h_perm.size();
+  size_t nslow = 0;  // pairs of the grouped kernels (the only ones with slices)
+  for (auto& pc : p->chunks)
+    for (int v : pc.h_vbi) nslow += v >= 0;
   size_t cap = 64;
-  while (cap < 4 * npairs + 64) cap <<= 1;
+  while (cap < 4 * nslow + 64) cap <<= 1;
   struct Ent {
     int64_t off = -1;
     int32_t trans = 0, rows = 0, cols = 0;
