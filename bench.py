@@ -115,8 +115,9 @@ def ramp_leg(ints, cfg, M, ck, verbose=1):
 
 def sweep_leg(ints, cfg, M, ck, tmp, verbose=1):
     """One full two-site sweep at M from the ramp's saved state, in FP64 and
-    with the precision ladder; every step starts from the previous step's
-    transformed wavefunction.  The FP64 leg saves its final state, whose
+    with the precision ladder (its split phase on the superblocks with at
+    least half of their flops on BF16 slices); every step starts from the
+    previous step's transformed wavefunction.  The FP64 leg saves its final state, whose
     blocks give the kept states of the timed sweep."""
     n = cfg["ramp_sweeps"]
     sched = [tuple(r) for r in cfg["ramp"]] + [(n, M, cfg["timed_tol"], 0.0)]
@@ -546,7 +547,8 @@ def main():
         hxs = h(x)
         s_wall, s_kern, s_disp = h.timed(c, args.warmup, args.steps)
         h.precision = block_amd.FP64
-        mixed = {"tflops": round(flops * args.steps / (s_wall * 1e-3) / 1e12, 4),
+        mixed = {"split_share": round(h.split_share, 4),
+                 "tflops": round(flops * args.steps / (s_wall * 1e-3) / 1e12, 4),
                  "ms_per_step": round(s_wall / args.steps, 4),
                  "kernel_ms_per_step": round(s_kern[0] / args.steps, 4),
                  "dispatches_per_step": s_disp / args.steps,

@@ -63,6 +63,7 @@ _sig(_h, "blkh_gpu_sigma_free", None, [_vp])
 _sig(_h, "blkh_gpu_sigma_apply", ctypes.c_int, [_vp, _dp, _dp])
 _sig(_h, "blkh_gpu_sigma_diag", ctypes.c_int, [_vp, _dp])
 _sig(_h, "blkh_gpu_sigma_flops", ctypes.c_double, [_vp])
+_sig(_h, "blkh_gpu_sigma_split_share", ctypes.c_double, [_vp])
 _sig(_h, "blkh_gpu_sigma_pairs", ctypes.c_long, [_vp])
 _sig(_h, "blkh_gpu_sigma_phase_flops", ctypes.c_int, [_vp, _dp, _dp])
 _sig(_h, "blkh_gpu_sigma_arena_bytes", ctypes.c_long, [_vp])
@@ -80,7 +81,7 @@ _sig(_h, "blkh_gpu_sigma_raw", ctypes.c_int, [_vp] + [ctypes.POINTER(_vp)] * 5)
 _sig(_h, "blkh_dmrg_gpu", ctypes.c_int,
      [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
-      ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp])
+      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp])
 _sig(_h, "blkh_build_check", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _lp])
 _sig(_h, "blkh_build_check_cpu", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _lp])
 _sig(_h, "blkh_build_times", ctypes.c_int, [_dp])
@@ -270,6 +271,8 @@ class multiply_h:
         self._precision = int(mode)
 
     flops = property(lambda s: host_lib.blkh_gpu_sigma_flops(s._h))
+    # share of the flops split mode runs on BF16 slices (the rest is FP64)
+    split_share = property(lambda s: host_lib.blkh_gpu_sigma_split_share(s._h))
     pairs = property(lambda s: host_lib.blkh_gpu_sigma_pairs(s._h))
     arena_bytes = property(lambda s: host_lib.blkh_gpu_sigma_arena_bytes(s._h))
 
@@ -351,7 +354,7 @@ def _schedule_array(schedule):
 
 def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=None, max_sweeps=20,
          sweep_tol=1e-8, device=0, verbose=0, ladder=0.0, guess=True, ladder_from=0, checkpoint=None, resume=None,
-         host_ahead=True, device_blocks=True):
+         host_ahead=True, device_blocks=True, ladder_min_share=0.5):
     """Two-site spin-adapted DMRG (Sweep::do_one, sweep.C:341) with the Davidson
     sigma on the GPU.  schedule: rows (start_sweep, M, davidson_tol, noise).
     device_blocks: build the composite operators in HBM from device-resident
@@ -359,7 +362,9 @@ def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=
     uploads (then host_ahead applies).
     ladder > 0: split-precision sigma until the Davidson residual norm falls
     below it, FP64 after (convergence is always decided on FP64 sigmas), from
-    sweep ladder_from on (0: also in the warm-up).
+    sweep ladder_from on (0: also in the warm-up), on superblocks whose flops
+    run at least ladder_min_share on BF16 slices (multiply_h.split_share; the
+    rest of a superblock stays FP64 in split mode).
     guess: start every step from the previous step's wavefunction carried into
     the new basis (guess_wavefunction.C:524-636), else from the diagonal.
     checkpoint: file the sweep state is written to after every sweep; resume:
@@ -381,7 +386,8 @@ def dmrg(ints, nelec=None, twoS=0, irrep=None, nroots=1, weights=None, schedule=
                                sweep_tol, float(ladder), int(ladder_from), int(guess),
                                None if checkpoint is None else str(checkpoint).encode(),
                                None if resume is None else str(resume).encode(), int(bool(host_ahead)),
-                               int(bool(device_blocks)), device, verbose, _ptr(e), _ptr(sw))
+                               int(bool(device_blocks)), device, verbose, float(ladder_min_share), _ptr(e),
+                               _ptr(sw))
     if n < 0:
         raise _err()
     return e, sw[: nc * n].reshape(n, nc)

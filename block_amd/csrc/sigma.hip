@@ -1447,6 +1447,7 @@ struct bsg_plan {
   int64_t wave_size = 0;
   int64_t arena_extent = 0;  // doubles of operator arena the pairs reach
   double flops = 0.0, flops1 = 0.0, flops2 = 0.0;
+  double flops_split = 0.0;  // of flops: the grouped kernels' pairs (BF16 slices in split mode)
   // split precision: the arena bsg_plan_split_arena sliced, the slice
   // recipes of the operator blocks (once per arena) and of the C blocks
   // (every application) and their BF16 buffers
@@ -1655,6 +1656,9 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         }
         const bool f = small_vb && ((pr.a_off >= 0 && pr.k1 <= 32 && pr.k2 <= 32) || (pr.a_off < 0 && pr.b_off < 0));
         (f ? fast : slow).push_back(vb.first + q);
+        if (!f)
+          p->flops_split += (pr.a_off >= 0 ? 2.0 * vb.m * pr.k1 * pr.k2 : 0.0) +
+                            (pr.b_off >= 0 ? 2.0 * vb.m * pr.k2 * vb.n : 0.0);
         if (f && (pr.k1 > 16 || pr.k2 > 16)) tiny = false;
       }
       // the block's pairs in device order: the small-kernel ones, then the rest
@@ -2021,6 +2025,7 @@ int bsg_plan_set_batch(bsg_context* ctx, bsg_plan* p, int32_t nroot) {
 }
 
 double bsg_plan_flops(const bsg_plan* p) { return p ? p->flops : 0.0; }
+double bsg_plan_split_share(const bsg_plan* p) { return p && p->flops > 0.0 ? p->flops_split / p->flops : 0.0; }
 int bsg_plan_phase_flops(const bsg_plan* p, double* f1, double* f2) {
   if (!p || !f1 || !f2) return fail(BSG_ERR_ARG, "bsg_plan_phase_flops: bad args");
   *f1 = p->flops1;

@@ -42,7 +42,8 @@ struct GpuSigmaHandle {
 
 struct GpuFunctor : DavidsonFunctor {
   GpuSigma sig;
-  GpuFunctor(GpuContext& g, const SuperBlock& sb) : sig(g, sb) {}
+  double min_split_share = 0.0;  // the ladder's split phase below this share is skipped
+  GpuFunctor(GpuContext& g, const SuperBlock& sb, double min_share = 0.0) : sig(g, sb), min_split_share(min_share) {}
   void operator()(const double* c, double* v) override { sig.apply(c, v); }
   bool set_precision(int mode) override {
     sig.set_precision(mode);
@@ -60,7 +61,7 @@ struct GpuFunctor : DavidsonFunctor {
     const int dmin = std::max(deflation_min, (int)b.size());
     if (std::max(deflation_max, dmin + 1) + 1 > kDeviceDavidsonMaxVectors) return false;
     block_davidson_device(sig, b, h_diag, normtol, useprecond, deflation_min, deflation_max, st, max_iter, ladder,
-                          lower);
+                          lower, min_split_share);
     return true;
   }
 };
@@ -305,6 +306,7 @@ int blkh_gpu_sigma_diag(void* h, double* out) {
   });
 }
 double blkh_gpu_sigma_flops(void* h) { return static_cast<GpuSigmaHandle*>(h)->sig->flops(); }
+double blkh_gpu_sigma_split_share(void* h) { return static_cast<GpuSigmaHandle*>(h)->sig->split_share(); }
 int blkh_gpu_sigma_phase_flops(void* h, double* f1, double* f2) {
   return bsg_plan_phase_flops(static_cast<GpuSigmaHandle*>(h)->sig->plan(), f1, f2);
 }
@@ -442,7 +444,7 @@ int blkh_davidson(void* sbp, int device, int mode, int nroots, const double* gue
 int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const double* weights, int n_sched,
                   const double* sched, int max_sweeps, double sweep_tol, double ladder, int ladder_from, int guess,
                   const char* checkpoint, const char* resume, int ahead, int device_blocks, int device, int verbose,
-                  double* energies_out, double* sweep_out) {
+                  double ladder_min_share, double* energies_out, double* sweep_out) {
   return guarded([&]() {
     const Integrals& I = *static_cast<Integrals*>(ints);
     DmrgParams P;
@@ -476,7 +478,7 @@ int blkh_dmrg_gpu(void* ints, int nelec, int twoS, int irrep, int nroots, const 
       P.preload_block = [&ctx](const SpinBlock& b) { preload_ops(ctx, 1, block_ops(b)); };
     }
     auto res = run_dmrg(I, P, [&](const SuperBlock& sb) {
-      return std::unique_ptr<DavidsonFunctor>(new GpuFunctor(ctx, sb));
+      return std::unique_ptr<DavidsonFunctor>(new GpuFunctor(ctx, sb, ladder_min_share));
     });
     for (int r = 0; r < nroots; ++r) energies_out[r] = res.energies[r];
     return write_sweep_records(res, sweep_out, max_sweeps);

@@ -30,7 +30,8 @@ struct DevBuf {
 
 void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std::vector<double>& h_diag,
                            double normtol, bool useprecond, int deflation_min, int deflation_max, DavidsonStats* st,
-                           int max_iter, double ladder, const std::vector<std::vector<double>>* lower) {
+                           int max_iter, double ladder, const std::vector<std::vector<double>>* lower,
+                           double min_split_share) {
   bsg_context* ctx = h.ctx();
   const int64_t N = h.wave_size();
   const int nroots = (int)b.size();
@@ -82,7 +83,10 @@ void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std
     project_lower(vec(Bp, 0));
     normalise(vec(Bp, 0));
   }
-  bool split = ladder > 0.0;
+  // the ladder's split phase only where split mode moves most of the work
+  // onto BF16 slices: elsewhere the split sigma is mostly the FP64 small-block
+  // kernel, and the restart at the switch would cost more than it saves
+  bool split = ladder > 0.0 && h.split_share() >= min_split_share;
   h.set_precision(split ? BSG_SPLIT_BF16 : BSG_FP64);
   // the first iteration's (and a ladder restart's) nroots sigmas in one launch set
   h.set_batch(nroots);

@@ -104,6 +104,8 @@ class GpuSigma {
   // (the BF16 slices of the arena are built on first use)
   void set_precision(int mode);
   int precision() const { return precision_; }
+  // share of the flops the split mode runs on BF16 slices (bsg_plan_split_share)
+  double split_share() const { return bsg_plan_split_share(plan_); }
   // host vectors: v = H c for nvec <= max_vec vectors (overwrites v)
   void apply(const double* c_host, double* v_host, int nvec = 1);
   // roots per launch of apply_device (bsg_plan_set_batch)
@@ -252,11 +254,19 @@ struct BuildTimes {
 const BuildTimes& gpu_build_times();
 
 // Largest Davidson subspace (deflation_max + 1 vectors) the device solve takes.
+// min_split_share: the ladder's split phase runs only when h.split_share()
+// reaches it.
 constexpr int kDeviceDavidsonMaxVectors = 128;
+// Default smallest share of a superblock's flops on BF16 slices
+// (split_share) for which a sweep's device Davidson runs the split phase of
+// its precision ladder (below it the split sigma is mostly the FP64
+// small-block kernel, and the restart at the switch costs more than it saves).
+constexpr double kLadderMinSplitShare = 0.5;
 // Linear::block_davidson on device vectors with the sigma of h (same control
 // flow and outputs as blk::block_davidson; b, h_diag host vectors in/out).
 void block_davidson_device(GpuSigma& h, std::vector<std::vector<double>>& b, std::vector<double>& h_diag,
                            double normtol, bool useprecond, int deflation_min, int deflation_max, DavidsonStats* st,
-                           int max_iter, double ladder, const std::vector<std::vector<double>>* lower = nullptr);
+                           int max_iter, double ladder, const std::vector<std::vector<double>>* lower = nullptr,
+                           double min_split_share = 0.0);
 
 }  // namespace blk

@@ -109,6 +109,11 @@ int bsg_plan_destroy(bsg_plan* plan);
 double bsg_plan_flops(const bsg_plan* plan);
 /* The same split by GEMM: first (W = op(A) C) and second (V += W op(B)^T). */
 int bsg_plan_phase_flops(const bsg_plan* plan, double* phase1, double* phase2);
+/* Share of the flops that BSG_SPLIT_BF16 runs on BF16 slices (the grouped
+ * kernels' pairs; V blocks up to 32 x 32 stay FP64).  No counterpart in the
+ * reference; the device Davidson skips the split phase of its precision
+ * ladder below a share of one half. */
+double bsg_plan_split_share(const bsg_plan* plan);
 
 /* Doubles of operator arena the plan's pairs reach (the arena passed to
  * bsg_sigma_apply must hold at least this many). */

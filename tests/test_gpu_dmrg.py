@@ -37,7 +37,7 @@ def test_gpu_dmrg_ladder_chain12():
     print("[gpu] dmrg hubbard chain 12 ladder", flush=True)
     ints = config_fcidump("hubbard12")
     e64, sw64 = block_amd.dmrg(ints, schedule=SCHED, max_sweeps=14, sweep_tol=1e-9)
-    e, sw = block_amd.dmrg(ints, schedule=SCHED, max_sweeps=14, sweep_tol=1e-9, ladder=1e-5)
+    e, sw = block_amd.dmrg(ints, schedule=SCHED, max_sweeps=14, sweep_tol=1e-9, ladder=1e-5, ladder_min_share=0.0)
     split_calls = sw[:, block_amd.SWEEP_COLS.index("split_calls")].sum()
     calls = sw[:, block_amd.SWEEP_COLS.index("sigma_calls")].sum()
     print(f"[gpu] ladder dE={e[0] - e64[0]:.3e} split calls {split_calls:.0f} of {calls:.0f}", flush=True)
@@ -77,7 +77,8 @@ def test_gpu_dmrg_four_roots():
     e_cpu, _ = _oracle.dmrg_cpu(ints, nroots=4, schedule=sched, max_sweeps=6, sweep_tol=0.0)
     assert e == pytest.approx(e_cpu, abs=1e-10)
     assert e == pytest.approx(singlets, abs=1e-8)
-    e_l, _ = block_amd.dmrg(ints, nroots=4, schedule=sched, max_sweeps=6, sweep_tol=0.0, ladder=1e-5)
+    e_l, _ = block_amd.dmrg(ints, nroots=4, schedule=sched, max_sweeps=6, sweep_tol=0.0, ladder=1e-5,
+                            ladder_min_share=0.0)
     assert e_l == pytest.approx(e, abs=1e-8)
 
 
