@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <unordered_map>
 #include <cstdio>
@@ -1615,20 +1616,38 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
   for (int ci = 0; ci < n_chunks; ++ci) {
     const bsg_chunk_t& ch = chunks[ci];
     PlanChunk pc;
-    std::vector<GUnit> gunits[NCLS], gunits_bf[3];  // the tiles of each work unit (FP64, split)
-    std::vector<bsg_pair_t> perm;  // pairs in device order (grouped per V block as given)
-    std::vector<std::pair<long, int>> sunits[2];  // (cost, segment) of small-block units
-    std::vector<Seg> segs;
-    std::vector<RedItem> reds[2];  // ordered reductions after the FP64 / the split-precision kernels
-    int64_t poff = 0;
-    std::vector<int> fast, slow;  // per V block, reused
-    std::vector<std::pair<int, int>> units;
-    std::vector<char> usmall;
-    for (int i = 0; i < ch.n_vblocks; ++i) {
+    // The V blocks are independent: contiguous ranges of them (about equal
+    // in pairs) are planned on host threads into local lists with local
+    // indices, which are then concatenated in V block order (the result does
+    // not depend on the number of threads).
+    struct Local {
+      std::vector<GUnit> gunits[NCLS], gunits_bf[3];  // the tiles of each work unit (FP64, split)
+      std::vector<bsg_pair_t> perm;                   // pairs in device order (grouped per V block as given)
+      std::vector<int> h_vbi;
+      std::vector<std::pair<long, int>> sunits[2];  // (cost, segment) of small-block units
+      std::vector<Seg> segs;
+      std::vector<RedItem> reds[2];  // ordered reductions after the FP64 / the split-precision kernels
+      int64_t poff = 0, arena_extent = 0;
+      double flops = 0.0, flops1 = 0.0, flops2 = 0.0, flops_split = 0.0;
+      std::string err;
+      std::vector<int> fast, slow;  // per V block, reused
+      std::vector<std::pair<int, int>> units;
+      std::vector<char> usmall;
+    };
+    auto plan_vblock = [&](int i, Local& L) -> bool {
+      std::vector<bsg_pair_t>& perm = L.perm;
+      std::vector<Seg>& segs = L.segs;
+      int64_t& poff = L.poff;
+      auto& fast = L.fast;
+      auto& slow = L.slow;
+      auto& units = L.units;
+      auto& usmall = L.usmall;
       const auto& vb = ch.vblocks[i];
       if (vb.m <= 0 || vb.n <= 0 || vb.v_off < 0 || vb.v_off + (int64_t)vb.m * vb.n > wave_size || vb.first < 0 ||
-          vb.count < 0 || vb.first + vb.count > ch.n_pairs)
-        return cleanup(BSG_ERR_ARG, "bsg_plan_create: vblock out of range");
+          vb.count < 0 || vb.first + vb.count > ch.n_pairs) {
+        L.err = "bsg_plan_create: vblock out of range";
+        return false;
+      }
       const Geom gm = pick_geom(vb.m, vb.n, 6);     // FP64 tiles
       const Geom gs = pick_geom(vb.m, vb.n, 4);     // split-precision tiles
       // small blocks (up to 32 x 32) go to the per-wave kernel, except pairs
@@ -1642,34 +1661,36 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       for (int q = 0; q < vb.count; ++q) {
         const auto& pr = ch.pairs[vb.first + q];
         if (pr.k1 <= 0 || pr.k2 <= 0 || pr.c_off < 0 || pr.c_off + (int64_t)pr.k1 * pr.k2 > wave_size ||
-            (pr.a_off < 0 && pr.k1 != vb.m) || (pr.b_off < 0 && pr.k2 != vb.n))
-          return cleanup(BSG_ERR_ARG, "bsg_plan_create: pair out of range");
-        if (pr.a_off >= 0) p->arena_extent = std::max(p->arena_extent, pr.a_off + (int64_t)vb.m * pr.k1);
-        if (pr.b_off >= 0) p->arena_extent = std::max(p->arena_extent, pr.b_off + (int64_t)vb.n * pr.k2);
+            (pr.a_off < 0 && pr.k1 != vb.m) || (pr.b_off < 0 && pr.k2 != vb.n)) {
+          L.err = "bsg_plan_create: pair out of range";
+          return false;
+        }
+        if (pr.a_off >= 0) L.arena_extent = std::max(L.arena_extent, pr.a_off + (int64_t)vb.m * pr.k1);
+        if (pr.b_off >= 0) L.arena_extent = std::max(L.arena_extent, pr.b_off + (int64_t)vb.n * pr.k2);
         if (pr.a_off >= 0) {
-          p->flops += 2.0 * vb.m * pr.k1 * pr.k2;
-          p->flops1 += 2.0 * vb.m * pr.k1 * pr.k2;
+          L.flops += 2.0 * vb.m * pr.k1 * pr.k2;
+          L.flops1 += 2.0 * vb.m * pr.k1 * pr.k2;
         }
         if (pr.b_off >= 0) {
-          p->flops += 2.0 * vb.m * pr.k2 * vb.n;
-          p->flops2 += 2.0 * vb.m * pr.k2 * vb.n;
+          L.flops += 2.0 * vb.m * pr.k2 * vb.n;
+          L.flops2 += 2.0 * vb.m * pr.k2 * vb.n;
         }
         const bool f = small_vb && ((pr.a_off >= 0 && pr.k1 <= 32 && pr.k2 <= 32) || (pr.a_off < 0 && pr.b_off < 0));
         (f ? fast : slow).push_back(vb.first + q);
         if (!f)
-          p->flops_split += (pr.a_off >= 0 ? 2.0 * vb.m * pr.k1 * pr.k2 : 0.0) +
-                            (pr.b_off >= 0 ? 2.0 * vb.m * pr.k2 * vb.n : 0.0);
+          L.flops_split += (pr.a_off >= 0 ? 2.0 * vb.m * pr.k1 * pr.k2 : 0.0) +
+                           (pr.b_off >= 0 ? 2.0 * vb.m * pr.k2 * vb.n : 0.0);
         if (f && (pr.k1 > 16 || pr.k2 > 16)) tiny = false;
       }
       // the block's pairs in device order: the small-kernel ones, then the rest
       const int base = (int)perm.size();
       for (int x : fast) {
         perm.push_back(ch.pairs[x]);
-        pc.h_vbi.push_back(-1);  // the small-block kernel reads no slices
+        L.h_vbi.push_back(-1);  // the small-block kernel reads no slices
       }
       for (int x : slow) {
         perm.push_back(ch.pairs[x]);
-        pc.h_vbi.push_back(i);
+        L.h_vbi.push_back(i);
       }
       // cut each list into work units of about SEG_COST(_SMALL) MFMAs per
       // wave.  A block on wide tiles is cut twice: for the FP64 wide tiles
@@ -1713,7 +1734,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           if (usmall[u]) {
             long sc = 0;
             for (int q = 0; q < sg.count; ++q) sc += small_cost(vb.m, vb.n, perm[sg.first + q]);
-            sunits[tiny ? 1 : 0].push_back({sc, si});
+            L.sunits[tiny ? 1 : 0].push_back({sc, si});
             continue;
           }
           long ucost = 0;
@@ -1724,8 +1745,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           gu.ntm = (vb.m + g.tm - 1) / g.tm;
           gu.ntn = (vb.n + g.jv - 1) / g.jv;
           gu.code = g.code;
-          if (mode != 2) gunits[g.cls].push_back(gu);
-          if (mode != 1 && g.cls < 3) gunits_bf[g.cls].push_back(gu);
+          if (mode != 2) L.gunits[g.cls].push_back(gu);
+          if (mode != 1 && g.cls < 3) L.gunits_bf[g.cls].push_back(gu);
         }
         if (nseg > 1) {
           for (int64_t e0 = 0; e0 < mn; e0 += 256) {
@@ -1736,8 +1757,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
             r.mn = (int32_t)mn;
             r.e0 = (int32_t)e0;
             r.len = (int32_t)std::min<int64_t>(256, mn - e0);
-            if (mode != 2) reds[0].push_back(r);
-            if (mode != 1) reds[1].push_back(r);
+            if (mode != 2) L.reds[0].push_back(r);
+            if (mode != 1) L.reds[1].push_back(r);
           }
           poff += nseg * mn;
         }
@@ -1748,6 +1769,78 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
         emit(gm, 1);
         emit(gs, 2);
       }
+      return true;
+    };
+    // ranges of V blocks with about equal pair counts, one per thread
+    int nth = (int)std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 16);
+    if (ch.n_pairs < 20000) nth = 1;
+    nth = std::max(1, std::min(nth, ch.n_vblocks));
+    std::vector<int> bounds(nth + 1, ch.n_vblocks);
+    bounds[0] = 0;
+    {
+      int64_t acc = 0, tot = 0;
+      for (int i = 0; i < ch.n_vblocks; ++i) tot += std::max(0, ch.vblocks[i].count) + 1;
+      int t = 1;
+      for (int i = 0; i < ch.n_vblocks && t < nth; ++i) {
+        acc += std::max(0, ch.vblocks[i].count) + 1;
+        if (acc * nth >= tot * t) bounds[t++] = i + 1;
+      }
+    }
+    std::vector<Local> loc(nth);
+    auto work = [&](int t) {
+      for (int i = bounds[t]; i < bounds[t + 1]; ++i)
+        if (!plan_vblock(i, loc[t])) return;
+    };
+    if (nth == 1) {
+      work(0);
+    } else {
+      std::vector<std::thread> th;
+      for (int t = 1; t < nth; ++t) th.emplace_back(work, t);
+      work(0);
+      for (auto& x : th) x.join();
+    }
+    for (auto& L : loc)
+      if (!L.err.empty()) return cleanup(BSG_ERR_ARG, L.err);
+    // concatenate with the local indices shifted
+    std::vector<GUnit> gunits[NCLS], gunits_bf[3];
+    std::vector<bsg_pair_t> perm;
+    std::vector<std::pair<long, int>> sunits[2];
+    std::vector<Seg> segs;
+    std::vector<RedItem> reds[2];
+    int64_t poff = 0;
+    for (auto& L : loc) {
+      const int pb = (int)perm.size(), sb = (int)segs.size();
+      const int64_t ob = poff;
+      p->flops += L.flops;
+      p->flops1 += L.flops1;
+      p->flops2 += L.flops2;
+      p->flops_split += L.flops_split;
+      p->arena_extent = std::max(p->arena_extent, L.arena_extent);
+      perm.insert(perm.end(), L.perm.begin(), L.perm.end());
+      pc.h_vbi.insert(pc.h_vbi.end(), L.h_vbi.begin(), L.h_vbi.end());
+      for (Seg sg : L.segs) {
+        sg.first += pb;
+        if (sg.p_off >= 0) sg.p_off += ob;
+        segs.push_back(sg);
+      }
+      for (int k = 0; k < 2; ++k)
+        for (auto x : L.sunits[k]) sunits[k].push_back({x.first, x.second + sb});
+      for (int k = 0; k < NCLS; ++k)
+        for (GUnit g : L.gunits[k]) {
+          g.seg += sb;
+          gunits[k].push_back(g);
+        }
+      for (int k = 0; k < 3; ++k)
+        for (GUnit g : L.gunits_bf[k]) {
+          g.seg += sb;
+          gunits_bf[k].push_back(g);
+        }
+      for (int k = 0; k < 2; ++k)
+        for (RedItem r : L.reds[k]) {
+          r.p_off += ob;
+          reds[k].push_back(r);
+        }
+      poff += L.poff;
     }
     pmax = std::max(pmax, poff);
     // every array of the chunk goes into one device allocation with one copy
