@@ -1126,17 +1126,21 @@ struct GUnit {
   long ntiles() const { return (long)ntm * ntn; }
 };
 inline std::vector<TileRef> xcd_order(std::vector<GUnit>& units) {
-  std::vector<int> ord(units.size());
-  for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
-  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-    return units[a].cost * units[a].ntiles() > units[b].cost * units[b].ntiles();
+  // (work, index) pairs sorted by work descending, index ascending on ties
+  std::vector<std::pair<long, int>> ord(units.size());
+  for (size_t i = 0; i < ord.size(); ++i) ord[i] = {units[i].cost * units[i].ntiles(), (int)i};
+  std::sort(ord.begin(), ord.end(), [](const std::pair<long, int>& a, const std::pair<long, int>& b) {
+    return a.first != b.first ? a.first > b.first : a.second < b.second;
   });
   std::vector<std::vector<TileRef>> q(NXCD);
-  std::vector<long> load(NXCD, 0);
-  for (int u : ord) {
-    int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    const GUnit& g = units[u];
-    load[x] += g.cost * g.ntiles();
+  for (auto& v : q) v.reserve(units.size() / NXCD + 16);
+  long load[NXCD] = {};
+  for (const auto& o : ord) {
+    int x = 0;
+    for (int y = 1; y < NXCD; ++y)
+      if (load[y] < load[x]) x = y;
+    const GUnit& g = units[o.second];
+    load[x] += o.first;
     for (int tm = 0; tm < g.ntm; ++tm)
       for (int tn = 0; tn < g.ntn; ++tn) q[x].push_back({g.seg, tm, tn, g.code});
   }
@@ -1847,6 +1851,9 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     // (256-byte aligned parts): one hipMalloc / hipFree per chunk
     std::vector<std::pair<void**, size_t>> parts;  // (device pointer to set, offset in the blob)
     std::vector<char> blob;
+    blob.reserve(sizeof(bsg_vblock_t) * ch.n_vblocks + sizeof(bsg_pair_t) * perm.size() + sizeof(Seg) * segs.size() +
+                 sizeof(RedItem) * (reds[0].size() + reds[1].size()) +
+                 sizeof(int4) * (sunits[0].size() + sunits[1].size() + 8) / 4 + (1 << 16));
     auto up = [&](void** d, const void* h, size_t bytes) {
       if (bytes == 0) return;
       const size_t off = (blob.size() + 255) & ~(size_t)255;
@@ -1865,8 +1872,9 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     // small-block units, costliest first, four per workgroup
     std::vector<int4> sq[2];
     for (int k = 0; k < 2; ++k) {
-      std::stable_sort(sunits[k].begin(), sunits[k].end(),
-                       [](const std::pair<long, int>& x, const std::pair<long, int>& y) { return x.first > y.first; });
+      std::sort(sunits[k].begin(), sunits[k].end(), [](const std::pair<long, int>& x, const std::pair<long, int>& y) {
+        return x.first != y.first ? x.first > y.first : x.second < y.second;
+      });
       for (size_t i = 0; i < sunits[k].size(); i += 4) {
         int e[4];
         for (int j = 0; j < 4; ++j) e[j] = i + j < sunits[k].size() ? sunits[k][i + j].second : -1;
