@@ -110,7 +110,8 @@ double bsg_plan_flops(const bsg_plan* plan);
 /* The same split by GEMM: first (W = op(A) C) and second (V += W op(B)^T). */
 int bsg_plan_phase_flops(const bsg_plan* plan, double* phase1, double* phase2);
 /* Share of the flops that BSG_SPLIT_BF16 runs on BF16 slices (the grouped
- * kernels' pairs; V blocks up to 32 x 32 stay FP64).  No counterpart in the
+ * kernels' pairs; the pairs of V blocks up to 32 x 32 stay FP64, except those
+ * with an identity A and an operator B).  No counterpart in the
  * reference; the device Davidson skips the split phase of its precision
  * ladder below a share of one half. */
 double bsg_plan_split_share(const bsg_plan* plan);

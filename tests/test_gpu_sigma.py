@@ -219,9 +219,10 @@ def test_syn30_four_root_superblock():
 @pytest.mark.parametrize("name,make", [c for c in _cases() if c[0] in ("exact6", "syn24_M500_dmrg")],
                          ids=["exact6", "syn24_M500_dmrg"])
 def test_split_share(name, make):
-    """Share of the flops split mode runs on BF16 slices: none when every V
-    block is at most 32 x 32 (they stay FP64), most of them when the V blocks
-    are hundreds wide."""
+    """Share of the flops split mode runs on BF16 slices: a small one when
+    every V block is at most 32 x 32 (their pairs stay FP64 except those
+    with an identity A and an operator B, which run in the grouped kernel),
+    most of them when the V blocks are hundreds wide."""
     print(f"[gpu] split share {name}", flush=True)
     sb = make()
     share = block_amd.multiply_h(sb).split_share
@@ -229,6 +230,6 @@ def test_split_share(name, make):
     print(f"[gpu] {name} largest V block side {big}, split share {share:.3f}", flush=True)
     assert 0.0 <= share <= 1.0
     if big <= 32:
-        assert share == 0.0
+        assert share < 0.1
     else:
         assert share > 0.5
