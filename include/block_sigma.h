@@ -129,8 +129,9 @@ int64_t bsg_plan_arena_extent(const bsg_plan* plan);
  * transposed slices of the wavefunction's C blocks (6 B per element), which
  * every split-precision application rewrites.  It then slices the arena's
  * operator blocks; call again after the arena moves or its contents change.
- * V blocks of at most 32 x 32 run in FP64 in both modes (the small-block
- * kernel is bound by operand latency, not by its MFMAs).
+ * The small-block kernel's pairs (V blocks of at most 32 x 32 whose A is an
+ * operator, or whose A and B are identities) run in FP64 in both modes (it
+ * is bound by operand latency, not by its MFMAs).
  * No counterpart in the reference (its sigma is FP64 only). */
 int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* plan, const double* arena, int64_t n);
 
@@ -157,8 +158,8 @@ int bsg_plan_set_batch(bsg_context* ctx, bsg_plan* plan, int32_t nroot);
  * in launches of up to the plan's batch size (bsg_plan_set_batch).
  * arena: device operator arena; c, v: device pointers.
  * precision: BSG_FP64, or BSG_SPLIT_BF16 (BF16 slice products on MFMA with
- * FP32 accumulation per 32-deep step and FP64 sums for the V blocks larger
- * than 32 x 32, FP64 for the rest; needs bsg_plan_split_arena on the same
+ * FP32 accumulation per 32-deep step and FP64 sums for the grouped kernels'
+ * pairs, FP64 for the small-block kernel's; needs bsg_plan_split_arena on the same
  * arena first; ~1e-7 relative error). */
 int bsg_sigma_apply(bsg_context* ctx, bsg_plan* plan, const double* arena, const double* c, double* v, int32_t nvec,
                     int32_t precision);
