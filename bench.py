@@ -32,6 +32,7 @@ import json
 import os
 import re
 import resource
+import shutil
 import sys
 import tempfile
 import time
@@ -473,8 +474,16 @@ def main():
         n0 = cfg["ramp_sweeps"]
         cfg = dict(cfg, ramp=[tuple(r) for r in cfg["ramp"]] + [(n0, M, cfg.get("settle_tol", 1e-10), 0.0)],
                    ramp_sweeps=n0 + settle)
-    norb = cfg["norb"]
     tmp = tempfile.mkdtemp(prefix="blkbench_")
+    try:
+        return run_bench(args, cfg, M, settle, tmp)
+    finally:
+        # sweep states are GBs at M >= 800: never leave them behind
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def run_bench(args, cfg, M, settle, tmp):
+    norb = cfg["norb"]
     fcidump = os.path.join(tmp, f"{args.config}.FCIDUMP")
     gen.make_config_fcidump(args.config, fcidump)
     ints = block_amd.Integrals(fcidump)
@@ -485,7 +494,6 @@ def main():
         ramp = ramp_leg(ints, cfg, M, ck, args.verbose)
         log(f"ramp done in {ramp['seconds']} s, kept states at mid-chain {ramp['kept_states_mid']}")
         if args.keep_state:
-            import shutil
             shutil.copyfile(ck, args.keep_state)
     t0 = time.time()
     sb = block_amd.SuperBlock.from_sweep_state(ints, ck, cfg["nelec"], 0, ints.isym)
