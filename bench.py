@@ -435,6 +435,9 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the timed sweeps")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-roots leg")
     ap.add_argument("--state", metavar="FILE", help="start from this saved sweep state instead of the ramp")
+    ap.add_argument("--from-warmup", action="store_true",
+                    help="superblock of the infinite-system warm-up at M (no ramp, no saved state, no timed "
+                         "sweep): the configurations whose sweep states do not fit on disk")
     ap.add_argument("--keep-state", metavar="FILE", help="copy the ramp's saved state here")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run a device copy of N doubles (HBM counter calibration)")
@@ -489,14 +492,21 @@ def run_bench(args, cfg, M, settle, tmp):
     ints = block_amd.Integrals(fcidump)
     ck = args.state or os.path.join(tmp, "ramp.state")
     ramp = None
-    if not args.state:
+    if args.from_warmup:
+        args.no_sweep = True
+        log(f"infinite-system warm-up to M={M} on {args.config}")
+        t0 = time.time()
+        sb = block_amd.SuperBlock.from_dmrg_warmup(ints, cfg["nelec"], 0, ints.isym, M)
+        log(f"warm-up done in {time.time() - t0:.1f} s")
+    elif not args.state:
         log(f"ramp to M={M} on {args.config}")
         ramp = ramp_leg(ints, cfg, M, ck, args.verbose)
         log(f"ramp done in {ramp['seconds']} s, kept states at mid-chain {ramp['kept_states_mid']}")
         if args.keep_state:
             shutil.copyfile(ck, args.keep_state)
     t0 = time.time()
-    sb = block_amd.SuperBlock.from_sweep_state(ints, ck, cfg["nelec"], 0, ints.isym)
+    if not args.from_warmup:
+        sb = block_amd.SuperBlock.from_sweep_state(ints, ck, cfg["nelec"], 0, ints.isym)
     t_build = time.time() - t0
     if args.tune:
         sbs = [sb]
@@ -588,8 +598,9 @@ def run_bench(args, cfg, M, settle, tmp):
         "data": "synthetic",
         "config": {"workload": f"{args.config}: one H.C on the mid-chain two-site superblock, {norb} orbitals / "
                                f"{cfg['nelec']} electrons, S=0, {'D2h' if cfg.get('d2h') else 'C1'}, M={M}, blocks "
-                               f"from a noisy two-site sweep ramp"
-                               f"{f' and {settle} noise-free sweeps' if settle else ''}, FP64 sigma",
+                               + ("from the infinite-system warm-up" if args.from_warmup else
+                                  f"from a noisy two-site sweep ramp{f' and {settle} noise-free sweeps' if settle else ''}")
+                               + ", FP64 sigma",
                    "wave_size": sb.wave_size, "flop_per_step": flops, "terms": sb.nterms,
                    "sectors_L_R_blocks": [nl, nr, nb], "largest_V_block": max(shapes, key=lambda s: s[0] * s[1]),
                    "gemm_pairs": pairs, "operator_arena_GB": arena_gb,
