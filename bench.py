@@ -496,7 +496,8 @@ def run_bench(args, cfg, M, settle, tmp):
         args.no_sweep = True
         log(f"infinite-system warm-up to M={M} on {args.config}")
         t0 = time.time()
-        sb = block_amd.SuperBlock.from_dmrg_warmup(ints, cfg["nelec"], 0, ints.isym, M)
+        # per-step lines (verbose 2) keep a long warm-up visibly alive
+        sb = block_amd.SuperBlock.from_dmrg_warmup(ints, cfg["nelec"], 0, ints.isym, M, verbose=max(2, args.verbose))
         log(f"warm-up done in {time.time() - t0:.1f} s")
     elif not args.state:
         log(f"ramp to M={M} on {args.config}")
