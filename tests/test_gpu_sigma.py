@@ -233,3 +233,27 @@ def test_split_share(name, make):
         assert share < 0.1
     else:
         assert share > 0.5
+
+
+@pytest.mark.timeout(480)
+def test_syn50_superblock():
+    """The 50-orbital / 50-electron C1 synthetic Hamiltonian (the large
+    configuration's system) at M = 300: the mid-chain superblock of its
+    infinite-system warm-up, sigma against the oracle at full size, FP64 and
+    split."""
+    print("[gpu] syn50 M=300 superblock", flush=True)
+    sb = block_amd.SuperBlock.from_dmrg_warmup(config_fcidump("syn50"), 50, 0, 0, 300)
+    print(f"[gpu] syn50 wave {sb.wave_size}, {sb.flops:.3e} flop", flush=True)
+    assert sb.wave_size > 50_000
+    h = block_amd.multiply_h(sb)
+    c = rng_vec(sb.wave_size, 50)
+    ref = _oracle.sigma(sb, c, nthreads=16)
+    got = h(c)
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"[gpu] syn50 FP64 rel={rel:.3e}", flush=True)
+    assert rel <= TOL
+    h.precision = block_amd.SPLIT_BF16
+    gs = h(c)
+    rel_s = np.linalg.norm(gs - ref) / np.linalg.norm(ref)
+    print(f"[gpu] syn50 split rel={rel_s:.3e}", flush=True)
+    assert 0.0 < rel_s <= SPLIT_TOL
