@@ -341,7 +341,7 @@ def fetch_val(d, name):
 TUNE_EDGES = [(1, 32), (33, 64), (65, 128), (129, 256), (257, 512), (513, 1 << 30)]
 
 
-def tune_tiles(sbs, out_path, steps, warmup, label, top=8, min_gain=0.005):
+def tune_tiles(sbs, out_path, steps, warmup, label, top=8, min_gain=0.005, start=(), passes=1):
     """Measured tile table over one or more superblocks: the V blocks are
     bucketed by (rows, cols) ranges; for the buckets with most of the flops
     (about m n (m + n) per block, each superblock's share normalised), every
@@ -350,7 +350,10 @@ def tune_tiles(sbs, out_path, steps, warmup, label, top=8, min_gain=0.005):
     current choice), scored by the sum over superblocks of the time relative
     to the MFMA-count model's, and a bucket keeps the best candidate when it
     beats the model by min_gain.  Writes the rules to out_path and checks
-    that the tuned plans give the same sigma."""
+    that the tuned plans give the same sigma.  start: rules to begin from
+    (a bucket's rule is its first choice, rules of other ranges are kept);
+    passes > 1 repeats the sweep over the buckets (coordinate descent), each
+    bucket decided again with the others on their latest choice."""
     weight = {}
     for sb in sbs:
         w = {}
@@ -371,10 +374,17 @@ def tune_tiles(sbs, out_path, steps, warmup, label, top=8, min_gain=0.005):
         h.replan()
     v0 = [h(c) for h, c in zip(hs, cs)]
     choice = {b: [-1, -1] for b in buckets}
+    edges = {(TUNE_EDGES[bm][0], TUNE_EDGES[bm][1], TUNE_EDGES[bn][0], TUNE_EDGES[bn][1]): (bm, bn) for bm, bn in buckets}
+    keep = []  # start rules of ranges that are not tuned buckets
+    for r in start:
+        if tuple(r[:4]) in edges:
+            choice[edges[tuple(r[:4])]] = [r[4], r[5]]
+        else:
+            keep.append(tuple(r))
 
     def rules():
         return [(TUNE_EDGES[bm][0], TUNE_EDGES[bm][1], TUNE_EDGES[bn][0], TUNE_EDGES[bn][1], f, sp)
-                for (bm, bn), (f, sp) in choice.items() if f >= 0 or sp >= 0]
+                for (bm, bn), (f, sp) in choice.items() if f >= 0 or sp >= 0] + keep
 
     def times():
         block_amd.set_tile_table(rules())
@@ -385,7 +395,7 @@ def tune_tiles(sbs, out_path, steps, warmup, label, top=8, min_gain=0.005):
         return out
 
     log_rows = []
-    for slot, prec in ((0, block_amd.FP64), (1, block_amd.SPLIT_BF16)):
+    for slot, prec in [x for _ in range(passes) for x in ((0, block_amd.FP64), (1, block_amd.SPLIT_BF16))]:
         for h in hs:
             h.precision = prec
         for b in buckets:
@@ -416,7 +426,8 @@ def tune_tiles(sbs, out_path, steps, warmup, label, top=8, min_gain=0.005):
         f.write("# m_lo m_hi n_lo n_hi fp64 split: tile geometry per V block shape range (-1: MFMA-count model)\n")
         f.write(f"# measured by bench.py --tune on {label} ({', '.join(str(sb.wave_size) for sb in sbs)}-element\n"
                 f"# superblocks): every superblock's whole sigma timed per candidate (min of 3 x {steps}\n"
-                f"# applications), kept if the mean time relative to the model's is >{min_gain:.1%} lower\n")
+                f"# applications), kept if the mean time relative to the model's is >{min_gain:.1%} lower;\n"
+                f"# fp64 candidate {block_amd.tile_candidates(block_amd.FP64) - 1} is the per-wave tile kernel\n")
         for r in rules():
             f.write(" ".join(str(x) for x in r) + "\n")
     return {"rules": rules(), "trials": log_rows, "sigma_rel_diff_tuned_vs_model": err}
@@ -449,6 +460,9 @@ def main():
     ap.add_argument("--settle", type=int, default=None,
                     help="noise-free sweeps at M after the ramp, before the timed sweep (default: the config's)")
     ap.add_argument("--tune", metavar="OUT", help="measure the tile table on this superblock and write it to OUT")
+    ap.add_argument("--tune-passes", type=int, default=1, help="passes of the tuner over the buckets")
+    ap.add_argument("--tune-from", default="", metavar="TABLE",
+                    help="tile table the tuner starts from (its bucket rules are first choices, the rest kept)")
     ap.add_argument("--tune-with", default="", metavar="CONFIGS",
                     help="comma-separated further configurations whose superblocks the tuner also times")
     ap.add_argument("--verbose", type=int, default=1, help="sweep log level (2: per step)")
@@ -521,8 +535,13 @@ def run_bench(args, cfg, M, settle, tmp):
             ramp_leg(eints, ecfg, ecfg["M"], eck, args.verbose)
             sbs.append(block_amd.SuperBlock.from_sweep_state(eints, eck, ecfg["nelec"], 0, eints.isym))
         label = f"--config {args.config}" + (f" --M {args.M}" if args.M else "") + \
-            (f" --tune-with {args.tune_with}" if args.tune_with else "")
-        print(json.dumps({"tune": tune_tiles(sbs, args.tune, args.steps, args.warmup, label)}))
+            (f" --tune-with {args.tune_with}" if args.tune_with else "") + \
+            (f" --tune-from {args.tune_from}" if args.tune_from else "")
+        start = block_amd.read_tile_table(args.tune_from) if args.tune_from else ()
+        if args.tune_passes > 1:
+            label += f" --tune-passes {args.tune_passes}"
+        print(json.dumps({"tune": tune_tiles(sbs, args.tune, args.steps, args.warmup, label, start=start,
+                                             passes=args.tune_passes)}))
         return 0
     h = block_amd.multiply_h(sb, device=0)
     c = np.random.default_rng(0).standard_normal(sb.wave_size)

@@ -981,6 +981,183 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SB == 1 ? 4
     }
 }
 
+// ---------------------------------------------------------------- FP64 per-wave tiles
+// Medium V blocks are latency-bound in the grouped kernel too: a pair's
+// MFMAs are few next to its operand round trips, and a 4-wave tile runs one
+// chain of round trips for 4 waves.  Here, as in the small-block kernel, each
+// wavefront owns one tile of a V block (16 rows x 16 NB columns) for one work
+// unit and walks the unit's pairs alone -- no LDS, no barriers, four
+// unrelated tiles per workgroup -- so a CU keeps four times as many operand
+// streams in flight.  A pair's op(A) row panel (16 rows x k1, k1 <= 4 WKA)
+// sits in registers; W^T = C^T op(A)^T is formed 16 W columns at a time and
+// its result registers are at once GEMM 2's A operand against the matching
+// 16 rows of B'.  The C operands of the next 16 W columns are requested
+// before the MFMAs of the current ones (two register sets, used in turn),
+// the B' operands of a column group before its GEMM 1 MFMAs, and the next
+// pair's descriptor before this pair's operands.
+// Identity-B' pairs use the direct orientation W = op(A) C, whose result
+// layout is the V accumulators'; identity-A pairs load W = alpha C straight
+// into GEMM 2's operand layout.  Operands past a block's rows, columns or
+// contraction depth read as zero (buffer range checks and masked lanes), and
+// every C column past k2 is masked, so B' rows past k2 only meet zero W.
+constexpr int WKA = 16;  // 4-deep K steps of op(A) held in registers (k1 <= 64)
+template <int NB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_wave(
+    const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
+    const TileRef* __restrict__ tiles, int ntiles, const double* __restrict__ arena, const double* __restrict__ c,
+    double* __restrict__ v, double* __restrict__ part, RootSet rs) {
+  int g;
+  const int root = root_of(blockIdx.x, rs.n, g);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tid = g * 4 + w;
+  if (tid >= ntiles) return;
+  const TileRef tr = tiles[tid];
+  if (tr.item < 0) return;  // padding: no tile for this wave (waves never synchronise here)
+  c += root * rs.wstride;
+  v += root * rs.wstride;
+  part += root * rs.pstride;
+  const Seg sg = segs[tr.item];
+  const bsg_vblock_t vb = vbs[sg.vb];
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int row0 = tr.tm * 16, col0 = tr.tn * 16 * NB;
+  const bool rowok = row0 + li < vb.m;  // op(A) / W row li of the tile inside the block
+  dbl4 accV[NB];  // register i of lane (li, lk): V(row0 + lk + 4 i, col0 + 16 t + li)
+#pragma unroll
+  for (int t = 0; t < NB; ++t) accV[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  bsg_pair_t nxt = prs[sg.first];
+  for (int q = 0; q < sg.count; ++q) {
+    const bsg_pair_t pr = nxt;
+    if (q + 1 < sg.count) nxt = prs[sg.first + q + 1];
+    const __amdgpu_buffer_rsrc_t rC = block_rsrc(c + pr.c_off, (int64_t)pr.k1 * pr.k2);
+    const uint32_t crow = (uint32_t)pr.k2 * 8u;
+    if (pr.a_off < 0 && pr.b_off < 0) {  // identity A and B': V += alpha C
+#pragma unroll
+      for (int t = 0; t < NB; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = row0 + lk + 4 * i, col = col0 + t * 16 + li;
+          const bool ok = row < vb.m && col < vb.n;
+          accV[t][i] += pr.alpha * bload(rC, ok ? (uint32_t)(row * pr.k2 + col) * 8u : OOB);
+        }
+      continue;
+    }
+    const int nk1 = (pr.k1 + 3) / 4;
+    double a[WKA];  // a[s] = op(A)(row0 + li, 4 s + lk)
+    if (pr.a_off >= 0) {
+      const __amdgpu_buffer_rsrc_t rA = block_rsrc(arena + pr.a_off, (int64_t)vb.m * pr.k1);
+      const int ar = pr.a_trans ? 1 : pr.k1, ak = pr.a_trans ? vb.m : 1;
+      const uint32_t off = rowok ? (uint32_t)((row0 + li) * ar + lk * ak) * 8u : OOB;
+#pragma unroll
+      for (int s = 0; s < WKA; ++s) a[s] = bload2(rA, s < nk1 ? off : OOB, (uint32_t)(s * ak) * 32u);
+    }
+    const double aw = pr.alpha_a * pr.alpha;
+    if (pr.b_off < 0) {
+      // identity B' (k2 = n): V(:, block t) += alpha alpha_a op(A) C(:, block t)
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const int col = col0 + t * 16 + li;
+        if (col0 + t * 16 >= vb.n) break;
+        const uint32_t coff = col < pr.k2 ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
+        double ct[WKA];
+#pragma unroll
+        for (int s = 0; s < WKA; ++s) ct[s] = bload2(rC, s < nk1 ? coff : OOB, (uint32_t)(4 * s) * crow);
+        dbl4 accW = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < WKA; s4 += 4)
+          if (s4 < nk1)
+#pragma unroll
+            for (int s = s4; s < s4 + 4; ++s) accW = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], ct[s], accW, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) accV[t][i] += aw * accW[i];
+      }
+      continue;
+    }
+    // B'(k, col) at k ks + col cs inside the block; per-lane offsets of the
+    // tile's columns, the K position in the uniform offset
+    const __amdgpu_buffer_rsrc_t rB = block_rsrc(arena + pr.b_off, (int64_t)vb.n * pr.k2);
+    const int ks = pr.b_trans ? 1 : vb.n, cs = pr.b_trans ? pr.k2 : 1;
+    uint32_t boff[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      const int col = col0 + t * 16 + li;
+      boff[t] = col < vb.n ? (uint32_t)(lk * ks + col * cs) * 8u : OOB;
+    }
+    const int njb = (pr.k2 + 15) / 16;
+    if (pr.a_off < 0) {
+      // identity A (k1 = m): W(row0 + li, 16 jb + 4 i + lk) = alpha C(...)
+      for (int jb = 0; jb < njb; ++jb) {
+        double wa[4], b[4][NB];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int k = jb * 16 + 4 * i + lk;
+          wa[i] = pr.alpha * bload(rC, rowok && k < pr.k2 ? (uint32_t)((row0 + li) * pr.k2 + k) * 8u : OOB);
+#pragma unroll
+          for (int t = 0; t < NB; ++t) b[i][t] = bload2(rB, boff[t], (uint32_t)((jb * 16 + 4 * i) * ks) * 8u);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int t = 0; t < NB; ++t) accV[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[i], b[i][t], accV[t], 0, 0, 0);
+      }
+      continue;
+    }
+    // W^T 16 columns at a time: ct[s] = C(4 s + lk, 16 jb + li) (columns past
+    // k2 masked), b[i][t] = B'(16 jb + 4 i + lk, col0 + 16 t + li); the B'
+    // operands of a column group are requested before its GEMM 1 MFMAs
+    auto load_jb = [&](int jb, double (&ct)[WKA]) {
+      const int col = jb * 16 + li;
+      const uint32_t coff = col < pr.k2 ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
+#pragma unroll
+      for (int s = 0; s < WKA; ++s) ct[s] = bload2(rC, s < nk1 ? coff : OOB, (uint32_t)(4 * s) * crow);
+    };
+    auto run_jb = [&](int jb, const double (&ct)[WKA]) {
+      double b[4][NB];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int t = 0; t < NB; ++t) b[i][t] = bload2(rB, boff[t], (uint32_t)((jb * 16 + 4 * i) * ks) * 8u);
+      // register i of lane (li, lk): W(row0 + li, 16 jb + lk + 4 i)
+      dbl4 accT = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s4 = 0; s4 < WKA; s4 += 4)
+        if (s4 < nk1)
+#pragma unroll
+          for (int s = s4; s < s4 + 4; ++s) accT = __builtin_amdgcn_mfma_f64_16x16x4f64(ct[s], a[s], accT, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const double wv = aw * accT[i];
+#pragma unroll
+        for (int t = 0; t < NB; ++t) accV[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(wv, b[i][t], accV[t], 0, 0, 0);
+      }
+    };
+    double ct0[WKA], ct1[WKA];
+    load_jb(0, ct0);
+    for (int jb = 0; jb < njb; jb += 2) {
+      if (jb + 1 < njb) load_jb(jb + 1, ct1);
+      run_jb(jb, ct0);
+      if (jb + 1 >= njb) break;
+      if (jb + 2 < njb) load_jb(jb + 2, ct0);
+      run_jb(jb + 1, ct1);
+    }
+  }
+  const bool direct = sg.p_off < 0;
+  double* out = direct ? v + vb.v_off : part + sg.p_off;
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const int col = col0 + t * 16 + li;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = row0 + lk + 4 * i;
+      if (row < vb.m && col < vb.n) {
+        if (direct)
+          out[(size_t)row * vb.n + col] += accV[t][i];
+        else
+          out[(size_t)row * vb.n + col] = accV[t][i];
+      }
+    }
+  }
+}
+
 
 
 // Tile geometry of the grouped kernels for an m x n V block: waves WR x WC
@@ -1099,6 +1276,30 @@ inline long small_cost(int m, int n, const bsg_pair_t& pr) {
   const long g1 = pr.a_off < 0 ? 0 : (pr.b_off < 0 ? nnb : njb) * cdiv(pr.k1, 4) * nrb;
   const long g2 = pr.b_off < 0 ? 0 : njb * 4 * nrb * nnb;
   return g1 + g2 + 24;
+}
+// Per-wave tiles: 16 x 16 NBW.  In the FP64 column of the tile table the
+// candidate index after the geometries selects them for a V block's pairs
+// within their contraction depth (k1 <= 4 WKA, or identity A); the block's
+// other pairs run on the geometry the model picks.  Without a rule, blocks
+// stay on the grouped kernel.
+constexpr int NBW = 2;
+inline int wave_candidate() {
+  static const int n = (int)geom_candidates(6).size();
+  return n;
+}
+inline bool wave_block(int m, int n) {
+  if (std::max(m, n) <= 32) return false;  // the small-block kernel's
+  for (const TileRule& r : g_tile_rules)
+    if (m >= r.m_lo && m <= r.m_hi && n >= r.n_lo && n <= r.n_hi) return r.fp64 == wave_candidate();
+  return false;
+}
+// MFMA instructions of one pair in a per-wave tile plus a latency allowance
+// (segment cost)
+inline long wave_cost(int n, const bsg_pair_t& pr) {
+  auto cdiv = [](long a, long b) { return (a + b - 1) / b; };
+  const long nnb = std::min<long>(NBW, cdiv(n, 16)), njb = cdiv(pr.k2, 16), nk1 = cdiv(pr.k1, 4);
+  if (pr.a_off < 0) return (pr.b_off < 0 ? 0 : njb * 4 * nnb) + 24;
+  return (pr.b_off < 0 ? nnb * nk1 : njb * (nk1 + 4 * nnb)) + 24;
 }
 // MFMA instructions per wavefront of one pair in a tile (segment cost)
 inline long pair_cost(const Geom& g, int n, const bsg_pair_t& pr) {
@@ -1436,6 +1637,8 @@ struct PlanChunk {
   int n_gbf[3] = {};
   int4* d_s[2] = {nullptr, nullptr};  // FP64 small-block units, 4 per workgroup (SB = 2, SB = 1)
   int n_s[2] = {0, 0};
+  TileRef* d_w = nullptr;  // FP64 per-wave tiles, 4 per workgroup (item -1: none), a multiple of 32
+  int n_w = 0;
   RedItem* d_red[2] = {nullptr, nullptr};  // after the FP64 / the split-precision kernels
   int n_red[2] = {0, 0};
 };
@@ -1613,6 +1816,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
   int64_t pmax = 0;
   const long SEG_COST = 2048;        // MFMAs per wavefront of a work unit before its pair list is split
   const long SEG_COST_SMALL = 1024;  // the same for the per-wave small-block units
+  const long SEG_COST_WAVE = 2048;   // and for the per-wave tiles
   auto cleanup = [&](int code, const std::string& m) {
     bsg_plan_destroy(p);
     return fail(code, m);
@@ -1629,6 +1833,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       std::vector<bsg_pair_t> perm;                   // pairs in device order (grouped per V block as given)
       std::vector<int> h_vbi;
       std::vector<std::pair<long, int>> sunits[2];  // (cost, segment) of small-block units
+      std::vector<GUnit> wunits;                    // per-wave tile units (cost per tile)
       std::vector<Seg> segs;
       std::vector<RedItem> reds[2];  // ordered reductions after the FP64 / the split-precision kernels
       int64_t poff = 0, arena_extent = 0;
@@ -1636,7 +1841,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       std::string err;
       std::vector<int> fast, slow;  // per V block, reused
       std::vector<std::pair<int, int>> units;
-      std::vector<char> usmall;
+      std::vector<char> ukind;  // per unit: 0 grouped kernel, 1 small-block kernel, 2 per-wave tiles
     };
     auto plan_vblock = [&](int i, Local& L) -> bool {
       std::vector<bsg_pair_t>& perm = L.perm;
@@ -1645,7 +1850,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       auto& fast = L.fast;
       auto& slow = L.slow;
       auto& units = L.units;
-      auto& usmall = L.usmall;
+      auto& ukind = L.ukind;
       const auto& vb = ch.vblocks[i];
       if (vb.m <= 0 || vb.n <= 0 || vb.v_off < 0 || vb.v_off + (int64_t)vb.m * vb.n > wave_size || vb.first < 0 ||
           vb.count < 0 || vb.first + vb.count > ch.n_pairs) {
@@ -1658,6 +1863,9 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       // whose C block is larger than that (or whose A is the identity and B
       // is not): those run in the grouped kernel with its K pipeline
       const bool small_vb = std::max(vb.m, vb.n) <= 32;
+      // medium blocks the tile table puts on per-wave tiles (FP64 only): the
+      // pairs within the per-wave kernel's contraction depth go there
+      const bool wave_vb = !small_vb && wave_block(vb.m, vb.n);
       // tiny: the block and every small-kernel pair's C block within 16 x 16
       bool tiny = std::max(vb.m, vb.n) <= 16;
       fast.clear();  // pair indices (absolute) in term order
@@ -1679,18 +1887,20 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           L.flops += 2.0 * vb.m * pr.k2 * vb.n;
           L.flops2 += 2.0 * vb.m * pr.k2 * vb.n;
         }
-        const bool f = small_vb && ((pr.a_off >= 0 && pr.k1 <= 32 && pr.k2 <= 32) || (pr.a_off < 0 && pr.b_off < 0));
+        const bool f = small_vb ? (pr.a_off >= 0 && pr.k1 <= 32 && pr.k2 <= 32) || (pr.a_off < 0 && pr.b_off < 0)
+                                : wave_vb && (pr.a_off < 0 || pr.k1 <= 4 * WKA);
         (f ? fast : slow).push_back(vb.first + q);
-        if (!f)
+        if (!f || wave_vb)  // split mode: every pair of a per-wave block runs on BF16 slices
           L.flops_split += (pr.a_off >= 0 ? 2.0 * vb.m * pr.k1 * pr.k2 : 0.0) +
                            (pr.b_off >= 0 ? 2.0 * vb.m * pr.k2 * vb.n : 0.0);
-        if (f && (pr.k1 > 16 || pr.k2 > 16)) tiny = false;
+        if (f && small_vb && (pr.k1 > 16 || pr.k2 > 16)) tiny = false;
       }
-      // the block's pairs in device order: the small-kernel ones, then the rest
+      // the block's pairs in device order: the small-kernel (per-wave
+      // kernel) ones, then the rest
       const int base = (int)perm.size();
       for (int x : fast) {
         perm.push_back(ch.pairs[x]);
-        L.h_vbi.push_back(-1);  // the small-block kernel reads no slices
+        L.h_vbi.push_back(wave_vb ? i : -1);  // the small-block kernel reads no slices
       }
       for (int x : slow) {
         perm.push_back(ch.pairs[x]);
@@ -1702,29 +1912,33 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       // with its own partial tiles and reduction); other blocks share one
       // cut (mode 0).  Wide blocks have no small-kernel pairs.
       const int64_t mn = (int64_t)vb.m * vb.n;
-      auto emit = [&](const Geom& g, int mode) {
+      // nf: the first nf pairs go to the kernel of kind fk (1 small-block, 2
+      // per-wave tiles), the rest to the grouped kernel on geometry g
+      auto emit = [&](const Geom& g, int mode, int nf, int fk) {
         units.clear();  // (first, count) in device order
-        usmall.clear();
-        auto cut = [&](int first, int n, bool sm) {
+        ukind.clear();
+        auto cut = [&](int first, int n, int kind) {
+          const long lim = kind == 1 ? SEG_COST_SMALL : (kind == 2 ? SEG_COST_WAVE : SEG_COST);
           int start = 0;
           long cost = 0;
           for (int q = 0; q < n; ++q) {
             const auto& pr = perm[first + q];
-            cost += sm ? small_cost(vb.m, vb.n, pr) : pair_cost(g, vb.n, pr);
-            if (cost >= (sm ? SEG_COST_SMALL : SEG_COST) && q + 1 < n) {
+            cost += kind == 1 ? small_cost(vb.m, vb.n, pr) : (kind == 2 ? wave_cost(vb.n, pr) : pair_cost(g, vb.n, pr));
+            if (cost >= lim && q + 1 < n) {
               units.push_back({first + start, q + 1 - start});
-              usmall.push_back(sm);
+              ukind.push_back((char)kind);
               start = q + 1;
               cost = 0;
             }
           }
           if (n > start) {
             units.push_back({first + start, n - start});
-            usmall.push_back(sm);
+            ukind.push_back((char)kind);
           }
         };
-        cut(base, (int)fast.size(), true);
-        cut(base + (int)fast.size(), (int)slow.size(), false);
+        const int ntot = (int)(fast.size() + slow.size());
+        cut(base, nf, fk);
+        cut(base + nf, ntot - nf, 0);
         const int nseg = (int)units.size();
         for (int u = 0; u < nseg; ++u) {
           Seg sg;
@@ -1735,10 +1949,21 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           sg.p_off = nseg == 1 ? -1 : poff + (int64_t)u * mn;
           int si = (int)segs.size();
           segs.push_back(sg);
-          if (usmall[u]) {
+          if (ukind[u] == 1) {
             long sc = 0;
             for (int q = 0; q < sg.count; ++q) sc += small_cost(vb.m, vb.n, perm[sg.first + q]);
             L.sunits[tiny ? 1 : 0].push_back({sc, si});
+            continue;
+          }
+          if (ukind[u] == 2) {
+            GUnit gu;
+            gu.cost = 0;
+            for (int q = 0; q < sg.count; ++q) gu.cost += wave_cost(vb.n, perm[sg.first + q]);
+            gu.seg = si;
+            gu.ntm = (vb.m + 15) / 16;
+            gu.ntn = (vb.n + 16 * NBW - 1) / (16 * NBW);
+            gu.code = 0;
+            L.wunits.push_back(gu);
             continue;
           }
           long ucost = 0;
@@ -1767,11 +1992,15 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           poff += nseg * mn;
         }
       };
-      if (gm.code == gs.code && gm.nv == gs.nv) {
-        emit(gm, 0);
+      const int nfast = (int)fast.size();
+      if (wave_vb) {
+        emit(gm, 1, nfast, 2);
+        emit(gs, 2, 0, 0);
+      } else if (gm.code == gs.code && gm.nv == gs.nv) {
+        emit(gm, 0, nfast, 1);
       } else {
-        emit(gm, 1);
-        emit(gs, 2);
+        emit(gm, 1, nfast, 1);
+        emit(gs, 2, nfast, 1);
       }
       return true;
     };
@@ -1809,6 +2038,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     std::vector<GUnit> gunits[NCLS], gunits_bf[3];
     std::vector<bsg_pair_t> perm;
     std::vector<std::pair<long, int>> sunits[2];
+    std::vector<GUnit> wunits;
     std::vector<Seg> segs;
     std::vector<RedItem> reds[2];
     int64_t poff = 0;
@@ -1829,6 +2059,10 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       }
       for (int k = 0; k < 2; ++k)
         for (auto x : L.sunits[k]) sunits[k].push_back({x.first, x.second + sb});
+      for (GUnit g : L.wunits) {
+        g.seg += sb;
+        wunits.push_back(g);
+      }
       for (int k = 0; k < NCLS; ++k)
         for (GUnit g : L.gunits[k]) {
           g.seg += sb;
@@ -1886,6 +2120,34 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     for (int k = 0; k < NCLS; ++k) {
       up((void**)&pc.d_g[k], gtiles[k].data(), sizeof(TileRef) * gtiles[k].size());
       pc.n_g[k] = (int)gtiles[k].size();
+    }
+    // per-wave tiles: units (costliest first) go to the least loaded of the
+    // 8 XCDs, a unit's tiles in consecutive wave slots of that XCD's
+    // workgroups (they share the unit's C and B' blocks in its L2);
+    // workgroup 8 j + x is XCD x's j-th
+    {
+      std::vector<std::pair<long, int>> order;
+      for (int u = 0; u < (int)wunits.size(); ++u)
+        order.push_back({wunits[u].cost * wunits[u].ntm * wunits[u].ntn, u});
+      std::sort(order.begin(), order.end(), [](const std::pair<long, int>& x, const std::pair<long, int>& y) {
+        return x.first != y.first ? x.first > y.first : x.second < y.second;
+      });
+      std::vector<TileRef> q[8];
+      long load[8] = {};
+      for (const auto& o : order) {
+        const int x = (int)(std::min_element(load, load + 8) - load);
+        const GUnit& gu = wunits[o.second];
+        for (int tm = 0; tm < gu.ntm; ++tm)
+          for (int tn = 0; tn < gu.ntn; ++tn) q[x].push_back(TileRef{gu.seg, tm, tn, 0});
+        load[x] += o.first;
+      }
+      size_t nwg = 0;
+      for (auto& e : q) nwg = std::max(nwg, (e.size() + 3) / 4);
+      std::vector<TileRef> wt(nwg * 32, TileRef{-1, 0, 0, 0});
+      for (int x = 0; x < 8; ++x)
+        for (size_t e = 0; e < q[x].size(); ++e) wt[((e / 4) * 8 + x) * 4 + e % 4] = q[x][e];
+      up((void**)&pc.d_w, wt.data(), sizeof(TileRef) * wt.size());
+      pc.n_w = (int)wt.size();
     }
     for (int k = 0; k < 3; ++k) {
       up((void**)&pc.d_gbf[k], gtiles_bf[k].data(), sizeof(TileRef) * gtiles_bf[k].size());
@@ -2085,7 +2347,7 @@ int bsg_set_tile_table(const int32_t* rules, int32_t n) {
   for (int32_t i = 0; i < n; ++i) {
     const int32_t* e = rules + 6 * i;
     TileRule r{e[0], e[1], e[2], e[3], e[4], e[5]};
-    if (r.m_lo > r.m_hi || r.n_lo > r.n_hi || r.fp64 < -1 || r.fp64 >= nfp || r.split < -1 || r.split >= nsp)
+    if (r.m_lo > r.m_hi || r.n_lo > r.n_hi || r.fp64 < -1 || r.fp64 > nfp || r.split < -1 || r.split >= nsp)
       return fail(BSG_ERR_ARG, "bsg_set_tile_table: bad rule");
     t.push_back(r);
   }
@@ -2094,7 +2356,8 @@ int bsg_set_tile_table(const int32_t* rules, int32_t n) {
   return BSG_OK;
 }
 int bsg_tile_candidates(int32_t precision) {
-  return (int)geom_candidates(precision == BSG_SPLIT_BF16 ? 4 : 6).size();
+  // FP64: the geometries, then the per-wave kernel
+  return precision == BSG_SPLIT_BF16 ? (int)geom_candidates(4).size() : (int)geom_candidates(6).size() + 1;
 }
 
 int bsg_plan_set_batch(bsg_context* ctx, bsg_plan* p, int32_t nroot) {
@@ -2191,6 +2454,11 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
       ++nl;
     }
   } else {
+    if (ch.n_w > 0) {
+      hipLaunchKernelGGL((k_wave<NBW>), grid(ch.n_w / 4), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_w, ch.n_w,
+                         arena, c, v, part, rs);
+      ++nl;
+    }
     if (ch.n_g[3] > 0) {
       hipLaunchKernelGGL((k_sigma<1, 6, PB, 8>), grid(ch.n_g[3]), dim3(512), 0, st, ch.d_vb, ch.d_pr, ch.d_seg,
                          ch.d_g[3], ch.n_g[3], arena, c, v, part, rs);

@@ -141,7 +141,12 @@ int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* plan, const double* arena, 
  * (split-precision kernels) of the candidate list, -1 leaving the MFMA-count
  * model's choice; the first matching rule applies.  Process-wide, read when
  * plans are created (n = 0 clears).  bsg_tile_candidates gives the list's
- * length for a precision.  The tuner that measures them is bench.py --tune;
+ * length for a precision.  For FP64 its last entry is not a geometry but the
+ * per-wave tile kernel (one wavefront per 16 x 32 V tile, no LDS or barriers)
+ * for a block's pairs with k1 <= 64 or an identity A; the block's other pairs
+ * keep the model's geometry, and split mode keeps the grouped BF16 kernel
+ * for the whole block.  Blocks of at most 32 x 32 stay on the small-block
+ * kernel whatever the rule.  The tuner that measures them is bench.py --tune;
  * block_amd/tile_table.txt holds the committed result. */
 int bsg_set_tile_table(const int32_t* rules, int32_t n);
 int bsg_tile_candidates(int32_t precision);

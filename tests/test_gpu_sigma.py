@@ -1,5 +1,6 @@
 """GPU multiply_h through the C-ABI against the CPU oracle (direct
 restatement of multiplyH / TensorMultiply): <= 1e-12 relative per sigma."""
+import os
 import sys
 
 import numpy as np
@@ -257,3 +258,63 @@ def test_syn50_superblock():
     rel_s = np.linalg.norm(gs - ref) / np.linalg.norm(ref)
     print(f"[gpu] syn50 split rel={rel_s:.3e}", flush=True)
     assert 0.0 < rel_s <= SPLIT_TOL
+
+
+def _wave_cases():
+    # superblocks with V blocks above 32 rows or columns (S = 0 and S = 1)
+    return [c for c in _cases() if c[0] in ("hubbard12_M100", "syn24_M500_dmrg")] + [
+        ("hubbard12_M100_S1", lambda: block_amd.SuperBlock(config_fcidump("hubbard12"), 12, 2, 0, 5, 100)),
+        ("syn16_M200_S1", lambda: block_amd.SuperBlock(random_fcidump(16, 21), 16, 2, 0, 7, 200)),
+    ]
+
+
+@pytest.mark.parametrize("name,make", _wave_cases(), ids=[c[0] for c in _wave_cases()])
+def test_wave_tiles(name, make):
+    """The per-wave tile kernel (the FP64 tile-table candidate after the
+    geometries) on every V block above 32 rows or columns: sigma against the
+    oracle at 1e-12, a different summation order from the grouped kernel
+    (so the per-wave kernel ran), bitwise reproducible, four roots batched
+    bitwise equal to single applications, every graph replay equal to the
+    eager sigma, and split precision (the grouped BF16 kernel keeps these
+    blocks, their pairs in the per-wave order) within its tolerance."""
+    print(f"[gpu] wave tiles {name}", flush=True)
+    sb = make()
+    big = max(max(m, n) for m, n in sb.block_shapes())
+    assert big > 32
+    c = np.stack([rng_vec(sb.wave_size, 50 + i) for i in range(4)])
+    ref = _oracle.sigma(sb, c[0], nthreads=8)
+    h = block_amd.multiply_h(sb, max_vec=4)
+    grouped = h(c[0])
+    h.precision = block_amd.SPLIT_BF16
+    split = h(c[0])
+    h.precision = block_amd.FP64
+    wave = block_amd.tile_candidates(block_amd.FP64) - 1
+    block_amd.set_tile_table([(1, 1 << 20, 1, 1 << 20, wave, -1)])
+    try:
+        h.replan()
+        got = h(c[0])
+        rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        print(f"[gpu] {name} largest V block side {big}, wave-tile rel={rel:.3e}", flush=True)
+        assert rel <= TOL
+        assert not np.array_equal(got, grouped)
+        assert np.array_equal(h(c[0]), got)
+        h.set_batch(4)
+        batch = h(c)
+        h.set_batch(1)
+        for i in range(4):
+            assert np.array_equal(batch[i], h(c[i])), i
+        h.set_graph(True)
+        for rep in range(3):
+            assert np.array_equal(h(c[0]), got), rep
+            h.set_graph(False)
+            assert np.array_equal(h(c[1]), batch[1])
+            h.set_graph(True)
+        h.set_graph(False)
+        h.precision = block_amd.SPLIT_BF16
+        got_s = h(c[0])
+        rel_s = np.linalg.norm(got_s - ref) / np.linalg.norm(ref)
+        assert 0.0 < rel_s <= SPLIT_TOL
+        assert np.linalg.norm(got_s - split) <= SPLIT_TOL * np.linalg.norm(ref)
+        h.precision = block_amd.FP64
+    finally:
+        block_amd.set_tile_table(block_amd.read_tile_table() if os.path.exists(block_amd.TILE_TABLE) else [])
