@@ -256,7 +256,11 @@ def summarize_trace(d, last, pattern):
               f"{r['LDS_Block_Size']} | {r['Scratch_Size']} |")
     # the fused kernels of one H.C run concurrently on two streams: the span
     # from the first start to the last end of the i-th last dispatch of each
-    for label, names in (("FP64", ("k_small<2, false>", "k_sigma<")), ("split", ("k_small<2, true>", "k_sigma_bf"))):
+    # (the small-block kernel runs in both precision modes: counted only in a
+    # trace without split-precision calls)
+    any_split = any("k_sigma_bf" in k for k in by)
+    fp64_names = ("k_sigma<", "k_wave<") + (() if any_split else ("k_small<",))
+    for label, names in (("FP64", fp64_names), ("split", ("k_sigma_bf",))):
         groups = [rs for k, rs in by.items() if any(n in k for n in names)]
         if not groups or min(len(g) for g in groups) == 0:
             continue
@@ -467,7 +471,7 @@ def main():
                     help="comma-separated further configurations whose superblocks the tuner also times")
     ap.add_argument("--verbose", type=int, default=1, help="sweep log level (2: per step)")
     ap.add_argument("--last", type=int, default=6, help="dispatches per kernel to summarise (the last ones)")
-    ap.add_argument("--kernels", default="k_sigma|k_small|k_reduce", help="regex of the kernel names to summarise")
+    ap.add_argument("--kernels", default="k_sigma|k_wave|k_small|k_reduce", help="regex of the kernel names to summarise")
     args = ap.parse_args()
     if args.summarize_trace:
         return summarize_trace(args.summarize_trace, args.last, args.kernels) or 0
@@ -552,13 +556,14 @@ def run_bench(args, cfg, M, settle, tmp):
     f1, f2 = h.phase_flops()
     ms_per_step = wall_ms / args.steps
     value = flops * args.steps / (wall_ms * 1e-3) / 1e12
-    # dominant kernels: the fused GEMM-pair kernels of one H.C (grouped tiles
-    # on the sigma stream, small blocks and basic tiles beside them), timed
+    # dominant kernels: the fused GEMM-pair kernels of one H.C (grouped and
+    # per-wave tiles on the sigma stream, small blocks and basic tiles beside
+    # them), timed
     # fork to join with HIP events on the sigma stream
     achieved = (f1 + f2) * args.steps / (kern_ms[0] * 1e-3) / 1e12
     traffic = traffic_from_profile(args.config, M, sb.wave_size, h.pairs)
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_MFMA, 5), "traffic": traffic, "kernel": "k_sigma+k_small",
+                "frac": round(achieved / PEAK_FP64_MFMA, 5), "traffic": traffic, "kernel": "k_sigma+k_wave+k_small",
                 "kernel_ms_per_step": round(kern_ms[0] / args.steps, 4),
                 "dispatches_per_step": disp / args.steps,
                 "kernel_ms_per_step_all": {"fused": round(kern_ms[0] / args.steps, 4),
