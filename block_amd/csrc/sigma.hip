@@ -2205,30 +2205,63 @@ int64_t bsg_plan_partial_doubles(const bsg_plan* p) { return p ? p->part_doubles
 }  // extern "C"
 
 namespace {
+// open-addressing tables (no allocation per entry), grown by doubling at
+// half load so that their size follows the distinct blocks, not the pairs:
+// keys of the operator blocks (offset, orientation, rows, cols) and of the
+// C blocks (offset)
+struct SliceTable {
+  struct E {
+    int64_t off, key, val;  // off < 0: empty
+  };
+  std::vector<E> e;
+  size_t n = 0;
+  explicit SliceTable(size_t hint) {  // hint: entries expected at most
+    size_t cap = 4096;
+    while (cap < hint) cap <<= 1;
+    e.assign(cap, E{-1, 0, 0});
+  }
+  static size_t hash(int64_t off, int64_t key) {
+    uint64_t h = (uint64_t)off * 0x9E3779B97F4A7C15ull ^ (uint64_t)key * 0xC2B2AE3D27D4EB4Full;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    return (size_t)(h ^ (h >> 29));
+  }
+  E& slot(int64_t off, int64_t key) {
+    const size_t mask = e.size() - 1;
+    for (size_t h = hash(off, key) & mask;; h = (h + 1) & mask)
+      if (e[h].off < 0 || (e[h].off == off && e[h].key == key)) return e[h];
+  }
+  // the value of (off, key), made by make() on first sight
+  template <class F>
+  int64_t get(int64_t off, int64_t key, F&& make) {
+    if (2 * (n + 1) > e.size()) {
+      std::vector<E> old(2 * e.size(), E{-1, 0, 0});
+      old.swap(e);
+      for (const E& x : old)
+        if (x.off >= 0) slot(x.off, x.key) = x;
+    }
+    E& x = slot(off, key);
+    if (x.off < 0) {
+      x = E{off, key, make()};
+      ++n;
+    }
+    return x.val;
+  }
+};
 // BF16 slice recipes of a plan, one per distinct (operator block,
 // orientation) and per C block its grouped-kernel pairs read, and the pairs'
 // slice offsets; made on the first split-precision set-up, so FP64-only
 // plans (the rotations, FP64 solves) never pay for them.
 int build_slices(bsg_plan* p) {
-  // open-addressing tables (no allocation per entry): keys of the operator
-  // blocks (offset, orientation, rows, cols) and of the C blocks (offset)
-  size_t nslow = 0;  // pairs of the grouped kernels (the only ones with slices)
+  // the operator table sized by its references (most operator blocks of a
+  // superblock are referenced once) and its item list reserved likewise;
+  // the C blocks are few (one per wavefunction block)
+  size_t n_ab = 0;
   for (auto& pc : p->chunks)
-    for (int v : pc.h_vbi) nslow += v >= 0;
-  size_t cap = 64;
-  while (cap < 4 * nslow + 64) cap <<= 1;
-  struct Ent {
-    int64_t off = -1;
-    int32_t trans = 0, rows = 0, cols = 0;
-    int64_t val = 0;
-  };
-  std::vector<Ent> abt(cap), ct(cap);
-  auto hash = [](int64_t off, int t, int r, int c) {
-    uint64_t h = (uint64_t)off * 0x9E3779B97F4A7C15ull ^ ((uint64_t)t << 61) ^ ((uint64_t)r << 30) ^ (uint64_t)c;
-    h ^= h >> 31;
-    h *= 0xBF58476D1CE4E5B9ull;
-    return (size_t)(h ^ (h >> 29));
-  };
+    for (size_t j = 0; j < pc.h_perm.size(); ++j)
+      if (pc.h_vbi[j] >= 0) n_ab += (pc.h_perm[j].a_off >= 0) + (pc.h_perm[j].b_off >= 0);
+  SliceTable abt(n_ab), ct(0);
+  p->ab_items.reserve(n_ab);
   auto slice_of = [&](std::vector<SliceItem>& items, int64_t& elems, int64_t src, int rows, int cols, int trans) {
     SliceItem it;
     it.src = src;
@@ -2242,29 +2275,12 @@ int build_slices(bsg_plan* p) {
     return it.dst;
   };
   auto ab_slice = [&](int64_t off, int trans, int rows, int cols) {
-    for (size_t h = hash(off, trans, rows, cols) & (cap - 1);; h = (h + 1) & (cap - 1)) {
-      Ent& e = abt[h];
-      if (e.off < 0) {
-        e.off = off;
-        e.trans = trans;
-        e.rows = rows;
-        e.cols = cols;
-        e.val = slice_of(p->ab_items, p->ab_elems, off, rows, cols, trans);
-        return e.val;
-      }
-      if (e.off == off && e.trans == trans && e.rows == rows && e.cols == cols) return e.val;
-    }
+    const int64_t key = ((int64_t)rows << 33) | ((int64_t)cols << 1) | trans;
+    return abt.get(off, key, [&] { return slice_of(p->ab_items, p->ab_elems, off, rows, cols, trans); });
   };
   auto c_slice = [&](int64_t off, int k1, int k2) {
-    for (size_t h = hash(off, 0, 0, 0) & (cap - 1);; h = (h + 1) & (cap - 1)) {
-      Ent& e = ct[h];
-      if (e.off < 0) {
-        e.off = off;
-        e.val = slice_of(p->c_items, p->c_elems, off, k2, k1, 1);  // C^T: (j, p) = C[p k2 + j]
-        return e.val;
-      }
-      if (e.off == off) return e.val;
-    }
+    // C^T: (j, p) = C[p k2 + j]
+    return ct.get(off, 0, [&] { return slice_of(p->c_items, p->c_elems, off, k2, k1, 1); });
   };
   for (auto& pc : p->chunks) {
     std::vector<PairS> ps(pc.h_perm.size(), PairS{-1, -1, -1});
