@@ -991,10 +991,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SB == 1 ? 4
 // streams in flight.  A pair's op(A) row panel (16 rows x k1, k1 <= 4 WKA)
 // sits in registers; W^T = C^T op(A)^T is formed 16 W columns at a time and
 // its result registers are at once GEMM 2's A operand against the matching
-// 16 rows of B'.  The C operands of the next 16 W columns are requested
-// before the MFMAs of the current ones (two register sets, used in turn),
-// the B' operands of a column group before its GEMM 1 MFMAs, and the next
-// pair's descriptor before this pair's operands.
+// 16 rows of B'.  On the 32-wide tiles (NB = 2) the C operands of the next
+// 16 W columns are requested before the MFMAs of the current ones (two
+// register sets, used in turn); the 64-wide tiles (NB = 4, which form each
+// W column once for twice the V columns) spend that room on B' and keep one
+// set.  The B' operands of a column group are requested before its GEMM 1
+// MFMAs, and the next pair's descriptor before this pair's operands.
 // Identity-B' pairs use the direct orientation W = op(A) C, whose result
 // layout is the V accumulators'; identity-A pairs load W = alpha C straight
 // into GEMM 2's operand layout.  Operands past a block's rows, columns or
@@ -1130,14 +1132,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int t = 0; t < NB; ++t) accV[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(wv, b[i][t], accV[t], 0, 0, 0);
       }
     };
-    double ct0[WKA], ct1[WKA];
-    load_jb(0, ct0);
-    for (int jb = 0; jb < njb; jb += 2) {
-      if (jb + 1 < njb) load_jb(jb + 1, ct1);
-      run_jb(jb, ct0);
-      if (jb + 1 >= njb) break;
-      if (jb + 2 < njb) load_jb(jb + 2, ct0);
-      run_jb(jb + 1, ct1);
+    if constexpr (NB <= 2) {
+      double ct0[WKA], ct1[WKA];
+      load_jb(0, ct0);
+      for (int jb = 0; jb < njb; jb += 2) {
+        if (jb + 1 < njb) load_jb(jb + 1, ct1);
+        run_jb(jb, ct0);
+        if (jb + 1 >= njb) break;
+        if (jb + 2 < njb) load_jb(jb + 2, ct0);
+        run_jb(jb + 1, ct1);
+      }
+    } else {  // one register set: the wider tile's B' operands take the room
+      for (int jb = 0; jb < njb; ++jb) {
+        double ct[WKA];
+        load_jb(jb, ct);
+        run_jb(jb, ct);
+      }
     }
   }
   const bool direct = sg.p_off < 0;
@@ -1277,12 +1287,13 @@ inline long small_cost(int m, int n, const bsg_pair_t& pr) {
   const long g2 = pr.b_off < 0 ? 0 : njb * 4 * nrb * nnb;
   return g1 + g2 + 24;
 }
-// Per-wave tiles: 16 x 16 NBW.  In the FP64 column of the tile table the
-// candidate index after the geometries selects them for a V block's pairs
-// within their contraction depth (k1 <= 4 WKA, or identity A); the block's
-// other pairs run on the geometry the model picks.  Without a rule, blocks
-// stay on the grouped kernel.
-constexpr int NBW = 2;
+// Per-wave tiles: 16 x 32 for V blocks up to 32 columns, 16 x 64 above
+// (wave_nb).  In the FP64 column of the tile table the candidate index after
+// the geometries selects them for a V block's pairs within their contraction
+// depth (k1 <= 4 WKA, or identity A); the block's other pairs run on the
+// geometry the model picks.  Without a rule, blocks stay on the grouped
+// kernel.
+inline int wave_nb(int n) { return n <= 32 ? 2 : 4; }
 inline int wave_candidate() {
   static const int n = (int)geom_candidates(6).size();
   return n;
@@ -1297,7 +1308,7 @@ inline bool wave_block(int m, int n) {
 // (segment cost)
 inline long wave_cost(int n, const bsg_pair_t& pr) {
   auto cdiv = [](long a, long b) { return (a + b - 1) / b; };
-  const long nnb = std::min<long>(NBW, cdiv(n, 16)), njb = cdiv(pr.k2, 16), nk1 = cdiv(pr.k1, 4);
+  const long nnb = std::min<long>(wave_nb(n), cdiv(n, 16)), njb = cdiv(pr.k2, 16), nk1 = cdiv(pr.k1, 4);
   if (pr.a_off < 0) return (pr.b_off < 0 ? 0 : njb * 4 * nnb) + 24;
   return (pr.b_off < 0 ? nnb * nk1 : njb * (nk1 + 4 * nnb)) + 24;
 }
@@ -1637,8 +1648,8 @@ struct PlanChunk {
   int n_gbf[3] = {};
   int4* d_s[2] = {nullptr, nullptr};  // FP64 small-block units, 4 per workgroup (SB = 2, SB = 1)
   int n_s[2] = {0, 0};
-  TileRef* d_w = nullptr;  // FP64 per-wave tiles, 4 per workgroup (item -1: none), a multiple of 32
-  int n_w = 0;
+  TileRef* d_w[2] = {};  // FP64 per-wave tiles (16 x 32, 16 x 64), 4 per workgroup (item -1: none), a multiple of 32
+  int n_w[2] = {};
   RedItem* d_red[2] = {nullptr, nullptr};  // after the FP64 / the split-precision kernels
   int n_red[2] = {0, 0};
 };
@@ -1833,7 +1844,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       std::vector<bsg_pair_t> perm;                   // pairs in device order (grouped per V block as given)
       std::vector<int> h_vbi;
       std::vector<std::pair<long, int>> sunits[2];  // (cost, segment) of small-block units
-      std::vector<GUnit> wunits;                    // per-wave tile units (cost per tile)
+      std::vector<GUnit> wunits[2];                 // per-wave tile units (cost per tile), 16 x 32 and 16 x 64
       std::vector<Seg> segs;
       std::vector<RedItem> reds[2];  // ordered reductions after the FP64 / the split-precision kernels
       int64_t poff = 0, arena_extent = 0;
@@ -1961,9 +1972,10 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
             for (int q = 0; q < sg.count; ++q) gu.cost += wave_cost(vb.n, perm[sg.first + q]);
             gu.seg = si;
             gu.ntm = (vb.m + 15) / 16;
-            gu.ntn = (vb.n + 16 * NBW - 1) / (16 * NBW);
+            const int nb = wave_nb(vb.n);
+            gu.ntn = (vb.n + 16 * nb - 1) / (16 * nb);
             gu.code = 0;
-            L.wunits.push_back(gu);
+            L.wunits[nb == 4].push_back(gu);
             continue;
           }
           long ucost = 0;
@@ -2038,7 +2050,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
     std::vector<GUnit> gunits[NCLS], gunits_bf[3];
     std::vector<bsg_pair_t> perm;
     std::vector<std::pair<long, int>> sunits[2];
-    std::vector<GUnit> wunits;
+    std::vector<GUnit> wunits[2];
     std::vector<Seg> segs;
     std::vector<RedItem> reds[2];
     int64_t poff = 0;
@@ -2059,10 +2071,11 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       }
       for (int k = 0; k < 2; ++k)
         for (auto x : L.sunits[k]) sunits[k].push_back({x.first, x.second + sb});
-      for (GUnit g : L.wunits) {
-        g.seg += sb;
-        wunits.push_back(g);
-      }
+      for (int k = 0; k < 2; ++k)
+        for (GUnit g : L.wunits[k]) {
+          g.seg += sb;
+          wunits[k].push_back(g);
+        }
       for (int k = 0; k < NCLS; ++k)
         for (GUnit g : L.gunits[k]) {
           g.seg += sb;
@@ -2121,14 +2134,14 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       up((void**)&pc.d_g[k], gtiles[k].data(), sizeof(TileRef) * gtiles[k].size());
       pc.n_g[k] = (int)gtiles[k].size();
     }
-    // per-wave tiles: units (costliest first) go to the least loaded of the
-    // 8 XCDs, a unit's tiles in consecutive wave slots of that XCD's
-    // workgroups (they share the unit's C and B' blocks in its L2);
+    // per-wave tiles of each width: units (costliest first) go to the least
+    // loaded of the 8 XCDs, a unit's tiles in consecutive wave slots of that
+    // XCD's workgroups (they share the unit's C and B' blocks in its L2);
     // workgroup 8 j + x is XCD x's j-th
-    {
+    for (int k = 0; k < 2; ++k) {
       std::vector<std::pair<long, int>> order;
-      for (int u = 0; u < (int)wunits.size(); ++u)
-        order.push_back({wunits[u].cost * wunits[u].ntm * wunits[u].ntn, u});
+      for (int u = 0; u < (int)wunits[k].size(); ++u)
+        order.push_back({wunits[k][u].cost * wunits[k][u].ntm * wunits[k][u].ntn, u});
       std::sort(order.begin(), order.end(), [](const std::pair<long, int>& x, const std::pair<long, int>& y) {
         return x.first != y.first ? x.first > y.first : x.second < y.second;
       });
@@ -2136,7 +2149,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       long load[8] = {};
       for (const auto& o : order) {
         const int x = (int)(std::min_element(load, load + 8) - load);
-        const GUnit& gu = wunits[o.second];
+        const GUnit& gu = wunits[k][o.second];
         for (int tm = 0; tm < gu.ntm; ++tm)
           for (int tn = 0; tn < gu.ntn; ++tn) q[x].push_back(TileRef{gu.seg, tm, tn, 0});
         load[x] += o.first;
@@ -2146,8 +2159,8 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
       std::vector<TileRef> wt(nwg * 32, TileRef{-1, 0, 0, 0});
       for (int x = 0; x < 8; ++x)
         for (size_t e = 0; e < q[x].size(); ++e) wt[((e / 4) * 8 + x) * 4 + e % 4] = q[x][e];
-      up((void**)&pc.d_w, wt.data(), sizeof(TileRef) * wt.size());
-      pc.n_w = (int)wt.size();
+      up((void**)&pc.d_w[k], wt.data(), sizeof(TileRef) * wt.size());
+      pc.n_w[k] = (int)wt.size();
     }
     for (int k = 0; k < 3; ++k) {
       up((void**)&pc.d_gbf[k], gtiles_bf[k].data(), sizeof(TileRef) * gtiles_bf[k].size());
@@ -2470,9 +2483,14 @@ int enqueue_chunk(bsg_context* ctx, const bsg_plan* p, const PlanChunk& ch, cons
       ++nl;
     }
   } else {
-    if (ch.n_w > 0) {
-      hipLaunchKernelGGL((k_wave<NBW>), grid(ch.n_w / 4), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_w, ch.n_w,
-                         arena, c, v, part, rs);
+    if (ch.n_w[1] > 0) {
+      hipLaunchKernelGGL((k_wave<4>), grid(ch.n_w[1] / 4), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_w[1],
+                         ch.n_w[1], arena, c, v, part, rs);
+      ++nl;
+    }
+    if (ch.n_w[0] > 0) {
+      hipLaunchKernelGGL((k_wave<2>), grid(ch.n_w[0] / 4), dim3(256), 0, st, ch.d_vb, ch.d_pr, ch.d_seg, ch.d_w[0],
+                         ch.n_w[0], arena, c, v, part, rs);
       ++nl;
     }
     if (ch.n_g[3] > 0) {
