@@ -284,13 +284,15 @@ def test_wave_tiles(name, make):
     c = np.stack([rng_vec(sb.wave_size, 50 + i) for i in range(4)])
     ref = _oracle.sigma(sb, c[0], nthreads=8)
     h = block_amd.multiply_h(sb, max_vec=4)
-    grouped = h(c[0])
-    h.precision = block_amd.SPLIT_BF16
-    split = h(c[0])
-    h.precision = block_amd.FP64
     wave = block_amd.tile_candidates(block_amd.FP64) - 1
-    block_amd.set_tile_table([(1, 1 << 20, 1, 1 << 20, wave, -1)])
     try:
+        block_amd.set_tile_table([])  # the grouped kernel for every block above 32 x 32
+        h.replan()
+        grouped = h(c[0])
+        h.precision = block_amd.SPLIT_BF16
+        split = h(c[0])
+        h.precision = block_amd.FP64
+        block_amd.set_tile_table([(1, 1 << 20, 1, 1 << 20, wave, -1)])
         h.replan()
         got = h(c[0])
         rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
