@@ -988,7 +988,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SB == 1 ? 4
 // wavefront owns one tile of a V block (16 rows x 16 NB columns) for one work
 // unit and walks the unit's pairs alone -- no LDS, no barriers, four
 // unrelated tiles per workgroup -- so a CU keeps four times as many operand
-// streams in flight.  A pair's op(A) row panel (16 rows x k1, k1 <= 4 WKA)
+// streams in flight.  A pair's op(A) row panel (16 rows x k1, k1 <= 4 KA)
 // sits in registers; W^T = C^T op(A)^T is formed 16 W columns at a time and
 // its result registers are at once GEMM 2's A operand against the matching
 // 16 rows of B'.  On the 32-wide tiles (NB = 2) the C operands of the next
@@ -1002,12 +1002,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SB == 1 ? 4
 // into GEMM 2's operand layout.  Operands past a block's rows, columns or
 // contraction depth read as zero (buffer range checks and masked lanes), and
 // every C column past k2 is masked, so B' rows past k2 only meet zero W.
-constexpr int WKA = 16;  // 4-deep K steps of op(A) held in registers (k1 <= 64)
+// 4-deep K steps of op(A) held in registers: k1 <= 64 on the 32-wide tiles,
+// k1 <= 80 on the 64-wide ones (their single C register set leaves room)
+constexpr int wave_ka(int nb) { return nb == 4 ? 20 : 16; }
 template <int NB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_wave(
     const bsg_vblock_t* __restrict__ vbs, const bsg_pair_t* __restrict__ prs, const Seg* __restrict__ segs,
     const TileRef* __restrict__ tiles, int ntiles, const double* __restrict__ arena, const double* __restrict__ c,
     double* __restrict__ v, double* __restrict__ part, RootSet rs) {
+  constexpr int KA = wave_ka(NB);
   int g;
   const int root = root_of(blockIdx.x, rs.n, g);
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1044,13 +1047,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       continue;
     }
     const int nk1 = (pr.k1 + 3) / 4;
-    double a[WKA];  // a[s] = op(A)(row0 + li, 4 s + lk)
+    double a[KA];  // a[s] = op(A)(row0 + li, 4 s + lk)
     if (pr.a_off >= 0) {
       const __amdgpu_buffer_rsrc_t rA = block_rsrc(arena + pr.a_off, (int64_t)vb.m * pr.k1);
       const int ar = pr.a_trans ? 1 : pr.k1, ak = pr.a_trans ? vb.m : 1;
       const uint32_t off = rowok ? (uint32_t)((row0 + li) * ar + lk * ak) * 8u : OOB;
 #pragma unroll
-      for (int s = 0; s < WKA; ++s) a[s] = bload2(rA, s < nk1 ? off : OOB, (uint32_t)(s * ak) * 32u);
+      for (int s = 0; s < KA; ++s) a[s] = bload2(rA, s < nk1 ? off : OOB, (uint32_t)(s * ak) * 32u);
     }
     const double aw = pr.alpha_a * pr.alpha;
     if (pr.b_off < 0) {
@@ -1060,12 +1063,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const int col = col0 + t * 16 + li;
         if (col0 + t * 16 >= vb.n) break;
         const uint32_t coff = col < pr.k2 ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
-        double ct[WKA];
+        double ct[KA];
 #pragma unroll
-        for (int s = 0; s < WKA; ++s) ct[s] = bload2(rC, s < nk1 ? coff : OOB, (uint32_t)(4 * s) * crow);
+        for (int s = 0; s < KA; ++s) ct[s] = bload2(rC, s < nk1 ? coff : OOB, (uint32_t)(4 * s) * crow);
         dbl4 accW = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int s4 = 0; s4 < WKA; s4 += 4)
+        for (int s4 = 0; s4 < KA; s4 += 4)
           if (s4 < nk1)
 #pragma unroll
             for (int s = s4; s < s4 + 4; ++s) accW = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], ct[s], accW, 0, 0, 0);
@@ -1106,13 +1109,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     // W^T 16 columns at a time: ct[s] = C(4 s + lk, 16 jb + li) (columns past
     // k2 masked), b[i][t] = B'(16 jb + 4 i + lk, col0 + 16 t + li); the B'
     // operands of a column group are requested before its GEMM 1 MFMAs
-    auto load_jb = [&](int jb, double (&ct)[WKA]) {
+    auto load_jb = [&](int jb, double (&ct)[KA]) {
       const int col = jb * 16 + li;
       const uint32_t coff = col < pr.k2 ? (uint32_t)(lk * pr.k2 + col) * 8u : OOB;
 #pragma unroll
-      for (int s = 0; s < WKA; ++s) ct[s] = bload2(rC, s < nk1 ? coff : OOB, (uint32_t)(4 * s) * crow);
+      for (int s = 0; s < KA; ++s) ct[s] = bload2(rC, s < nk1 ? coff : OOB, (uint32_t)(4 * s) * crow);
     };
-    auto run_jb = [&](int jb, const double (&ct)[WKA]) {
+    auto run_jb = [&](int jb, const double (&ct)[KA]) {
       double b[4][NB];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
@@ -1121,7 +1124,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       // register i of lane (li, lk): W(row0 + li, 16 jb + lk + 4 i)
       dbl4 accT = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int s4 = 0; s4 < WKA; s4 += 4)
+      for (int s4 = 0; s4 < KA; s4 += 4)
         if (s4 < nk1)
 #pragma unroll
           for (int s = s4; s < s4 + 4; ++s) accT = __builtin_amdgcn_mfma_f64_16x16x4f64(ct[s], a[s], accT, 0, 0, 0);
@@ -1133,7 +1136,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       }
     };
     if constexpr (NB <= 2) {
-      double ct0[WKA], ct1[WKA];
+      double ct0[KA], ct1[KA];
       load_jb(0, ct0);
       for (int jb = 0; jb < njb; jb += 2) {
         if (jb + 1 < njb) load_jb(jb + 1, ct1);
@@ -1144,7 +1147,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       }
     } else {  // one register set: the wider tile's B' operands take the room
       for (int jb = 0; jb < njb; ++jb) {
-        double ct[WKA];
+        double ct[KA];
         load_jb(jb, ct);
         run_jb(jb, ct);
       }
@@ -1290,7 +1293,7 @@ inline long small_cost(int m, int n, const bsg_pair_t& pr) {
 // Per-wave tiles: 16 x 32 for V blocks up to 32 columns, 16 x 64 above
 // (wave_nb).  In the FP64 column of the tile table the candidate index after
 // the geometries selects them for a V block's pairs within their contraction
-// depth (k1 <= 4 WKA, or identity A); the block's other pairs run on the
+// depth (k1 <= 4 wave_ka, or identity A); the block's other pairs run on the
 // geometry the model picks.  Without a rule, blocks stay on the grouped
 // kernel.
 inline int wave_nb(int n) { return n <= 32 ? 2 : 4; }
@@ -1899,7 +1902,7 @@ int bsg_plan_create(bsg_context* ctx, const bsg_chunk_t* chunks, int32_t n_chunk
           L.flops2 += 2.0 * vb.m * pr.k2 * vb.n;
         }
         const bool f = small_vb ? (pr.a_off >= 0 && pr.k1 <= 32 && pr.k2 <= 32) || (pr.a_off < 0 && pr.b_off < 0)
-                                : wave_vb && (pr.a_off < 0 || pr.k1 <= 4 * WKA);
+                                : wave_vb && (pr.a_off < 0 || pr.k1 <= 4 * wave_ka(wave_nb(vb.n)));
         (f ? fast : slow).push_back(vb.first + q);
         if (!f || wave_vb)  // split mode: every pair of a per-wave block runs on BF16 slices
           L.flops_split += (pr.a_off >= 0 ? 2.0 * vb.m * pr.k1 * pr.k2 : 0.0) +

@@ -144,7 +144,8 @@ int bsg_plan_split_arena(bsg_context* ctx, bsg_plan* plan, const double* arena, 
  * length for a precision.  For FP64 its last entry is not a geometry but the
  * per-wave tile kernel (one wavefront per 16 x 32 V tile, 16 x 64 for blocks
  * wider than 32 columns; no LDS or barriers)
- * for a block's pairs with k1 <= 64 or an identity A; the block's other pairs
+ * for a block's pairs with k1 <= 64 (80 on the 64-wide tiles) or an identity
+ * A; the block's other pairs
  * keep the model's geometry, and split mode keeps the grouped BF16 kernel
  * for the whole block.  Blocks of at most 32 x 32 stay on the small-block
  * kernel whatever the rule.  The tuner that measures them is bench.py --tune;
