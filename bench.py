@@ -345,6 +345,29 @@ def fetch_val(d, name):
 TUNE_EDGES = [(1, 32), (33, 64), (65, 128), (129, 256), (257, 512), (513, 1 << 30)]
 
 
+def tuner_start(buckets, start):
+    """The tuner's first choices: (fp64, split) per bucket (row, column edge
+    indices into TUNE_EDGES) from the start rules whose ranges are exactly a
+    bucket's, -1 elsewhere; the start rules of other ranges are kept as
+    they are."""
+    choice = {b: [-1, -1] for b in buckets}
+    edges = {(TUNE_EDGES[bm][0], TUNE_EDGES[bm][1], TUNE_EDGES[bn][0], TUNE_EDGES[bn][1]): (bm, bn) for bm, bn in buckets}
+    keep = []
+    for r in start:
+        if tuple(r[:4]) in edges:
+            choice[edges[tuple(r[:4])]] = [r[4], r[5]]
+        else:
+            keep.append(tuple(r))
+    return choice, keep
+
+
+def tuner_rules(choice, keep):
+    """Tile table rules of the tuner's current choices (buckets on the model
+    in both precisions give no rule), then the kept rules."""
+    return [(TUNE_EDGES[bm][0], TUNE_EDGES[bm][1], TUNE_EDGES[bn][0], TUNE_EDGES[bn][1], f, sp)
+            for (bm, bn), (f, sp) in choice.items() if f >= 0 or sp >= 0] + list(keep)
+
+
 def tune_tiles(sbs, out_path, steps, warmup, label, top=8, min_gain=0.005, start=(), passes=1):
     """Measured tile table over one or more superblocks: the V blocks are
     bucketed by (rows, cols) ranges; for the buckets with most of the flops
@@ -377,18 +400,10 @@ def tune_tiles(sbs, out_path, steps, warmup, label, top=8, min_gain=0.005, start
     for h in hs:
         h.replan()
     v0 = [h(c) for h, c in zip(hs, cs)]
-    choice = {b: [-1, -1] for b in buckets}
-    edges = {(TUNE_EDGES[bm][0], TUNE_EDGES[bm][1], TUNE_EDGES[bn][0], TUNE_EDGES[bn][1]): (bm, bn) for bm, bn in buckets}
-    keep = []  # start rules of ranges that are not tuned buckets
-    for r in start:
-        if tuple(r[:4]) in edges:
-            choice[edges[tuple(r[:4])]] = [r[4], r[5]]
-        else:
-            keep.append(tuple(r))
+    choice, keep = tuner_start(buckets, start)
 
     def rules():
-        return [(TUNE_EDGES[bm][0], TUNE_EDGES[bm][1], TUNE_EDGES[bn][0], TUNE_EDGES[bn][1], f, sp)
-                for (bm, bn), (f, sp) in choice.items() if f >= 0 or sp >= 0] + keep
+        return tuner_rules(choice, keep)
 
     def times():
         block_amd.set_tile_table(rules())
